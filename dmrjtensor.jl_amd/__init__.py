@@ -1,0 +1,24 @@
+"""dmrjtensor.jl_amd: MI355X-native two-site DMRG hot path.
+
+Host-side mirror (Python) of the reference's API for the two-site DMRG path
+(bakerte/DMRJtensor.jl: algs/DMRG.jl, algs/optimizeMPS.jl, lib/MPO_MPS,
+lib/Env, lib/models.jl).  All arithmetic runs in libdmrjhip (HIP/CDNA4
+kernels behind include/dmrj.h); this package only plans and moves handles.
+The directory name contains a dot, so it is loaded with ``load_package()``
+from ``build.py`` or imported as ``dmrjtensor_jl_amd`` after registration.
+"""
+from . import _lib
+from ._lib import Context, DMRJError, default_context
+from .models import fermionOps, heisenbergMPO, hubbardMPO, j1j2_bulk, spinOps, XXZ
+from .network import (MPO, MPS, DeviceTensor, Lupdate, Rupdate, environment, expect, makeEnv,
+                      makeMPO, makeq, mpo_deltas, product_state, randMPS_normal, site2_from_dense,
+                      site_from_dense)
+from .dmrg import algvars, dmrg, optmps, singlesite_update, startval, twostep
+
+__all__ = [
+    "Context", "DMRJError", "default_context", "fermionOps", "heisenbergMPO", "hubbardMPO",
+    "j1j2_bulk", "spinOps", "XXZ", "MPO", "MPS", "DeviceTensor", "Lupdate", "Rupdate",
+    "environment", "expect", "makeEnv", "makeMPO", "makeq", "mpo_deltas", "product_state",
+    "randMPS_normal", "site2_from_dense", "site_from_dense", "algvars", "dmrg", "optmps",
+    "singlesite_update", "startval", "twostep",
+]

@@ -1,0 +1,159 @@
+"""ctypes binding of libdmrjhip (include/dmrj.h).
+
+The shared library is built in-tree (``lib/libdmrjhip.so``) by
+``build.py``; loading fails loudly when it is missing.  There is no CPU
+fallback: every numerical operation of the package runs through this library
+on the GPU.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libdmrjhip.so")
+
+PREC = {"f64": 0, "bf16x3": 1, "f16s": 2}
+
+_p = C.c_void_p
+_i64 = C.c_int64
+_i32p = C.POINTER(C.c_int32)
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+
+# name -> argtypes (all functions return int, except dmrj_last_error)
+SIGNATURES = {
+    "dmrj_version": [],
+    "dmrj_ctx_create": [C.c_int, C.POINTER(_p)],
+    "dmrj_ctx_destroy": [_p],
+    "dmrj_ctx_sync": [_p],
+    "dmrj_ctx_memstats": [_p, _i64p],
+    "dmrj_ctx_set_gemm_cfg": [_p, C.c_int],
+    "dmrj_ctx_flops": [_p, _dp, C.c_int],
+    "dmrj_site_from_dense": [_p, C.c_int, _i64, _i64, _i64, _dp, _i32p, _i32p, _i32p, C.POINTER(_p)],
+    "dmrj_site2_from_dense": [_p, C.c_int, _i64, _i64, _i64, _i64, _dp, _i32p, _i32p, _i32p, _i32p,
+                              C.POINTER(_p)],
+    "dmrj_tensor_info": [_p, _i64p],
+    "dmrj_tensor_to_dense": [_p, _p, _dp, _i32p, _i32p],
+    "dmrj_tensor_free": [_p, _p],
+    "dmrj_tensor_copy": [_p, _p, C.POINTER(_p)],
+    "dmrj_mpo_create": [_p, C.c_int, _i64, _i64, _i64, _dp, _i32p, _i32p, _i32p, C.POINTER(_p)],
+    "dmrj_mpo_free": [_p, _p],
+    "dmrj_env_boundary": [_p, C.c_int, C.c_int, _i32p, C.POINTER(_p)],
+    "dmrj_env_left": [_p, _p, _p, _p, C.POINTER(_p)],
+    "dmrj_env_right": [_p, _p, _p, _p, C.POINTER(_p)],
+    "dmrj_theta2": [_p, _p, _p, C.POINTER(_p)],
+    "dmrj_heff2_apply": [_p, _p, _p, _p, _p, _p, C.c_int, C.POINTER(_p)],
+    "dmrj_heff1_apply": [_p, _p, _p, _p, _p, C.c_int, C.POINTER(_p)],
+    "dmrj_lanczos2": [_p, _p, _p, _p, _p, _p, _p, C.c_int, C.POINTER(_p), _dp],
+    "dmrj_svd_trunc2": [_p, _p, _i64, _i64, C.c_double, C.c_int, C.POINTER(_p), C.POINTER(_p), _dp,
+                        _i64p, _dp, _i64],
+    "dmrj_twostep": [_p, _p, _p, _p, _p, _p, _p, C.c_int, _i64, _i64, C.c_double, C.c_int,
+                     C.POINTER(_p), C.POINTER(_p), _dp, _dp, _i64p, _dp, _i64],
+    "dmrj_dot": [_p, _p, _p, _dp],
+    "dmrj_norm": [_p, _p, _dp],
+    "dmrj_scale": [_p, _p, C.c_double],
+    "dmrj_axpy": [_p, C.c_double, _p, _p],
+    "dmrj_normalize": [_p, _p],
+    "dmrj_selftest_gemm": [_p, C.c_int, _i64, _i64, _i64, C.c_int, C.c_int, _dp, _dp, _dp],
+    "dmrj_selftest_mfma_f64": [_p, _dp],
+}
+
+_lib = None
+
+
+class DMRJError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libdmrjhip.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise DMRJError("libdmrjhip.so is not built (%s); run build()" % LIB_PATH)
+    lib = C.CDLL(LIB_PATH)
+    for name, args in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    lib.dmrj_last_error.argtypes = [_p]
+    lib.dmrj_last_error.restype = C.c_char_p
+    _lib = lib
+    return lib
+
+
+def dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_i32p)
+
+
+def charges_arg(q, n, nq):
+    """Charges as a contiguous int32 [n][nq] array (or None for nq == 0)."""
+    if nq == 0 or q is None:
+        return None
+    a = np.ascontiguousarray(np.asarray(q, dtype=np.int32).reshape(n, nq))
+    return a
+
+
+class Context:
+    """One device context = one HIP stream (dmrj_ctx)."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = _p()
+        rc = self.lib.dmrj_ctx_create(device, C.byref(h))
+        if rc != 0:
+            raise DMRJError("dmrj_ctx_create failed (%d): no usable GPU" % rc)
+        self.h = h
+
+    def check(self, rc, what=""):
+        if rc != 0:
+            msg = self.lib.dmrj_last_error(self.h)
+            raise DMRJError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    def call(self, name, *args):
+        self.check(getattr(self.lib, name)(self.h, *args), name)
+
+    def sync(self):
+        self.call("dmrj_ctx_sync")
+
+    def memstats(self):
+        out = (C.c_int64 * 3)()
+        self.call("dmrj_ctx_memstats", out)
+        return dict(live=out[0], cached=out[1], peak=out[2])
+
+    def flops(self, reset=False):
+        out = (C.c_double * 2)()
+        self.call("dmrj_ctx_flops", out, 1 if reset else 0)
+        return out[0], out[1]
+
+    def set_gemm_cfg(self, cfg):
+        self.call("dmrj_ctx_set_gemm_cfg", int(cfg))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dmrj_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default = None
+
+
+def default_context():
+    global _default
+    if _default is None:
+        _default = Context(0)
+    return _default

@@ -1,0 +1,245 @@
+// Internal types of libdmrjhip: graded bond spaces, block-sparse tensor
+// metadata, contraction plans and the per-context device runtime.
+//
+// Block-sparse layout (shared by dense and quantum-number runs; a dense bond
+// is a space with one sector of charge 0):
+//   * site tensor A[alpha, s, beta] with P physical states: one block per
+//     (left sector a, physical state s) whose right sector b has charge
+//     q(a) + q(s).  Blocks are stored as column-major n_a x n_b matrices,
+//     grouped in "row panels": for fixed a all s-blocks are adjacent, so the
+//     panel is itself an n_a x C_a column-major matrix (ld = n_a).
+//   * environment E[bra, w, ket] (left) or E[bra, u, ket] (right, stored
+//     transposed with respect to the reference's Renv[ket, u, bra]): one
+//     block per (bra sector a', MPO channel w) whose ket sector a has charge
+//     q(a') - delta(w); blocks of one bra sector are adjacent ("bra panel",
+//     an n_a' x sum_w n_a matrix, ld = n_a').
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace dmrj {
+
+constexpr int kMaxQ = 2;
+
+struct Charge {
+  int32_t v[kMaxQ] = {0, 0};
+  bool operator<(const Charge& o) const {
+    for (int i = 0; i < kMaxQ; ++i)
+      if (v[i] != o.v[i]) return v[i] < o.v[i];
+    return false;
+  }
+  bool operator==(const Charge& o) const {
+    for (int i = 0; i < kMaxQ; ++i)
+      if (v[i] != o.v[i]) return false;
+    return true;
+  }
+  bool operator!=(const Charge& o) const { return !(*this == o); }
+  Charge operator+(const Charge& o) const {
+    Charge r;
+    for (int i = 0; i < kMaxQ; ++i) r.v[i] = v[i] + o.v[i];
+    return r;
+  }
+  Charge operator-(const Charge& o) const {
+    Charge r;
+    for (int i = 0; i < kMaxQ; ++i) r.v[i] = v[i] - o.v[i];
+    return r;
+  }
+};
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// Graded vector space of a bond: sectors sorted by charge.
+struct Space {
+  std::vector<Charge> q;
+  std::vector<int> dim;
+  std::vector<int64_t> off;  // offset of the sector in the sorted basis
+  int64_t total = 0;
+  int nsec() const { return (int)q.size(); }
+  int find(const Charge& c) const;
+  void finish();  // fills off/total
+  bool same(const Space& o) const;
+};
+
+struct SiteBlock {
+  int a, s, b;
+  int64_t off;      // element offset in the tensor buffer
+  int64_t colpos;   // first column of the block inside its row panel
+};
+
+struct SiteMeta {
+  Space left, right;
+  int P = 0;
+  std::vector<Charge> qp;          // charge of each physical state
+  int d1 = 0, d2 = 0;              // two-site tensor: P = d1*d2, s = j1 + d1*j2
+  std::vector<Charge> qp1, qp2;
+  std::vector<SiteBlock> blocks;   // ordered by (a, s)
+  std::vector<int> bidx;           // a*P + s -> block index or -1
+  std::vector<int64_t> panel_off;  // per left sector
+  std::vector<int64_t> panel_cols; // per left sector
+  int64_t size = 0;
+  void build();
+  int block(int a, int s) const { return bidx[(size_t)a * P + s]; }
+};
+
+struct EnvBlock {
+  int ap, w, a;     // bra sector, channel, ket sector
+  int64_t off;
+  int64_t colpos;   // column offset inside the bra panel
+};
+
+struct EnvMeta {
+  Space bra, ket;
+  int D = 0;
+  std::vector<Charge> delta;       // per channel
+  std::vector<EnvBlock> blocks;    // ordered by (ap, w)
+  std::vector<int> bidx;           // ap*D + w -> block index or -1
+  std::vector<int64_t> panel_off;  // per bra sector
+  std::vector<int64_t> panel_cols; // per bra sector
+  int64_t size = 0;
+  void build();
+  int block(int ap, int w) const { return bidx[(size_t)ap * D + w]; }
+};
+
+// Grouped GEMM descriptors (device side).  C[MxN] = beta*C + sum_seg alpha *
+// op(A_seg)[MxK] * op(B_seg)[KxN]; flags bit0: A stored KxM (transposed),
+// bit1: B stored NxK (transposed).
+struct GemmSeg {
+  const double* A;
+  const double* B;
+  int64_t lda, ldb;
+  int K;
+  int flags;
+  double alpha;
+};
+struct GemmTask {
+  double* C;
+  int64_t ldc;
+  int M, N;
+  int seg0, nseg;
+  double beta;
+};
+struct GemmTile {
+  int task, m0, n0, cfg;
+};
+
+struct MixTerm {
+  const double* src;
+  double coef;
+};
+struct MixDst {
+  double* dst;
+  int64_t n;
+  int t0, nt;
+};
+struct MixWork {
+  int dst;
+  int pad;
+  int64_t e0;
+};
+
+struct CopyOp {
+  const double* src;   // null -> write zeros
+  double* dst;
+  int64_t lds, ldd;
+  int rows, cols;
+  const double* rscale;  // optional per-row factor
+  const double* cscale;  // optional per-column factor
+  double alpha;
+};
+struct CopyWork {
+  int op;
+  int pad;
+  int64_t e0;
+};
+
+// ---- runtime ----
+struct Allocator {
+  std::multimap<size_t, void*> free_;
+  std::map<void*, size_t> live_;
+  size_t bytes_live = 0, bytes_cached = 0, peak = 0;
+  void* alloc(size_t bytes);
+  void release(void* p);
+  void trim();
+  ~Allocator();
+};
+
+struct HostStage {   // pinned staging memory for plan uploads
+  char* base = nullptr;
+  size_t cap = 0, used = 0;
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  void* rocblas = nullptr;   // rocblas_handle
+  Allocator alloc;
+  HostStage stage;
+  std::vector<void*> transient;  // device copies of plan tables, freed at sync points
+  std::string err;
+  double* scal = nullptr;       // device scalars for Lanczos
+  double* hscal = nullptr;      // pinned mirror
+  double* partial = nullptr;    // reduction partials
+  int num_cu = 256;
+  // tuning: preferred GEMM tile config per operation (-1 = automatic)
+  int gemm_cfg_override = -1;
+  int last_gemm_cfg = -1;
+  // statistics
+  double flops_heff = 0.0, flops_env = 0.0;
+  void* upload(const void* host, size_t bytes);   // staged async copy, returns device ptr
+  void stage_reset();
+  void release_transients();
+};
+
+struct MpoSite {
+  int DL = 0, d = 0, DR = 0;
+  std::vector<double> W;  // column-major W[w + DL*(j + d*(k + d*u))]
+  std::vector<Charge> qp, dL, dR;
+  double at(int w, int j, int k, int u) const {
+    return W[(size_t)w + (size_t)DL * ((size_t)j + (size_t)d * ((size_t)k + (size_t)d * u))];
+  }
+};
+
+struct Tensor {
+  int kind = 0;  // 0 = site tensor, 1 = left env, 2 = right env
+  int nq = 0;
+  SiteMeta site;
+  EnvMeta env;
+  double* data = nullptr;
+  int64_t nelem() const { return kind == 0 ? site.size : env.size; }
+};
+
+// ---- kernel launchers (kernels.hip) ----
+void launch_gemm(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs,
+                 int cfg);
+void launch_mix(Context& c, const std::vector<MixDst>& dsts, const std::vector<MixTerm>& terms);
+void launch_copy(Context& c, const std::vector<CopyOp>& ops);
+void launch_fill(Context& c, double* p, int64_t n, double v);
+void launch_dots(Context& c, int npairs, const double* const* xs, const double* const* ys,
+                 int64_t n, double* out);
+void launch_axpy_dev(Context& c, double* y, const double* x, int64_t n, const double* coef);
+void launch_scale_dev(Context& c, double* x, int64_t n, const double* coef);
+void launch_comb_dev(Context& c, double* out, const double* a, const double* b, int64_t n,
+                     const double* ca, const double* cb);
+void launch_lz_scalar(Context& c, int stage);
+void launch_mfma_probe(Context& c, double* out);
+int choose_gemm_cfg(const std::vector<GemmTask>& tasks, int num_cu);
+double gemm_flops(const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs);
+
+// Lanczos scalar slots in Context::scal
+enum Slot {
+  S_TT = 0, S_TH = 1, S_PP = 2, S_PHP = 3, S_OC = 4,
+  S_NRM = 8, S_A1 = 9, S_CAXPY = 10, S_B1 = 11, S_INVB = 12, S_FLAG = 13,
+  S_A2 = 14, S_E = 15, S_CA = 16, S_CB = 17, S_INVOC = 18, S_CHK = 19,
+  S_COUNT = 32
+};
+
+}  // namespace dmrj

@@ -1,0 +1,491 @@
+// Device kernels of libdmrjhip (gfx950 / CDNA4).
+//
+//  * k_gemm_f64<BM,BN>: persistent grouped GEMM over a flat tile list.  Every
+//    task is C = beta*C + sum over K-segments of alpha*op(A)*op(B) with its own
+//    pointers, leading dimensions and transpose flags, so one launch covers all
+//    symmetry blocks of one contraction step.  FP64 MFMA
+//    v_mfma_f64_16x16x4f64, 256 threads = 4 waves in a 2x2 arrangement, LDS
+//    double buffer with register prefetch (one barrier per K step).
+//  * k_mix: small dense "label" contraction with the MPO coefficients
+//    (HBM-bound weighted sums of equally shaped blocks).
+//  * k_copy: batched strided block copy with optional row/column scaling
+//    (SVD sector gather and split).
+//  * deterministic two-pass dot products and the scalar steps of the
+//    two-vector Lanczos, all reading and writing device scalars so that a
+//    Lanczos application needs no host round trip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "internal.h"
+
+namespace dmrj {
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+#define HIP_OK(x)                                                                    \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess) throw Error(-3, std::string("HIP: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int kThreads = 256;
+constexpr int kBK = 16;
+constexpr int kPad = 16;  // row pitch = BM+16 doubles: the two 16-lane halves of a
+                          // ds_read_b64 wave access land on disjoint bank halves
+
+template <int BM, int BN>
+__global__ __launch_bounds__(kThreads) void k_gemm_f64(const GemmTile* __restrict__ tiles,
+                                                       int ntiles,
+                                                       const GemmTask* __restrict__ tasks,
+                                                       const GemmSeg* __restrict__ segs) {
+  constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
+  constexpr int AE = BM * kBK / kThreads, BE = BN * kBK / kThreads;
+  __shared__ double sA[2][kBK][BM + kPad];
+  __shared__ double sB[2][kBK][BN + kPad];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int lr = lane & 15, lk = lane >> 4;
+  int buf = 0;
+
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const GemmTile tl = tiles[t];
+    const GemmTask tk = tasks[tl.task];
+    const int m0 = tl.m0, n0 = tl.n0;
+    double4_t acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+
+    for (int sidx = 0; sidx < tk.nseg; ++sidx) {
+      const GemmSeg sg = segs[tk.seg0 + sidx];
+      const bool tA = sg.flags & 1, tB = sg.flags & 2;
+      const int K = sg.K;
+      double ra[AE], rb[BE];
+      auto load = [&](int k0) {
+#pragma unroll
+        for (int r = 0; r < AE; ++r) {
+          const int idx = tid + r * kThreads;
+          int m, k;
+          if (!tA) { m = idx % BM; k = idx / BM; } else { k = idx % kBK; m = idx / kBK; }
+          const int gm = m0 + m, gk = k0 + k;
+          double v = 0.0;
+          if (gm < tk.M && gk < K)
+            v = tA ? sg.A[(int64_t)gk + (int64_t)gm * sg.lda] : sg.A[(int64_t)gm + (int64_t)gk * sg.lda];
+          ra[r] = v * sg.alpha;
+        }
+#pragma unroll
+        for (int r = 0; r < BE; ++r) {
+          const int idx = tid + r * kThreads;
+          int n, k;
+          if (!tB) { k = idx % kBK; n = idx / kBK; } else { n = idx % BN; k = idx / BN; }
+          const int gn = n0 + n, gk = k0 + k;
+          double v = 0.0;
+          if (gn < tk.N && gk < K)
+            v = tB ? sg.B[(int64_t)gn + (int64_t)gk * sg.ldb] : sg.B[(int64_t)gk + (int64_t)gn * sg.ldb];
+          rb[r] = v;
+        }
+      };
+      auto store = [&](int b) {
+#pragma unroll
+        for (int r = 0; r < AE; ++r) {
+          const int idx = tid + r * kThreads;
+          int m, k;
+          if (!tA) { m = idx % BM; k = idx / BM; } else { k = idx % kBK; m = idx / kBK; }
+          sA[b][k][m] = ra[r];
+        }
+#pragma unroll
+        for (int r = 0; r < BE; ++r) {
+          const int idx = tid + r * kThreads;
+          int n, k;
+          if (!tB) { k = idx % kBK; n = idx / kBK; } else { n = idx % BN; k = idx / BN; }
+          sB[b][k][n] = rb[r];
+        }
+      };
+      if (K <= 0) continue;
+      load(0);
+      for (int k0 = 0; k0 < K; k0 += kBK) {
+        store(buf);
+        __syncthreads();
+        if (k0 + kBK < K) load(k0 + kBK);
+#pragma unroll
+        for (int kk = 0; kk < kBK / 4; ++kk) {
+          double a[FM], b[FN];
+#pragma unroll
+          for (int i = 0; i < FM; ++i) a[i] = sA[buf][kk * 4 + lk][wm * WM + i * 16 + lr];
+#pragma unroll
+          for (int j = 0; j < FN; ++j) b[j] = sB[buf][kk * 4 + lk][wn * WN + j * 16 + lr];
+#pragma unroll
+          for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        buf ^= 1;
+      }
+    }
+    // epilogue: accumulator item r of lane l holds row 4r + l/16, column l%16
+    // of each 16x16 block (gfx950 v_mfma_f64_16x16x4f64 output layout)
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int col = n0 + wn * WN + j * 16 + lr;
+        if (col >= tk.N) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm * WM + i * 16 + r * 4 + lk;
+          if (row < tk.M) {
+            double* p = tk.C + (int64_t)row + (int64_t)col * tk.ldc;
+            double v = acc[i][j][r];
+            if (tk.beta != 0.0) v += tk.beta * *p;
+            *p = v;
+          }
+        }
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+constexpr int kMixChunk = 2048;
+
+__global__ __launch_bounds__(kThreads) void k_mix(const MixWork* __restrict__ work, int nwork,
+                                                  const MixDst* __restrict__ dsts,
+                                                  const MixTerm* __restrict__ terms) {
+  for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+    const MixWork w = work[wi];
+    const MixDst d = dsts[w.dst];
+    const int64_t e1 = min(w.e0 + (int64_t)kMixChunk, d.n);
+    for (int64_t e = w.e0 + threadIdx.x; e < e1; e += kThreads) {
+      double acc = 0.0;
+      for (int t = 0; t < d.nt; ++t) {
+        const MixTerm tm = terms[d.t0 + t];
+        acc += tm.coef * tm.src[e];
+      }
+      d.dst[e] = acc;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_copy(const CopyWork* __restrict__ work, int nwork,
+                                                   const CopyOp* __restrict__ ops) {
+  for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+    const CopyWork w = work[wi];
+    const CopyOp op = ops[w.op];
+    const int64_t n = (int64_t)op.rows * op.cols;
+    const int64_t e1 = min(w.e0 + (int64_t)kMixChunk, n);
+    for (int64_t e = w.e0 + threadIdx.x; e < e1; e += kThreads) {
+      const int r = (int)(e % op.rows), c = (int)(e / op.rows);
+      double v = 0.0;
+      if (op.src) {
+        v = op.alpha * op.src[(int64_t)r + (int64_t)c * op.lds];
+        if (op.rscale) v *= op.rscale[r];
+        if (op.cscale) v *= op.cscale[c];
+      }
+      op.dst[(int64_t)r + (int64_t)c * op.ldd] = v;
+    }
+  }
+}
+
+__global__ void k_fill(double* p, int64_t n, double v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+constexpr int kDotBlocks = 256;
+
+struct DotArgs {
+  const double* x[4];
+  const double* y[4];
+};
+
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(kThreads) void k_dot_partial(DotArgs args, int64_t n, double* partial) {
+  const int pair = blockIdx.y;
+  const double* __restrict__ x = args.x[pair];
+  const double* __restrict__ y = args.y[pair];
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kThreads)
+    s += x[i] * y[i];
+  __shared__ double red[kThreads / 64];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < kThreads / 64; ++i) t += red[i];
+    partial[pair * kDotBlocks + blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_dot_final(const double* partial, double* out) {
+  const int pair = blockIdx.x;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < kDotBlocks; i += kThreads) s += partial[pair * kDotBlocks + i];
+  __shared__ double red[kThreads / 64];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < kThreads / 64; ++i) t += red[i];
+    out[pair] = t;
+  }
+}
+
+__global__ void k_axpy_dev(double* y, const double* x, int64_t n, const double* coef) {
+  const double c = *coef;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] += c * x[i];
+}
+
+__global__ void k_scale_dev(double* x, int64_t n, const double* coef) {
+  const double c = *coef;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] *= c;
+}
+
+__global__ void k_comb_dev(double* out, const double* a, const double* b, int64_t n,
+                           const double* ca, const double* cb) {
+  const double u = *ca, v = *cb;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = u * a[i] + v * b[i];
+}
+
+// Scalar steps of algs/DMRG.jl:57-90 on device scalars.
+__global__ void k_lz_scalar(double* s, int stage) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (stage == 0) {            // normalise the orthogonality-centre tensor
+    s[S_INVOC] = 1.0 / sqrt(s[S_OC]);
+  } else if (stage == 1) {     // alpha1 = <AA|Hpsi>, AA = theta/|theta|
+    const double nrm = sqrt(s[S_TT]);
+    s[S_NRM] = nrm;
+    s[S_A1] = s[S_TH] / nrm;
+    s[S_CAXPY] = -s[S_A1] / nrm;   // psi2 = Hpsi - alpha1*theta/|theta|
+  } else if (stage == 2) {     // beta1 = |psi2|
+    const double b = sqrt(s[S_PP]);
+    s[S_B1] = b;
+    const bool go = b > 1e-10;
+    s[S_FLAG] = go ? 1.0 : 0.0;
+    s[S_INVB] = go ? 1.0 / b : 0.0;
+  } else if (stage == 3) {     // 2x2 Rayleigh-Ritz, lowest eigenpair
+    const double a1 = s[S_A1];
+    if (s[S_FLAG] == 0.0) {
+      s[S_E] = a1;
+      s[S_CA] = 1.0 / s[S_NRM];
+      s[S_CB] = 0.0;
+      return;
+    }
+    const double b = s[S_B1], a2 = s[S_PHP];
+    s[S_A2] = a2;
+    const double dd = 0.5 * (a2 - a1);
+    const double r = sqrt(dd * dd + b * b);
+    const double lam = 0.5 * (a1 + a2) - r;
+    double v1, v2;
+    if (dd >= 0.0) { v1 = -(dd + r); v2 = b; } else { v1 = b; v2 = -(r - dd); }
+    const double nv = sqrt(v1 * v1 + v2 * v2);
+    v1 /= nv;
+    v2 /= nv;
+    s[S_E] = lam;
+    s[S_CA] = v1 / s[S_NRM];
+    s[S_CB] = v2;
+  }
+}
+
+// Lane-layout probes of v_mfma_f64_16x16x4f64 (one wave, in-bounds):
+//  mode 0: a = lane+1, b = 1      -> each output is a row sum of A
+//  mode 1: a = 1, b = lane+1      -> each output is a column sum of B
+//  mode 2: A[:,0] = i+1, A[:,1] = 1, B[0,:] = 1, B[1,:] = 1000(j+1) under the
+//          layout A[i][k] <- lane i+16k, B[k][j] <- lane j+16k, giving
+//          D[i][j] = (i+1) + 1000(j+1), found in item r of lane l for
+//          i = 4r + l/16, j = l%16
+__global__ void k_mfma_probe(double* out) {
+  const int l = threadIdx.x;
+  const int r16 = l & 15, k = l >> 4;
+  for (int mode = 0; mode < 3; ++mode) {
+    double a, b;
+    if (mode == 0) {
+      a = (double)(l + 1);
+      b = 1.0;
+    } else if (mode == 1) {
+      a = 1.0;
+      b = (double)(l + 1);
+    } else {
+      a = k == 0 ? (double)(r16 + 1) : (k == 1 ? 1.0 : 0.0);
+      b = k == 0 ? 1.0 : (k == 1 ? 1000.0 * (r16 + 1) : 0.0);
+    }
+    double4_t acc = double4_t{0.0, 0.0, 0.0, 0.0};
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    for (int r = 0; r < 4; ++r) out[mode * 256 + l * 4 + r] = acc[r];
+  }
+}
+
+void launch_mfma_probe(Context& c, double* out) {
+  hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, c.stream, out);
+  HIP_OK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+static int grid_for(int64_t n, int per = kThreads * 4) {
+  int64_t g = (n + per - 1) / per;
+  if (g < 1) g = 1;
+  if (g > 4096) g = 4096;
+  return (int)g;
+}
+
+struct TileShape {
+  int bm, bn;
+};
+static const TileShape kShapes[3] = {{32, 32}, {64, 64}, {128, 128}};
+
+int choose_gemm_cfg(const std::vector<GemmTask>& tasks, int num_cu) {
+  // Pick the largest tile whose tile count still fills the chip ~2x; small
+  // blocks (quantum-number sectors) fall back to smaller tiles.
+  for (int cfg = 2; cfg >= 1; --cfg) {
+    int64_t tiles = 0, waste = 0, useful = 0;
+    for (const auto& t : tasks) {
+      const int bm = kShapes[cfg].bm, bn = kShapes[cfg].bn;
+      const int64_t tm = (t.M + bm - 1) / bm, tn = (t.N + bn - 1) / bn;
+      tiles += tm * tn;
+      useful += (int64_t)t.M * t.N;
+      waste += tm * bm * tn * bn;
+    }
+    if (tiles >= 2 * num_cu && useful * 10 >= waste * 7) return cfg;
+  }
+  return 1;
+}
+
+double gemm_flops(const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs) {
+  double f = 0.0;
+  for (const auto& t : tasks)
+    for (int s = 0; s < t.nseg; ++s) f += 2.0 * t.M * (double)t.N * segs[t.seg0 + s].K;
+  return f;
+}
+
+void launch_gemm(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs,
+                 int cfg) {
+  if (tasks.empty()) return;
+  if (cfg < 0) cfg = c.gemm_cfg_override >= 0 ? c.gemm_cfg_override : choose_gemm_cfg(tasks, c.num_cu);
+  c.last_gemm_cfg = cfg;
+  const int bm = kShapes[cfg].bm, bn = kShapes[cfg].bn;
+  // tiles ordered by descending work so the persistent grid balances
+  std::vector<std::pair<int64_t, int>> order;
+  order.reserve(tasks.size());
+  for (size_t i = 0; i < tasks.size(); ++i) {
+    int64_t k = 0;
+    for (int s = 0; s < tasks[i].nseg; ++s) k += segs[tasks[i].seg0 + s].K;
+    order.push_back({-(k + 1), (int)i});
+  }
+  std::stable_sort(order.begin(), order.end());
+  std::vector<GemmTile> tiles;
+  for (auto& o : order) {
+    const GemmTask& t = tasks[o.second];
+    for (int n0 = 0; n0 < t.N; n0 += bn)
+      for (int m0 = 0; m0 < t.M; m0 += bm) tiles.push_back({o.second, m0, n0, cfg});
+  }
+  if (tiles.empty()) return;
+  auto* dt = (GemmTile*)c.upload(tiles.data(), tiles.size() * sizeof(GemmTile));
+  auto* dk = (GemmTask*)c.upload(tasks.data(), tasks.size() * sizeof(GemmTask));
+  auto* ds = (GemmSeg*)c.upload(segs.data(), std::max<size_t>(1, segs.size()) * sizeof(GemmSeg));
+  const int occ = cfg == 2 ? 2 : 4;
+  const int grid = (int)std::min<int64_t>((int64_t)tiles.size(), (int64_t)c.num_cu * occ);
+  const int nt = (int)tiles.size();
+  if (cfg == 0)
+    hipLaunchKernelGGL((k_gemm_f64<32, 32>), dim3(grid), dim3(kThreads), 0, c.stream, dt, nt, dk, ds);
+  else if (cfg == 1)
+    hipLaunchKernelGGL((k_gemm_f64<64, 64>), dim3(grid), dim3(kThreads), 0, c.stream, dt, nt, dk, ds);
+  else
+    hipLaunchKernelGGL((k_gemm_f64<128, 128>), dim3(grid), dim3(kThreads), 0, c.stream, dt, nt, dk, ds);
+  HIP_OK(hipGetLastError());
+}
+
+void launch_mix(Context& c, const std::vector<MixDst>& dsts, const std::vector<MixTerm>& terms) {
+  if (dsts.empty()) return;
+  std::vector<MixWork> work;
+  for (size_t i = 0; i < dsts.size(); ++i)
+    for (int64_t e = 0; e < dsts[i].n; e += kMixChunk) work.push_back({(int)i, 0, e});
+  if (work.empty()) return;
+  auto* dw = (MixWork*)c.upload(work.data(), work.size() * sizeof(MixWork));
+  auto* dd = (MixDst*)c.upload(dsts.data(), dsts.size() * sizeof(MixDst));
+  auto* dt = (MixTerm*)c.upload(terms.data(), std::max<size_t>(1, terms.size()) * sizeof(MixTerm));
+  const int grid = (int)std::min<size_t>(work.size(), (size_t)c.num_cu * 8);
+  hipLaunchKernelGGL(k_mix, dim3(grid), dim3(kThreads), 0, c.stream, dw, (int)work.size(), dd, dt);
+  HIP_OK(hipGetLastError());
+}
+
+void launch_copy(Context& c, const std::vector<CopyOp>& ops) {
+  if (ops.empty()) return;
+  std::vector<CopyWork> work;
+  for (size_t i = 0; i < ops.size(); ++i) {
+    const int64_t n = (int64_t)ops[i].rows * ops[i].cols;
+    for (int64_t e = 0; e < n; e += kMixChunk) work.push_back({(int)i, 0, e});
+  }
+  if (work.empty()) return;
+  auto* dw = (CopyWork*)c.upload(work.data(), work.size() * sizeof(CopyWork));
+  auto* dops = (CopyOp*)c.upload(ops.data(), ops.size() * sizeof(CopyOp));
+  const int grid = (int)std::min<size_t>(work.size(), (size_t)c.num_cu * 8);
+  hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kThreads), 0, c.stream, dw, (int)work.size(), dops);
+  HIP_OK(hipGetLastError());
+}
+
+void launch_fill(Context& c, double* p, int64_t n, double v) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(kThreads), 0, c.stream, p, n, v);
+  HIP_OK(hipGetLastError());
+}
+
+void launch_dots(Context& c, int npairs, const double* const* xs, const double* const* ys,
+                 int64_t n, double* out) {
+  if (npairs < 1 || npairs > 4) throw Error(-2, "launch_dots: 1..4 pairs");
+  DotArgs a{};
+  for (int i = 0; i < npairs; ++i) {
+    a.x[i] = xs[i];
+    a.y[i] = ys[i];
+  }
+  hipLaunchKernelGGL(k_dot_partial, dim3(kDotBlocks, npairs), dim3(kThreads), 0, c.stream, a, n,
+                     c.partial);
+  HIP_OK(hipGetLastError());
+  hipLaunchKernelGGL(k_dot_final, dim3(npairs), dim3(kThreads), 0, c.stream, c.partial, out);
+  HIP_OK(hipGetLastError());
+}
+
+void launch_axpy_dev(Context& c, double* y, const double* x, int64_t n, const double* coef) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_axpy_dev, dim3(grid_for(n)), dim3(kThreads), 0, c.stream, y, x, n, coef);
+  HIP_OK(hipGetLastError());
+}
+
+void launch_scale_dev(Context& c, double* x, int64_t n, const double* coef) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_scale_dev, dim3(grid_for(n)), dim3(kThreads), 0, c.stream, x, n, coef);
+  HIP_OK(hipGetLastError());
+}
+
+void launch_comb_dev(Context& c, double* out, const double* a, const double* b, int64_t n,
+                     const double* ca, const double* cb) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_comb_dev, dim3(grid_for(n)), dim3(kThreads), 0, c.stream, out, a, b, n, ca,
+                     cb);
+  HIP_OK(hipGetLastError());
+}
+
+void launch_lz_scalar(Context& c, int stage) {
+  hipLaunchKernelGGL(k_lz_scalar, dim3(1), dim3(64), 0, c.stream, c.scal, stage);
+  HIP_OK(hipGetLastError());
+}
+
+}  // namespace dmrj

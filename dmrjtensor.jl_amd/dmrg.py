@@ -1,0 +1,182 @@
+"""Two-site DMRG driver on the GPU path.
+
+Mirrors algs/DMRG.jl and algs/optimizeMPS.jl:
+  dmrg           algs/DMRG.jl:549-589 (method dispatch; "twosite" only)
+  dmrg_twosite   algs/DMRG.jl:675-691
+  optmps         algs/optimizeMPS.jl:759-849 (sweep driver, plugin kwargs)
+  algvars        algs/optimizeMPS.jl:36-106, defaults :514-577, SvNbond :614
+  twostep        algs/DMRG.jl:362-405 (one C-ABI call: normalise, lanczos2,
+                 truncated SVD; the reference's unused twosite_update is not
+                 computed)
+  SvNcheck       algs/DMRG.jl:172-210
+  dmrgcvg        algs/DMRG.jl:610-652
+  startval       algs/optimizeMPS.jl:255-258 (single-site H_eff at the centre)
+
+The host loop only moves handles; all arithmetic runs in libdmrjhip.
+"""
+import ctypes as C
+import math
+import time
+
+import numpy as np
+
+from . import _lib
+from .network import DeviceTensor, Lupdate, Rupdate, makeEnv
+
+
+class algvars:
+    """Parameter/state record of the two-site path (field names of the
+    reference's algvars)."""
+
+    def __init__(self, psi, maxm=0, minm=2, sweeps=1, cutoff=0.0, goal=0.0, silent=True,
+                 SvNbond=None, precision="f64"):
+        self.maxm = maxm
+        self.minm = minm
+        self.sweeps = sweeps
+        self.cutoff = cutoff
+        self.goal = goal
+        self.silent = silent
+        self.SvNbond = (len(psi) + 1) // 2 if SvNbond is None else SvNbond
+        self.precision = precision
+        self.energy = 0.0
+        self.lastenergy = 0.0
+        self.truncerr = 0.0
+        self.maxtrunc = 0.0
+        self.biggestm = 0
+        self.entropy = 0.0
+        self.lastentropy = 0.0
+        self.storeD = [0.0] * 8
+        self.maxshowD = 8
+        self.cvgE = True
+        self.efficient = False
+        self.Lenv = None
+        self.Renv = None
+        self.history = []
+        self.sweep_times = []
+
+
+def singlesite_update(Lenv, Renv, A, mpo, i, precision="f64"):
+    """H_1site A (algs/optimizeMPS.jl:503-511)."""
+    ctx = A.ctx
+    h = C.c_void_p()
+    ctx.call("dmrj_heff1_apply", Lenv.h, mpo.device(ctx, i), Renv.h, A.h, _lib.PREC[precision],
+             C.byref(h))
+    return DeviceTensor(ctx, h)
+
+
+def startval(Lenv, Renv, psi, mpo, i):
+    """<psi_oc|H_1site|psi_oc> (algs/optimizeMPS.jl:255-258)."""
+    H = singlesite_update(Lenv, Renv, psi[i], mpo, i)
+    e = psi[i].dot(H)
+    H.free()
+    return e
+
+
+def SvNcheck(i, j, svals, Ns, params):
+    """algs/DMRG.jl:172-210 (allSvNbond = false)."""
+    if params.maxtrunc < params.truncerr:
+        params.maxtrunc = params.truncerr
+    if params.biggestm < len(svals):
+        params.biggestm = len(svals)
+    if i == min(max(2, params.SvNbond + 1), Ns) and j < 0:
+        D = np.sort(np.asarray(svals))[::-1]
+        p = D ** 2
+        params.entropy = float(-np.sum(np.where(p > 0, p * np.log(np.where(p > 0, p, 1.0)), 0.0)))
+        params.storeD = [float(x) for x in D[:min(len(D), params.maxshowD)]]
+
+
+def twostep(n, j, i, iL, iR, psi, mpo, Lenv, Renv, params):
+    """One bond step (algs/DMRG.jl:362-405) through dmrj_twostep."""
+    ctx = psi.ctx
+    cap = max(8, 4 * (params.maxm if params.maxm > 0 else 4096))
+    sv = np.zeros(cap)
+    nl, nr = C.c_void_p(), C.c_void_p()
+    e, te = C.c_double(), C.c_double()
+    kept = C.c_int64()
+    ctx.call("dmrj_twostep", Lenv[iL].h, Renv[iR].h, psi[iL].h, psi[iR].h, mpo.device(ctx, iL),
+             mpo.device(ctx, iR), 1 if j > 0 else -1, int(params.maxm), int(params.minm),
+             float(params.cutoff), _lib.PREC[params.precision], C.byref(nl), C.byref(nr), C.byref(e),
+             C.byref(te), C.byref(kept), _lib.dptr(sv), cap)
+    psi[iL].free()
+    psi[iR].free()
+    psi[iL] = DeviceTensor(ctx, nl)
+    psi[iR] = DeviceTensor(ctx, nr)
+    params.energy = e.value
+    params.truncerr = te.value
+    params.biggestm = max(params.biggestm, int(kept.value))
+    SvNcheck(i, j, sv[:min(cap, kept.value)], len(psi), params)
+
+
+def dmrgcvg(n, timer, params):
+    """algs/DMRG.jl:610-652 (cvgE = true)."""
+    if not params.silent:
+        print("Sweep %d (back and forth): %.5f sec" % (n, timer))
+        print("Largest truncation = %s, m = %d" % (params.maxtrunc, params.biggestm))
+        print("Energy at sweep %d is %.12f" % (n, params.energy))
+        print(flush=True)
+    params.history.append(dict(sweep=n, energy=float(params.energy), maxtrunc=float(params.maxtrunc),
+                               biggestm=int(params.biggestm), entropy=float(params.entropy),
+                               storeD=list(params.storeD), time=timer))
+    params.biggestm = 0
+    params.truncerr = params.maxtrunc
+    params.maxtrunc = 0.0
+    checkE = abs(params.energy - params.lastenergy)
+    if checkE < params.goal:
+        return True
+    params.lastenergy = params.energy
+    return False
+
+
+def optmps(psi, mpo, params, stepfct=twostep, cvgfct=dmrgcvg, measfct=startval):
+    """Sweep driver (algs/optimizeMPS.jl:759-849) with the same pluggable
+    step / convergence / measurement functions."""
+    Ns = len(psi)
+    if params.Lenv is None or params.Renv is None:
+        params.Lenv, params.Renv = makeEnv(psi, mpo)
+    Lenv, Renv = params.Lenv, params.Renv
+    if params.energy == 0.0 and measfct is not None:
+        params.energy = measfct(Lenv[psi.oc], Renv[psi.oc], psi, mpo, psi.oc)
+    params.lastenergy = params.energy
+    Nsteps = 2 * (Ns - 1)
+    if psi.oc == 1:
+        j = 1
+    elif psi.oc == Ns:
+        j = -1
+    else:
+        j = 1
+    for n in range(1, params.sweeps + 1):
+        t0 = time.perf_counter()
+        for _ in range(Nsteps):
+            i = psi.oc
+            iL, iR = (i, i + 1) if j > 0 else (i - 1, i)
+            stepfct(n, j, i, iL, iR, psi, mpo, Lenv, Renv, params)
+            if j > 0:
+                Lenv[i + 1] = Lupdate(Lenv[i], psi[i], mpo, i)
+            else:
+                Renv[i - 1] = Rupdate(Renv[i], psi[i], mpo, i)
+            psi.oc += j
+            if (psi.oc == Ns and j > 0) or (psi.oc == 1 and j < 0):
+                j = -j
+        psi.ctx.sync()
+        timer = time.perf_counter() - t0
+        params.sweep_times.append(timer)
+        if cvgfct(n, timer, params):
+            break
+    return params.energy
+
+
+def dmrg(psi, mpo, m=0, minm=2, sweeps=1, cutoff=0.0, silent=True, goal=0.0, method="twosite",
+         SvNbond=None, precision="f64", params=None):
+    """dmrg(psi, mpo; m, minm, sweeps, cutoff, silent, goal, method)
+    (algs/DMRG.jl:549-589).  Optimises ``psi`` in place and returns the
+    energy; the per-sweep report is in ``params.history`` when a params
+    object is passed."""
+    if method != "twosite":
+        raise ValueError("DMRG method (%s) not defined on the GPU path" % method)
+    if params is None:
+        params = algvars(psi)
+    params.maxm, params.minm, params.sweeps, params.cutoff = m, minm, sweeps, cutoff
+    params.silent, params.goal, params.precision = silent, goal, precision
+    if SvNbond is not None:
+        params.SvNbond = SvNbond
+    return optmps(psi, mpo, params)

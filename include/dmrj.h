@@ -1,0 +1,170 @@
+/* libdmrjhip: MI355X (gfx950) two-site DMRG hot path behind a plain C ABI.
+ *
+ * Scope: the inner loop of the reference's two-site DMRG (bakerte/
+ * DMRJtensor.jl v0.19.0): H_eff*psi inside the two-vector Lanczos, the left
+ * and right environment updates, the truncated SVD of the two-site tensor, on
+ * dense tensors and on U(1) / U(1)xU(1) block-sparse tensors.
+ *
+ * Conventions (same index meaning as the reference):
+ *   MPS tensor     A[alpha, s, beta]          (lib/types.jl:84)
+ *   MPO tensor     W[w, j(ket), k(bra), u]    (lib/MPO/makeMPO.jl:76)
+ *   left env       Lenv[bra, w, ket]          (lib/Env/makeEnv.jl:17-26)
+ *   right env      Renv[ket, u, bra]          (lib/Env/makeEnv.jl:17-26)
+ * Dense host arrays crossing this boundary are column-major (Julia order).
+ * Charges: nq = 0 (no quantum numbers), 1 (U(1)) or 2 (U(1)xU(1)); charge
+ * arrays are int32 [n][nq].  Bond charges must be sorted (lexicographically
+ * non-decreasing) along each bond index; a block A[a, s, b] may be non-zero
+ * only when q(a) + q(s) == q(b).  MPO link charges delta satisfy
+ * delta(u) == delta(w) + q(k) - q(j) on every non-zero W[w, j, k, u].
+ *
+ * Ownership: the library owns device memory; callers hold opaque handles and
+ * free them explicitly.  Host arrays are borrowed only during a call.
+ * Errors: every call returns 0 on success or a negative code; the message is
+ * available from dmrj_last_error(ctx).  One context = one HIP stream; a
+ * context is not re-entrant.  Scalar outputs synchronise the stream.
+ */
+#ifndef DMRJ_H
+#define DMRJ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dmrj_ctx dmrj_ctx;
+typedef struct dmrj_tensor dmrj_tensor;
+typedef struct dmrj_mpo dmrj_mpo;
+
+enum {
+  DMRJ_OK = 0,
+  DMRJ_ERR_ARG = -1,      /* invalid argument or inconsistent shapes/charges */
+  DMRJ_ERR_STATE = -2,    /* operation not valid for this handle */
+  DMRJ_ERR_DEVICE = -3,   /* HIP runtime or device failure */
+  DMRJ_ERR_SOLVER = -4,   /* rocSOLVER failure */
+  DMRJ_ERR_NUMERIC = -5   /* NaN/Inf or non-symmetric input detected */
+};
+
+enum { DMRJ_PREC_F64 = 0, DMRJ_PREC_BF16X3 = 1, DMRJ_PREC_F16S = 2 };
+
+/* ---- context --------------------------------------------------------- */
+/* No counterpart in the reference (Julia holds no device state). */
+int dmrj_ctx_create(int device, dmrj_ctx** ctx);
+int dmrj_ctx_destroy(dmrj_ctx* ctx);
+const char* dmrj_last_error(const dmrj_ctx* ctx);
+int dmrj_ctx_sync(dmrj_ctx* ctx);
+int dmrj_version(void);
+/* memory statistics: bytes live, bytes cached, peak bytes */
+int dmrj_ctx_memstats(dmrj_ctx* ctx, int64_t* out3);
+/* force a GEMM tile shape (0: 32x32, 1: 64x64, 2: 128x128, -1: automatic) */
+int dmrj_ctx_set_gemm_cfg(dmrj_ctx* ctx, int cfg);
+/* accumulated algorithmic flops of H_eff applications and env updates */
+int dmrj_ctx_flops(dmrj_ctx* ctx, double* out2, int reset);
+
+/* ---- tensors --------------------------------------------------------- */
+/* Replaces the dense->Qtens conversion of an MPS tensor
+ * (lib/MPS/makeqMPS.jl:26-112, TensorPACK Qtens) and tensor upload.
+ * A is column-major [mL, d, mR].  Entries outside charge-allowed blocks must
+ * be zero (|dropped| <= 1e-12 |A|) or the call fails with DMRJ_ERR_NUMERIC. */
+int dmrj_site_from_dense(dmrj_ctx* ctx, int nq, int64_t mL, int64_t d, int64_t mR,
+                         const double* A, const int32_t* qL, const int32_t* qP,
+                         const int32_t* qR, dmrj_tensor** out);
+/* Two-site tensor AA[alpha, s1, s2, beta] (column-major [mL, d1, d2, mR]), the
+ * object simplelanczos returns (algs/DMRG.jl:57-90). */
+int dmrj_site2_from_dense(dmrj_ctx* ctx, int nq, int64_t mL, int64_t d1, int64_t d2, int64_t mR,
+                          const double* A, const int32_t* qL, const int32_t* qP1,
+                          const int32_t* qP2, const int32_t* qR, dmrj_tensor** out);
+/* dims: [kind, n0, n1, n2, n3, nelem, nblocks, nsectors_left, nsectors_right]
+ * kind 0: MPS tensor [mL,d,mR,1]; 1: two-site tensor [mL,d1,d2,mR];
+ * 2: left env [bra,D,ket,1]; 3: right env [ket,D,bra,1]. */
+int dmrj_tensor_info(const dmrj_tensor* t, int64_t* dims9);
+/* Dense column-major copy in the reference's index order; optional charge
+ * outputs (may be NULL) for the first and last index. */
+int dmrj_tensor_to_dense(dmrj_ctx* ctx, const dmrj_tensor* t, double* out, int32_t* qfirst,
+                         int32_t* qlast);
+int dmrj_tensor_free(dmrj_ctx* ctx, dmrj_tensor* t);
+int dmrj_tensor_copy(dmrj_ctx* ctx, const dmrj_tensor* t, dmrj_tensor** out);
+
+/* ---- MPO ------------------------------------------------------------- */
+/* One MPO site W[w,j,k,u] (lib/MPO/makeMPO.jl:59-92; charges as
+ * lib/MPO/makeqMPO.jl:23-93).  W column-major [DL, d, d, DR]; deltaL [DL][nq],
+ * deltaR [DR][nq] (NULL when nq == 0). */
+int dmrj_mpo_create(dmrj_ctx* ctx, int nq, int64_t DL, int64_t d, int64_t DR, const double* W,
+                    const int32_t* qP, const int32_t* deltaL, const int32_t* deltaR,
+                    dmrj_mpo** out);
+int dmrj_mpo_free(dmrj_ctx* ctx, dmrj_mpo* W);
+
+/* ---- environments ---------------------------------------------------- */
+/* makeBoundary (lib/Env/makeEnv.jl:79-84): 1x1x1 tensor holding 1.0 on a
+ * bond of the given charge (q may be NULL when nq == 0). left: 1 = Lenv. */
+int dmrj_env_boundary(dmrj_ctx* ctx, int left, int nq, const int32_t* q, dmrj_tensor** out);
+/* Lupdate (lib/MPO_MPS/Lupdate.jl:17-26): Lout[b',u,b] =
+ * sum L[a',w,a] conj(A[a',k,b']) W[w,j,k,u] A[a,j,b]. */
+int dmrj_env_left(dmrj_ctx* ctx, const dmrj_tensor* Lin, const dmrj_tensor* A,
+                  const dmrj_mpo* W, dmrj_tensor** Lout);
+/* Rupdate (lib/MPO_MPS/Rupdate.jl:17-26): Rout[a,w,a'] =
+ * sum A[a,j,b] W[w,j,k,u] conj(A[a',k,b']) R[b,u,b']. */
+int dmrj_env_right(dmrj_ctx* ctx, const dmrj_tensor* Rin, const dmrj_tensor* A,
+                   const dmrj_mpo* W, dmrj_tensor** Rout);
+
+/* ---- H_eff and Lanczos ----------------------------------------------- */
+/* psiL*psiR: contract(psiL,3,psiR,1) (algs/DMRG.jl:60). */
+int dmrj_theta2(dmrj_ctx* ctx, const dmrj_tensor* psiL, const dmrj_tensor* psiR,
+                dmrj_tensor** theta);
+/* Two-site H_eff*theta, make2site / DMRG.jl:69-73 (algs/DMRG.jl:48-54). */
+int dmrj_heff2_apply(dmrj_ctx* ctx, const dmrj_tensor* L, const dmrj_mpo* W1,
+                     const dmrj_mpo* W2, const dmrj_tensor* R, const dmrj_tensor* theta,
+                     int precision, dmrj_tensor** out);
+/* One-site H_eff*A, singlesite_update (algs/optimizeMPS.jl:503-511). */
+int dmrj_heff1_apply(dmrj_ctx* ctx, const dmrj_tensor* L, const dmrj_mpo* W,
+                     const dmrj_tensor* R, const dmrj_tensor* A, int precision,
+                     dmrj_tensor** out);
+/* simplelanczos (algs/DMRG.jl:57-90): two-vector Krylov on psiL*psiR;
+ * returns the Ritz vector AA (two-site tensor) and the energy. */
+int dmrj_lanczos2(dmrj_ctx* ctx, const dmrj_tensor* L, const dmrj_tensor* R,
+                  const dmrj_tensor* psiL, const dmrj_tensor* psiR, const dmrj_mpo* W1,
+                  const dmrj_mpo* W2, int precision, dmrj_tensor** AA, double* energy);
+
+/* ---- truncated SVD ---------------------------------------------------- */
+/* fsvd!(AA,[[1,2],[3,4]]; m, minm, cutoff, mag=1.) and the split of
+ * algs/DMRG.jl:387-395: dir > 0 -> (U, D*V); dir < 0 -> (U*D, V).
+ * svals receives the kept singular values (descending, renormalised to unit
+ * 2-norm, at most svals_cap of them); kept = number kept. */
+int dmrj_svd_trunc2(dmrj_ctx* ctx, const dmrj_tensor* AA, int64_t m, int64_t minm, double cutoff,
+                    int dir, dmrj_tensor** psiL, dmrj_tensor** psiR, double* truncerr,
+                    int64_t* kept, double* svals, int64_t svals_cap);
+
+/* twostep (algs/DMRG.jl:362-405) without the unused twosite_update: normalise
+ * the orthogonality centre (psiL if dir > 0 else psiR, in place), lanczos2,
+ * svd_trunc2.  psiL/psiR are replaced by new handles (the old ones stay
+ * valid and are owned by the caller). */
+int dmrj_twostep(dmrj_ctx* ctx, const dmrj_tensor* L, const dmrj_tensor* R, dmrj_tensor* psiL,
+                 dmrj_tensor* psiR, const dmrj_mpo* W1, const dmrj_mpo* W2, int dir, int64_t m,
+                 int64_t minm, double cutoff, int precision, dmrj_tensor** newL,
+                 dmrj_tensor** newR, double* energy, double* truncerr, int64_t* kept,
+                 double* svals, int64_t svals_cap);
+
+/* ---- vector operations (TensorPACK norm/div!/sub!/add!/ccontract) ---- */
+int dmrj_dot(dmrj_ctx* ctx, const dmrj_tensor* x, const dmrj_tensor* y, double* out);
+int dmrj_norm(dmrj_ctx* ctx, const dmrj_tensor* x, double* out);
+int dmrj_scale(dmrj_ctx* ctx, dmrj_tensor* x, double a);
+int dmrj_axpy(dmrj_ctx* ctx, double a, const dmrj_tensor* x, dmrj_tensor* y);
+/* div!(psi[oc], norm(psi[oc])) without a host round trip (algs/DMRG.jl:365-369). */
+int dmrj_normalize(dmrj_ctx* ctx, dmrj_tensor* x);
+
+/* ---- kernel self-tests (no counterpart in the reference) -------------- */
+/* One grouped-GEMM launch on host arrays: C[MxN] = op(A) op(B) with the tile
+ * configuration cfg (0: 32x32, 1: 64x64, 2: 128x128).  A is stored KxM when
+ * tA != 0 (else MxK), B is stored NxK when tB != 0 (else KxN), column-major,
+ * leading dimension = stored row count. */
+int dmrj_selftest_gemm(dmrj_ctx* ctx, int cfg, int64_t M, int64_t N, int64_t K, int tA, int tB,
+                       const double* A, const double* B, double* C);
+/* Lane layout of v_mfma_f64_16x16x4f64: writes the 64x4 accumulator values
+ * of three probe products (row sums, column sums, encoded product), 768
+ * doubles. */
+int dmrj_selftest_mfma_f64(dmrj_ctx* ctx, double* out768);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DMRJ_H */

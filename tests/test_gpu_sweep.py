@@ -1,0 +1,111 @@
+"""Whole two-site DMRG runs on the GPU path against the oracle and the
+reference's constants.
+
+Per-sweep energies: <= 1e-12 relative to the oracle (north_star).  Centre
+bond spectra (top-8 singular values) and the largest truncation error per
+sweep are compared within 1e-8 relative (gauge- and tie-robust quantities).
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import dmrg as od
+from oracle import models as om
+from oracle import mps as omps
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_cfg1_dense_heisenberg_L20_m64(pkg, ctx):
+    """open Heisenberg L=20, m=64, 6 sweeps, Krylov 2, cutoff 1e-12, seed-0
+    N(0,1) initial MPS."""
+    say("case cfg1 L=20 m=64 6 sweeps")
+    with open(os.path.join(GOLD, "cfg1.json")) as f:
+        g = json.load(f)
+    L = 20
+    psi = pkg.randMPS_normal(L, 2, 64, seed=0, ctx=ctx)
+    host, _ = psi.to_dense()
+    opsi = omps.seeded_normal_mps(L, 2, 64, seed=0)
+    for a, b in zip(host, opsi.A):
+        assert np.allclose(a, b, rtol=1e-13, atol=1e-15)
+    mpo = pkg.makeMPO(pkg.heisenbergMPO(), 2, L)
+    params = pkg.algvars(psi)
+    E = pkg.dmrg(psi, mpo, m=64, sweeps=6, cutoff=1e-12, params=params)
+    for h, gh in zip(params.history, g["sweeps"]):
+        rel = abs(h["energy"] - gh["energy"]) / abs(gh["energy"])
+        say(" sweep", h["sweep"], "%.15f" % h["energy"], "oracle %.15f" % gh["energy"], "rel %.2e" % rel,
+            "maxtrunc", h["maxtrunc"], gh["maxtrunc"], "t=%.3fs" % h["time"])
+        assert rel <= 1e-12
+        assert np.allclose(h["storeD"], gh["storeD"], rtol=1e-8, atol=1e-12)
+        assert abs(h["maxtrunc"] - gh["maxtrunc"]) <= 1e-8 * max(gh["maxtrunc"], 1e-14) + 1e-15
+        assert h["biggestm"] == gh["biggestm"]
+    Eexp = pkg.expect(psi, mpo)
+    assert abs(Eexp - E) < 1e-10
+
+
+def test_heisenberg_L10_reaches_ed(pkg, ctx):
+    say("case L=10 ED")
+    L = 10
+    psi = pkg.randMPS_normal(L, 2, 32, seed=1, ctx=ctx)
+    mpo = pkg.makeMPO(pkg.heisenbergMPO(), 2, L)
+    E = pkg.dmrg(psi, mpo, m=32, sweeps=8, cutoff=1e-12)
+    say(" E", E)
+    assert abs(E - (-4.258035207282883)) < 1e-9
+
+
+def test_u1_heisenberg_L24_vs_oracle(pkg, ctx):
+    """U(1) Sz blocks, Neel start; per-sweep energies vs the QN oracle and
+    dense-vs-QN agreement (test/dmrg_test.jl:82-85)."""
+    say("case U(1) L=24 m=48")
+    L = 24
+    qphys = np.array([[1], [-1]])
+    states = [i % 2 for i in range(L)]
+    mpo_h = pkg.makeMPO(pkg.heisenbergMPO(), 2, L).H
+    qmpo = pkg.MPO(mpo_h, qphys=qphys)
+    psi = pkg.product_state(states, 2, qphys=qphys, ctx=ctx)
+    params = pkg.algvars(psi)
+    pkg.dmrg(psi, qmpo, m=48, sweeps=4, cutoff=1e-12, params=params)
+    opsi = omps.product_mps(states, 2, qphys=qphys)
+    oE, ohist = od.dmrg(opsi, om.makeMPO(om.heisenbergMPO(), 2, L), m=48, sweeps=4, cutoff=1e-12)
+    for h, oh in zip(params.history, ohist):
+        rel = abs(h["energy"] - oh["energy"]) / abs(oh["energy"])
+        say(" sweep", h["sweep"], "%.15f" % h["energy"], "%.15f" % oh["energy"], "rel %.2e" % rel,
+            "m", h["biggestm"], oh["biggestm"], "t=%.3fs" % h["time"])
+        assert rel <= 1e-12
+        assert h["biggestm"] == oh["biggestm"]
+        assert np.allclose(h["storeD"], oh["storeD"], rtol=1e-8, atol=1e-12)
+    # dense path on the same start agrees within the reference's tolerance
+    dpsi = pkg.product_state(states, 2, ctx=ctx)
+    Ed = pkg.dmrg(dpsi, pkg.makeMPO(pkg.heisenbergMPO(), 2, L), m=48, sweeps=4, cutoff=1e-12)
+    assert abs(Ed - params.energy) < 1e-4
+
+
+def test_hubbard_L10_u1xu1_reference_constant(pkg, ctx):
+    """test/model_test.jl:362-364: m=45 DMRG energy -25.380567747816 +- 1e-4."""
+    say("case Hubbard L=10 U(1)xU(1)")
+    L = 10
+    Cup, Cdn, F, Nup, Ndn, Ndens, O, Id = om.fermionOps()
+    opsi = omps.product_mps([0] * L, 4)
+    omps.apply_ops(opsi, list(range(1, 6)), Cup.T, trail=F)
+    omps.apply_ops(opsi, list(range(1, 6)), Cdn.T, trail=F)
+    qphys = np.array([[0, 0], [1, 1], [1, -1], [2, 0]])
+    omps.product_charges(opsi, qphys)
+    psi = pkg.MPS([a.copy() for a in opsi.A], oc=1, qphys=qphys, qbond=opsi.qbond, ctx=ctx, canonical=True)
+    mpo = pkg.MPO(pkg.makeMPO(pkg.hubbardMPO(), 4, L).H, qphys=qphys)
+    params = pkg.algvars(psi)
+    E = pkg.dmrg(psi, mpo, m=45, sweeps=10, cutoff=1e-12, params=params)
+    oE, ohist = od.dmrg(opsi, om.makeMPO(om.hubbardMPO(), 4, L), m=45, sweeps=10, cutoff=1e-12)
+    for h, oh in zip(params.history, ohist):
+        rel = abs(h["energy"] - oh["energy"]) / abs(oh["energy"])
+        say(" sweep", h["sweep"], "%.15f" % h["energy"], "%.15f" % oh["energy"], "rel %.2e" % rel)
+        assert rel <= 1e-12
+    assert abs(E - (-25.380567747816)) < 1e-4

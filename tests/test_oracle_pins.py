@@ -1,0 +1,225 @@
+"""Pins of the numpy oracle against exact diagonalisation, the reference's
+test constants and mathematical identities (CPU only)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import dmrg as od
+from oracle import models as om
+from oracle import mps as omps
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def ed_ground(mpo):
+    H = om.fullH(mpo)
+    assert np.allclose(H, H.T)
+    return np.linalg.eigh(H)
+
+
+def test_ed_heisenberg_L10_reference_constant():
+    # test/MPtask_test.jl:323: isapprox(D[1,1], -4.258035207282883)
+    E, _ = ed_ground(om.makeMPO(om.heisenbergMPO(), 2, 10))
+    assert np.isclose(E[0], -4.258035207282883, rtol=1e-13, atol=0)
+
+
+def test_ed_tiny_closed_forms():
+    # two spins: singlet -3/4; three spins (open): -1
+    assert np.isclose(ed_ground(om.makeMPO(om.heisenbergMPO(), 2, 2))[0][0], -0.75)
+    assert np.isclose(ed_ground(om.makeMPO(om.heisenbergMPO(), 2, 3))[0][0], -1.0)
+    # four spins (open): -(3/4 + sqrt(3)/2)
+    assert np.isclose(ed_ground(om.makeMPO(om.heisenbergMPO(), 2, 4))[0][0], -(0.75 + np.sqrt(3) / 2))
+
+
+def test_hubbard_two_site_closed_form():
+    # two-site Hubbard at half filling: E = (U - sqrt(U^2+16 t^2))/2 + 2 mu
+    t, U, mu = 1.0, 4.0, -2.0
+    E, V = ed_ground(om.makeMPO(om.hubbardMPO(t=t, mu=mu, HubU=U), 4, 2))
+    n = np.array([0, 1, 1, 2])
+    N = (n[:, None] + n[None, :]).T.ravel()  # site 1 fastest
+    H = om.fullH(om.makeMPO(om.hubbardMPO(t=t, mu=mu, HubU=U), 4, 2))
+    sel = np.nonzero(N == 2)[0]
+    e2 = np.linalg.eigvalsh(H[np.ix_(sel, sel)])[0]
+    assert np.isclose(e2, 0.5 * (U - np.sqrt(U * U + 16 * t * t)) + 2 * mu, rtol=1e-13)
+
+
+def rand(shape, rng):
+    return rng.standard_normal(shape)
+
+
+def test_env_and_heff_against_brute_force():
+    rng = np.random.default_rng(1)
+    a, b, c, d, D = 3, 4, 5, 2, 5
+    L = rand((a, D, a), rng)
+    R = rand((c, D, c), rng)
+    A1 = rand((a, d, b), rng)
+    A2 = rand((b, d, c), rng)
+    W1 = rand((D, d, d, D), rng)
+    W2 = rand((D, d, d, D), rng)
+    bf = np.einsum('xwa,wjkv,vluy,ajlb,byz->xkuz', L, W1, W2, np.einsum('ajc,clb->ajlb', A1, A2), R)
+    assert np.allclose(od.make2site(L, R, A1, A2, W1, W2), bf, rtol=1e-12, atol=1e-12)
+    th = np.tensordot(A1, A2, axes=(2, 0))
+    assert np.allclose(od.heff2(L, R, th, W1, W2), bf, rtol=1e-12, atol=1e-12)
+    assert np.allclose(od.twosite_update(L, R, A1, A2, W1, W2), bf, rtol=1e-12, atol=1e-12)
+    Lb = np.einsum('xwa,xkB,wjku,ajb->Bub', L, A1, W1, A1)
+    assert np.allclose(od.Lupdate(L, A1, W1), Lb, rtol=1e-12, atol=1e-12)
+    Rb = np.einsum('ajb,wjku,xkB,buB->awx', A2, W2, A2, R)
+    assert np.allclose(od.Rupdate(R, A2, W2), Rb, rtol=1e-12, atol=1e-12)
+    R1 = od.Rupdate(R, A2, W2)
+    H1 = np.einsum('xwa,wjku,ajb,buB->xkB', L, W1, A1, R1)
+    assert np.allclose(od.singlesite_update(L, R1, A1, W1), H1, rtol=1e-12, atol=1e-12)
+
+
+def test_singlesite_energy_equals_expect_and_env_consistency():
+    # test/MPmeas_test.jl:158-206
+    L = 8
+    mpo = om.makeMPO(om.heisenbergMPO(), 2, L)
+    psi = omps.seeded_normal_mps(L, 2, 16, seed=3)
+    omps.move(psi, 4)
+    E = od.expect(psi, mpo)
+    Lenv, Renv = od.makeEnv(psi, mpo)
+    H1 = od.singlesite_update(Lenv[4], Renv[4], psi[4], mpo[3])
+    nrm = np.vdot(psi[4], psi[4])
+    assert np.isclose(np.vdot(psi[4], H1) / nrm, E / nrm, rtol=1e-12)
+    # hermiticity of H_eff on the two-site space
+    x = np.random.default_rng(0).standard_normal(np.tensordot(psi[4], psi[5], axes=(2, 0)).shape)
+    y = np.random.default_rng(1).standard_normal(x.shape)
+    Hx = od.heff2(Lenv[4], Renv[5], x, mpo[3], mpo[4])
+    Hy = od.heff2(Lenv[4], Renv[5], y, mpo[3], mpo[4])
+    assert np.isclose(np.vdot(y, Hx), np.vdot(x, Hy), rtol=1e-12)
+    # Lupdate chain equals makeEnv after moving the centre
+    omps.move(psi, L)
+    Lenv2, _ = od.makeEnv(psi, mpo)
+    for i in range(1, L):
+        assert np.allclose(Lenv2[i + 1], od.Lupdate(Lenv2[i], psi[i], mpo[i - 1]))
+    # full left contraction equals the energy
+    Lfull = od.Lupdate(Lenv2[L], psi[L], mpo[L - 1])
+    assert np.isclose(Lfull[0, 0, 0], od.expect(psi, mpo), rtol=1e-12)
+
+
+def test_eig2_closed_form():
+    for (a1, b1, a2) in [(-1.0, 0.3, 2.0), (1.0, 1e-8, 1.0), (-3.0, 2.0, -3.5)]:
+        e, u, v = od.eig2(a1, b1, a2)
+        lam = 0.5 * (a1 + a2) - np.sqrt((0.5 * (a1 - a2)) ** 2 + b1 * b1)
+        assert np.isclose(e, lam, rtol=1e-14, atol=1e-15)
+        M = np.array([[a1, b1], [b1, a2]])
+        assert np.allclose(M @ np.array([u, v]), lam * np.array([u, v]), atol=1e-13)
+
+
+def test_fsvd_invariants_dense_and_qn():
+    rng = np.random.default_rng(7)
+    AA = rng.standard_normal((6, 2, 2, 5))
+    U, s, V, terr, _ = od.fsvd(AA, m=4, minm=2, cutoff=0.0)
+    full = np.linalg.svd(AA.reshape(12, 10), compute_uv=False)
+    assert np.allclose(s * np.sqrt(np.sum(full[:4] ** 2)), full[:4])
+    assert np.isclose(terr, np.sum(full[4:] ** 2))
+    Um = U.reshape(12, -1)
+    assert np.allclose(Um.T @ Um, np.eye(4), atol=1e-13)
+    rec = np.tensordot(U * (s * np.sqrt(np.sum(full[:4] ** 2)))[None, None, :], V, axes=(2, 0))
+    assert np.isclose(np.sum((rec - AA) ** 2), terr, rtol=1e-10)
+    # cutoff drops the tail whose summed weight stays below it
+    AA /= np.linalg.norm(AA)
+    sv = np.linalg.svd(AA.reshape(12, 10), compute_uv=False)
+    cut = sv[-1] ** 2 + sv[-2] ** 2 + 1e-16
+    _, s2, _, terr2, _ = od.fsvd(AA, m=0, minm=2, cutoff=cut)
+    assert len(s2) == len(sv) - 2 and np.isclose(terr2, sv[-1] ** 2 + sv[-2] ** 2)
+    # charged tensor: blocks found per sector equal the dense spectrum
+    qL = np.array([[-1], [0], [0], [1]])
+    qP = np.array([[1], [-1]])
+    qR = np.array([[-1], [0], [1]])
+    T = np.zeros((4, 2, 2, 3))
+    for a in range(4):
+        for s1 in range(2):
+            for s2 in range(2):
+                for b in range(3):
+                    if qL[a, 0] + qP[s1, 0] + qP[s2, 0] == qR[b, 0]:
+                        T[a, s1, s2, b] = rng.standard_normal()
+    U, s, V, terr, qn = od.fsvd(T, m=0, minm=2, cutoff=0.0, qL=qL, qP=qP, qR=qR)
+    dense = np.linalg.svd(T.reshape(8, 6), compute_uv=False)
+    dense = dense[dense > 1e-12]
+    assert np.allclose(np.sort(s * np.linalg.norm(dense))[::-1], np.sort(dense)[::-1][:len(s)])
+
+
+def test_dmrg_L10_converges_to_ed():
+    mpo = om.makeMPO(om.heisenbergMPO(), 2, 10)
+    psi = omps.seeded_normal_mps(10, 2, 32, seed=0)
+    E, hist = od.dmrg(psi, mpo, m=32, sweeps=8, cutoff=1e-12)
+    assert abs(E - (-4.258035207282883)) < 1e-9
+    assert np.isclose(od.expect(psi, mpo), E, rtol=1e-10)
+
+
+def test_dmrg_dense_vs_qn_heisenberg():
+    # test/dmrg_test.jl:82-85 (tighter: same algorithm, equal m)
+    L = 12
+    H = om.heisenbergMPO()
+    mpo = om.makeMPO(H, 2, L)
+    qphys = np.array([[1], [-1]])
+    states = [0 if i % 2 == 0 else 1 for i in range(L)]
+    p1 = omps.product_mps(states, 2)
+    p2 = omps.product_mps(states, 2, qphys=qphys)
+    E1, _ = od.dmrg(p1, mpo, m=24, sweeps=6, cutoff=1e-12)
+    E2, _ = od.dmrg(p2, mpo, m=24, sweeps=6, cutoff=1e-12)
+    Eex = np.linalg.eigvalsh(om.fullH(mpo))[0]
+    assert abs(E1 - E2) < 1e-4
+    assert abs(E2 - Eex) < 1e-6
+
+
+def test_hubbard_L10_reference_constant():
+    # test/model_test.jl:362-364 (m=45 DMRG within 1e-4 of -25.380567747816)
+    L = 10
+    Cup, Cdn, F, Nup, Ndn, Ndens, O, Id = om.fermionOps()
+    mpo = om.makeMPO(om.hubbardMPO(), 4, L)
+    psi = omps.product_mps([0] * L, 4)
+    omps.apply_ops(psi, list(range(1, 6)), Cup.T, trail=F)
+    omps.apply_ops(psi, list(range(1, 6)), Cdn.T, trail=F)
+    qphys = np.array([[0, 0], [1, 1], [1, -1], [2, 0]])
+    omps.product_charges(psi, qphys)
+    E, _ = od.dmrg(psi, mpo, m=45, sweeps=10, cutoff=1e-12)
+    assert abs(E - (-25.380567747816)) < 1e-4
+
+
+def test_mpo_deltas_conserve_charge():
+    qphys = np.array([[1], [-1]])
+    mpo = om.makeMPO(om.heisenbergMPO(), 2, 6)
+    deltas = omps.mpo_deltas(mpo, qphys)
+    assert deltas[0].tolist() == [[0]] and deltas[-1].tolist() == [[0]]
+    qh = np.array([[0, 0], [1, 1], [1, -1], [2, 0]])
+    d2 = omps.mpo_deltas(om.makeMPO(om.hubbardMPO(), 4, 6), qh)
+    assert d2[-1].tolist() == [[0, 0]]
+
+
+def test_j1j2_mpo_matches_explicit_sum():
+    Lx, Ly = 3, 2
+    mpo, bonds = om.j1j2_cylinder_mpo(Lx, Ly, J1=1.0, J2=0.5)
+    H = om.fullH(mpo)
+    Sp, Sm, Sz, O, Id = om.spinOps()
+    N = Lx * Ly
+
+    def op_at(o, i):
+        out = np.ones((1, 1))
+        for s in range(N - 1, -1, -1):   # site 1 fastest
+            out = np.kron(out, o if s == i else Id)
+        return out
+
+    Hex = np.zeros((2 ** N, 2 ** N))
+    for (a, b), J in bonds.items():
+        Hex += J * (op_at(Sz, a) @ op_at(Sz, b) + 0.5 * op_at(Sp, a) @ op_at(Sm, b)
+                    + 0.5 * op_at(Sm, a) @ op_at(Sp, b))
+    assert np.allclose(H, Hex, atol=1e-12)
+
+
+def test_cfg1_golden_oracle_regression():
+    """The committed per-sweep report of the L=20, m=64 run (seed 0 N(0,1)
+    initial state, 6 sweeps, cutoff 1e-12) is reproduced by the oracle."""
+    with open(os.path.join(GOLD, "cfg1.json")) as f:
+        g = json.load(f)
+    psi = omps.seeded_normal_mps(20, 2, 64, seed=0)
+    assert np.allclose(psi[1].ravel()[:4], g["init_site1_first4"], rtol=1e-14)
+    mpo = om.makeMPO(om.heisenbergMPO(), 2, 20)
+    E, hist = od.dmrg(psi, mpo, m=64, sweeps=6, cutoff=1e-12)
+    for h, gh in zip(hist, g["sweeps"]):
+        assert abs(h["energy"] - gh["energy"]) <= 1e-12 * abs(gh["energy"])
+        assert np.allclose(h["storeD"], gh["storeD"], rtol=1e-9, atol=1e-12)
+    assert abs(E - g["ed_energy"]) < 1e-7
