@@ -1,0 +1,186 @@
+"""Benchmark: two-site DMRG sweeps on one MI355X.
+
+Workload (the metric's configuration): open spin-1/2 Heisenberg chain,
+L=100, J=1, d=2, MPO bond dimension 5, U(1) Sz block-sparse tensors (total
+Sz = 0), seeded random product-state start (a permutation of 50 up / 50 down
+from numpy default_rng(0)), bond dimension ramped 128 -> 256 -> 512 -> 1024,
+two-vector Lanczos (Krylov 2), cutoff 1e-12, minm 2.
+
+A "step" is one full DMRG sweep (2(L-1) two-site bond steps) at m = 1024.
+Warmup = the ramp sweeps plus --warmup sweeps at m = 1024; exactly --steps
+sweeps are timed between two stream synchronisations.  All tensors are
+resident in HBM before the timed region.
+
+Output: one JSON line.  value = algorithmic TFLOP/s of the timed sweeps
+(H_eff*psi plus environment updates, block-sparse counts, FP64-equivalent),
+ms_per_step = sweep wall time.  roofline = the dominant kernel (FP64 grouped
+GEMM) from HIP events on the library's stream.  cpu_baseline = the numpy
+oracle's block-sparse H_eff*psi on the host cores, timed on a bounded sample
+of the same state (the centre bond at m = 1024).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64 = 78.6      # TFLOP/s, MI355X dense FP64 matrix
+PEAK_BF16 = 2500.0    # TFLOP/s, MI355X dense BF16 matrix
+PEAK_HBM = 8000.0     # GB/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--L", type=int, default=100)
+    ap.add_argument("--m", type=int, default=1024)
+    ap.add_argument("--ramp", default="128,256,512")
+    ap.add_argument("--precision", default="f64")
+    ap.add_argument("--max-grow", type=int, default=12)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import __graft_entry__ as ge
+    pkg = ge.load_package()
+    ctx = pkg.default_context()
+    L, m = args.L, args.m
+    rng = np.random.default_rng(0)
+    states = list(rng.permutation([0] * (L // 2) + [1] * (L - L // 2)))
+    qphys = np.array([[1], [-1]])
+    mpo = pkg.MPO(pkg.makeMPO(pkg.heisenbergMPO(), 2, L).H, qphys=qphys)
+    psi = pkg.product_state(states, 2, qphys=qphys, ctx=ctx)
+    params = pkg.algvars(psi)
+    t_ramp = time.perf_counter()
+    for mr in [int(x) for x in args.ramp.split(",") if x]:
+        pkg.dmrg(psi, mpo, m=mr, sweeps=1, cutoff=1e-12, params=params, precision=args.precision)
+        log("ramp m=%d E=%.12f maxm=%d t=%.2fs" % (mr, params.energy, params.history[-1]["biggestm"],
+                                                   params.history[-1]["time"]))
+    # keep sweeping at the target m until the bond dimension has grown to it
+    for _ in range(args.max_grow):
+        if params.history and params.history[-1]["biggestm"] >= int(0.95 * m):
+            break
+        pkg.dmrg(psi, mpo, m=m, sweeps=1, cutoff=1e-12, params=params, precision=args.precision)
+        log("grow m=%d E=%.12f maxm=%d t=%.2fs" % (m, params.energy, params.history[-1]["biggestm"],
+                                                   params.history[-1]["time"]))
+    for _ in range(args.warmup):
+        pkg.dmrg(psi, mpo, m=m, sweeps=1, cutoff=1e-12, params=params, precision=args.precision)
+        log("warmup m=%d E=%.12f maxm=%d t=%.2fs" % (m, params.energy, params.history[-1]["biggestm"],
+                                                     params.history[-1]["time"]))
+    log("ramp+warmup %.1fs" % (time.perf_counter() - t_ramp))
+
+    ctx.profile(True)
+    ctx.sync()
+    ctx.flops(reset=True)
+    for k in ("gemm_f64", "mix", "copy", "svd"):
+        ctx.kernel_stats(k, reset=True)
+    t0 = time.perf_counter()
+    pkg.dmrg(psi, mpo, m=m, sweeps=args.steps, cutoff=1e-12, params=params, precision=args.precision)
+    ctx.sync()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    fl_heff, fl_env = ctx.flops()
+    stats = {k: ctx.kernel_stats(k) for k in ("gemm_f64", "mix", "copy", "svd")}
+    ctx.profile(False)
+    for h in params.history[-args.steps:]:
+        log("timed sweep E=%.12f maxtrunc=%.3e m=%d t=%.3fs" % (h["energy"], h["maxtrunc"], h["biggestm"],
+                                                               h["time"]))
+    sweep_s = elapsed / args.steps
+    tflops = (fl_heff + fl_env) / elapsed / 1e12
+    g = stats["gemm_f64"]
+    g_tflops = g["flops"] / (g["ms"] * 1e-3) / 1e12 if g["ms"] > 0 else 0.0
+    g_avg_ms = g["ms"] / max(1, g["launches"])
+    log("gemm launches=%d avg=%.4f ms total=%.3f s, %.2f TFLOP/s; mix %.3f s %.0f GB/s; copy %.3f s; "
+        "svd %.3f s" % (g["launches"], g_avg_ms, g["ms"] / 1e3, g_tflops, stats["mix"]["ms"] / 1e3,
+                        stats["mix"]["bytes"] / max(1e-9, stats["mix"]["ms"] * 1e-3) / 1e9,
+                        stats["copy"]["ms"] / 1e3, stats["svd"]["ms"] / 1e3))
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(pkg, psi, mpo, params, args.cpu_seconds)
+    out = {
+        "metric": "DMRG sweep time + H_eff*psi TFLOP/s (FP64 GPU drop-in, two-site, U(1))",
+        "value": round(tflops, 4),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sweep_s * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "f64" else args.precision,
+        "data": "synthetic (seeded random product state, Heisenberg MPO)",
+        "config": {"workload": "Heisenberg L=%d d=2 D=5 U(1) Sz=0, two-site DMRG sweep at m=%d "
+                               "(ramp %s), Krylov 2, cutoff 1e-12" % (L, m, args.ramp),
+                   "L": L, "m": m, "energy": params.energy,
+                   "heff_flops_per_sweep": fl_heff / args.steps, "env_flops_per_sweep": fl_env / args.steps,
+                   "sweep_s": sweep_s},
+        "roofline": {"bound": "mfma", "kernel": "k_gemm_f64 (FP64 grouped GEMM, MFMA 16x16x4)",
+                     "achieved": round(g_tflops, 3), "peak": PEAK_FP64, "unit": "TFLOP/s",
+                     "frac": round(g_tflops / PEAK_FP64, 4), "traffic": None,
+                     "avg_launch_ms": round(g_avg_ms, 5), "launches": g["launches"],
+                     "flops_per_launch": g["flops"] / max(1, g["launches"])},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(pkg, psi, mpo, params, budget_s):
+    """Time the oracle's block-sparse H_eff*psi at the centre bond of the
+    current (m = 1024) state on the host cores.  The GPU state is first moved
+    to the centre (untimed bond steps) so that both environments are current."""
+    import multiprocessing
+    from oracle import blocks as ob
+    L = len(psi)
+    c = L // 2
+    j = 1
+    while psi.oc < c:
+        i = psi.oc
+        pkg.twostep(0, j, i, i, i + 1, psi, mpo, params.Lenv, params.Renv, params)
+        params.Lenv[i + 1] = pkg.Lupdate(params.Lenv[i], psi[i], mpo, i)
+        psi.oc += 1
+    nq = psi.nq
+    Lh, qa, _ = params.Lenv[c].to_dense(nq)
+    Rh, qb, _ = params.Renv[c + 1].to_dense(nq)
+    A1, ql, _ = psi[c].to_dense(nq)
+    A2, _, qr = psi[c + 1].to_dense(nq)
+    theta = np.tensordot(A1, A2, axes=(2, 0))
+    plan = ob.Heff2Plan(Lh, Rh, mpo.H[c - 1], mpo.H[c], ql, psi.qphys, qr, mpo.deltas[c - 1],
+                        mpo.deltas[c + 1])
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        plan.apply(theta)
+        n += 1
+        if time.perf_counter() - t0 > budget_s or n >= 50:
+            break
+    dt = (time.perf_counter() - t0) / n
+    tf = plan.flops / dt / 1e12
+    cores = int(os.environ.get("OMP_NUM_THREADS", multiprocessing.cpu_count()))
+    log("cpu baseline: %d H_eff applications at bond %d|%d (chi %d x %d), %.3f s each, %.4f TFLOP/s"
+        % (n, c, c + 1, theta.shape[0], theta.shape[3], dt, tf))
+    return {"value": round(tf, 5), "unit": "TFLOP/s", "cores": cores, "kind": "port",
+            "sample": "%d block-sparse H_eff*psi applications at the centre bond %d|%d (chi %dx%d) of the "
+                      "m=%d state, numpy/OpenBLAS oracle, %.3f s each" % (n, c, c + 1, theta.shape[0],
+                                                                          theta.shape[3], params.maxm, dt)}
+
+
+if __name__ == "__main__":
+    main()

@@ -30,6 +30,9 @@ SIGNATURES = {
     "dmrj_ctx_memstats": [_p, _i64p],
     "dmrj_ctx_set_gemm_cfg": [_p, C.c_int],
     "dmrj_ctx_flops": [_p, _dp, C.c_int],
+    "dmrj_ctx_profile": [_p, C.c_int],
+    "dmrj_ctx_set_svd_backend": [_p, C.c_int],
+    "dmrj_ctx_kernel_stats": [_p, C.c_int, _dp, C.c_int],
     "dmrj_site_from_dense": [_p, C.c_int, _i64, _i64, _i64, _dp, _i32p, _i32p, _i32p, C.POINTER(_p)],
     "dmrj_site2_from_dense": [_p, C.c_int, _i64, _i64, _i64, _i64, _dp, _i32p, _i32p, _i32p, _i32p,
                               C.POINTER(_p)],
@@ -133,6 +136,19 @@ class Context:
         out = (C.c_double * 2)()
         self.call("dmrj_ctx_flops", out, 1 if reset else 0)
         return out[0], out[1]
+
+    def profile(self, enable=True):
+        self.call("dmrj_ctx_profile", 1 if enable else 0)
+
+    KINDS = {"gemm_f64": 0, "mix": 1, "copy": 2, "svd": 3, "gemm_lp": 4}
+
+    def kernel_stats(self, kind, reset=False):
+        out = (C.c_double * 4)()
+        self.call("dmrj_ctx_kernel_stats", self.KINDS.get(kind, kind), out, 1 if reset else 0)
+        return dict(launches=int(out[0]), ms=out[1], flops=out[2], bytes=out[3])
+
+    def set_svd_backend(self, mode):
+        self.call("dmrj_ctx_set_svd_backend", int(mode))
 
     def set_gemm_cfg(self, cfg):
         self.call("dmrj_ctx_set_gemm_cfg", int(cfg))

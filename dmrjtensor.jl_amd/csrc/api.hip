@@ -179,6 +179,52 @@ void Context::stage_reset() { stage.used = 0; }
 void Context::release_transients() {
   for (void* p : transient) alloc.release(p);
   transient.clear();
+  prof_resolve();
+}
+
+int Context::prof_begin() {
+  if (!prof) return -1;
+  while (prof_pool.size() < 2) {
+    hipEvent_t e;
+    HIP_OK(hipEventCreate(&e));
+    prof_pool.push_back(e);
+  }
+  ProfRec r;
+  r.a = prof_pool.back();
+  prof_pool.pop_back();
+  r.b = prof_pool.back();
+  prof_pool.pop_back();
+  r.kind = -1;
+  r.flops = r.bytes = 0.0;
+  HIP_OK(hipEventRecord(r.a, stream));
+  prof_pending.push_back(r);
+  return (int)prof_pending.size() - 1;
+}
+
+void Context::prof_end(int slot, int kind, double flops, double bytes) {
+  if (slot < 0) return;
+  ProfRec& r = prof_pending[slot];
+  r.kind = kind;
+  r.flops = flops;
+  r.bytes = bytes;
+  HIP_OK(hipEventRecord(r.b, stream));
+}
+
+void Context::prof_resolve() {
+  for (auto& r : prof_pending) {
+    if (r.kind >= 0 && r.kind < 8) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+        prof_ms[r.kind] += ms;
+        prof_flops[r.kind] += r.flops;
+        prof_bytes[r.kind] += r.bytes;
+        prof_count[r.kind] += 1;
+      }
+    }
+    prof_pool.push_back(r.a);
+    prof_pool.push_back(r.b);
+  }
+  prof_pending.clear();
 }
 
 }  // namespace dmrj
@@ -825,16 +871,88 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
   auto* hb = (rocblas_handle)c.rocblas;
   rocblas_int* info = (rocblas_int*)c.alloc.alloc(sizeof(rocblas_int) * std::max<size_t>(1, secs.size()));
   int nsvd = 0;
+  const int psvd = c.prof_begin();
+  double svd_flops = 0.0;
+  // small sectors: one batched Jacobi launch; large ones: rocSOLVER gesdd
+  std::vector<SmallSvdJob> small;
+  size_t small_lds = 0;
+  std::vector<char> is_small(secs.size(), 0);
   for (int si : order) {
     Sec& s = secs[si];
-    if (s.kmin == 0) continue;
-    rocblas_status st = rocsolver_dgesvd(hb, rocblas_svect_singular, rocblas_svect_singular, (rocblas_int)s.R,
-                                         (rocblas_int)s.C, s.M, (rocblas_int)s.R, s.s, s.U,
-                                         (rocblas_int)s.R, s.Vt, (rocblas_int)s.kmin, s.E,
-                                         rocblas_outofplace, info + nsvd);
-    if (st != rocblas_status_success) throw Error(DMRJ_ERR_SOLVER, "rocsolver_dgesvd failed");
-    ++nsvd;
+    if (s.kmin == 0 || s.kmin > 64 || c.svd_backend == 1 || c.svd_backend == 2) continue;
+    const size_t l = small_svd_lds((int)s.R, (int)s.C);
+    if (l > kSmallSvdMaxLds) continue;
+    is_small[si] = 1;
+    small_lds = std::max(small_lds, l);
+    small.push_back({s.M, s.U, s.s, s.Vt, (int)s.R, (int)s.C});
+    const double mm = (double)std::max(s.R, s.C), nn = (double)std::min(s.R, s.C);
+    svd_flops += 4.0 * mm * nn * nn + 8.0 * nn * nn * nn;
   }
+  launch_svd_small(c, small, small_lds);
+  // large sectors, default backend: one batched symmetric eigensolve of the
+  // augmented matrices [[0, M], [M^T, 0]] (eigenvalues +-sigma, absolute
+  // accuracy eps |M| like a bidiagonal SVD, no squaring of M)
+  double* aug = nullptr;
+  double* augW = nullptr;
+  double* augE = nullptr;
+  rocblas_int* auginfo = nullptr;
+  int naug = 0;
+  if (c.svd_backend == 0) {
+    std::vector<int> big;
+    int N = 0;
+    for (int si : order) {
+      Sec& s = secs[si];
+      if (s.kmin == 0 || is_small[si]) continue;
+      big.push_back(si);
+      N = std::max<int>(N, (int)(s.R + s.C));
+    }
+    if (!big.empty()) {
+      naug = (int)big.size();
+      const int64_t NN = (int64_t)N * N;
+      aug = (double*)c.alloc.alloc(NN * naug * sizeof(double));
+      augW = (double*)c.alloc.alloc((int64_t)N * naug * sizeof(double));
+      augE = (double*)c.alloc.alloc((int64_t)N * naug * sizeof(double));
+      auginfo = (rocblas_int*)c.alloc.alloc(sizeof(rocblas_int) * naug);
+      launch_fill(c, aug, NN * naug, 0.0);
+      std::vector<CopyOp> place;
+      for (int b = 0; b < naug; ++b) {
+        Sec& s = secs[big[b]];
+        place.push_back({s.M, aug + b * NN + s.R * (int64_t)N, s.R, N, (int)s.R, (int)s.C, nullptr, nullptr, 1.0});
+      }
+      launch_copy(c, place);
+      rocblas_status st = rocsolver_dsyevd_strided_batched(hb, rocblas_evect_original, rocblas_fill_upper, N, aug, N,
+                                                           NN, augW, N, augE, N, auginfo, naug);
+      if (st != rocblas_status_success) throw Error(DMRJ_ERR_SOLVER, "rocsolver_dsyevd_strided_batched failed");
+      std::vector<AugJob> jobs;
+      for (int b = 0; b < naug; ++b) {
+        Sec& s = secs[big[b]];
+        jobs.push_back({aug + b * NN, augW + (int64_t)b * N, s.U, s.s, s.Vt, (int)s.R, (int)s.C, N, s.kmin});
+        is_small[big[b]] = 1;  // handled
+        const double n2 = (double)N;
+        svd_flops += 9.0 * n2 * n2 * n2;
+      }
+      launch_aug_extract(c, jobs);
+    }
+  }
+  for (int si : order) {
+    Sec& s = secs[si];
+    if (s.kmin == 0 || is_small[si]) continue;
+    rocblas_status st;
+    if (c.svd_backend == 2 || c.svd_backend == 3)
+      st = rocsolver_dgesvd(hb, rocblas_svect_singular, rocblas_svect_singular, (rocblas_int)s.R,
+                            (rocblas_int)s.C, s.M, (rocblas_int)s.R, s.s, s.U, (rocblas_int)s.R, s.Vt,
+                            (rocblas_int)s.kmin, s.E, rocblas_outofplace, info + nsvd);
+    else  // divide-and-conquer SVD, one call per centre-charge sector
+      st = rocsolver_dgesdd(hb, rocblas_svect_singular, rocblas_svect_singular, (rocblas_int)s.R,
+                            (rocblas_int)s.C, s.M, (rocblas_int)s.R, s.s, s.U, (rocblas_int)s.R, s.Vt,
+                            (rocblas_int)s.kmin, info + nsvd);
+    if (st != rocblas_status_success) throw Error(DMRJ_ERR_SOLVER, "rocsolver SVD failed");
+    ++nsvd;
+    // nominal Golub-Kahan-Reinsch count with both singular-vector sets: 4 m n^2 + 8 n^3 (m >= n)
+    const double mm = (double)std::max(s.R, s.C), nn = (double)std::min(s.R, s.C);
+    svd_flops += 4.0 * mm * nn * nn + 8.0 * nn * nn * nn;
+  }
+  c.prof_end(psvd, KIND_SVD, svd_flops, 0.0);
   // singular values to host
   std::vector<double> hs(std::max<int64_t>(total_s, 1));
   {
@@ -848,11 +966,20 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
   }
   std::vector<rocblas_int> hinfo(std::max(1, nsvd));
   if (nsvd) HIP_OK(hipMemcpyAsync(hinfo.data(), info, nsvd * sizeof(rocblas_int), hipMemcpyDeviceToHost, c.stream));
+  std::vector<rocblas_int> hainfo(std::max(1, naug));
+  if (naug)
+    HIP_OK(hipMemcpyAsync(hainfo.data(), auginfo, naug * sizeof(rocblas_int), hipMemcpyDeviceToHost, c.stream));
   HIP_OK(hipStreamSynchronize(c.stream));
+  c.alloc.release(aug);
+  c.alloc.release(augW);
+  c.alloc.release(augE);
+  c.alloc.release(auginfo);
+  for (int i = 0; i < naug; ++i)
+    if (hainfo[i] != 0) throw Error(DMRJ_ERR_SOLVER, "rocsolver_dsyevd did not converge");
   c.release_transients();
   c.stage_reset();
   for (int i = 0; i < nsvd; ++i)
-    if (hinfo[i] != 0) throw Error(DMRJ_ERR_SOLVER, "rocsolver_dgesvd did not converge");
+    if (hinfo[i] != 0) throw Error(DMRJ_ERR_SOLVER, "rocsolver SVD did not converge");
   // global sort: sigma descending, ties by sector order then position
   struct Ent {
     double v;
@@ -1119,10 +1246,40 @@ int dmrj_ctx_memstats(dmrj_ctx* c, int64_t* out3) {
   });
 }
 
+int dmrj_ctx_set_svd_backend(dmrj_ctx* c, int mode) {
+  return guard(c, [&] {
+    if (mode < 0 || mode > 3) throw Error(DMRJ_ERR_ARG, "svd backend must be 0..3");
+    c->svd_backend = mode;
+  });
+}
+
 int dmrj_ctx_set_gemm_cfg(dmrj_ctx* c, int cfg) {
   return guard(c, [&] {
     if (cfg < -1 || cfg > 2) throw Error(DMRJ_ERR_ARG, "gemm cfg must be -1..2");
     c->gemm_cfg_override = cfg;
+  });
+}
+
+int dmrj_ctx_profile(dmrj_ctx* c, int enable) {
+  return guard(c, [&] {
+    HIP_OK(hipStreamSynchronize(c->stream));
+    c->release_transients();
+    c->stage_reset();
+    c->prof = enable != 0;
+  });
+}
+
+int dmrj_ctx_kernel_stats(dmrj_ctx* c, int kind, double* out4, int reset) {
+  return guard(c, [&] {
+    if (kind < 0 || kind >= 8 || !out4) throw Error(DMRJ_ERR_ARG, "kernel kind 0..7");
+    HIP_OK(hipStreamSynchronize(c->stream));
+    c->release_transients();
+    c->stage_reset();
+    out4[0] = c->prof_count[kind];
+    out4[1] = c->prof_ms[kind];
+    out4[2] = c->prof_flops[kind];
+    out4[3] = c->prof_bytes[kind];
+    if (reset) c->prof_count[kind] = c->prof_ms[kind] = c->prof_flops[kind] = c->prof_bytes[kind] = 0.0;
   });
 }
 

@@ -191,9 +191,26 @@ struct Context {
   int num_cu = 256;
   // tuning: preferred GEMM tile config per operation (-1 = automatic)
   int gemm_cfg_override = -1;
+  // SVD backend: 0 = batched Jacobi for small sectors + batched symmetric
+  // eigensolver of the augmented matrix for large ones, 1 = gesdd,
+  // 2 = gesvd, 3 = batched Jacobi for small sectors + gesvd
+  int svd_backend = 0;
   int last_gemm_cfg = -1;
   // statistics
   double flops_heff = 0.0, flops_env = 0.0;
+  // per-kernel-kind HIP-event timing on this context's stream
+  bool prof = false;
+  struct ProfRec {
+    hipEvent_t a, b;
+    int kind;
+    double flops, bytes;
+  };
+  std::vector<ProfRec> prof_pending;
+  std::vector<hipEvent_t> prof_pool;
+  double prof_ms[8] = {0}, prof_flops[8] = {0}, prof_bytes[8] = {0}, prof_count[8] = {0};
+  int prof_begin();                                // returns slot or -1
+  void prof_end(int slot, int kind, double flops, double bytes);
+  void prof_resolve();                             // after a stream sync
   void* upload(const void* host, size_t bytes);   // staged async copy, returns device ptr
   void stage_reset();
   void release_transients();
@@ -232,7 +249,29 @@ void launch_comb_dev(Context& c, double* out, const double* a, const double* b, 
 void launch_lz_scalar(Context& c, int stage);
 void launch_mfma_probe(Context& c, double* out);
 int choose_gemm_cfg(const std::vector<GemmTask>& tasks, int num_cu);
+struct SmallSvdJob {
+  const double* M;
+  double* U;
+  double* S;
+  double* Vt;
+  int R, C;
+};
+void launch_svd_small(Context& c, const std::vector<SmallSvdJob>& jobs, size_t lds_bytes);
+struct AugJob {
+  const double* A;
+  const double* W;
+  double* U;
+  double* S;
+  double* Vt;
+  int R, C, N, kmin;
+};
+void launch_aug_extract(Context& c, const std::vector<AugJob>& jobs);
+size_t small_svd_lds(int R, int C);
+constexpr size_t kSmallSvdMaxLds = 60 * 1024;  // bytes of dynamic LDS per sector
 double gemm_flops(const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs);
+
+// kernel kinds for Context profiling
+enum KernelKind { KIND_GEMM_F64 = 0, KIND_MIX = 1, KIND_COPY = 2, KIND_SVD = 3, KIND_GEMM_LP = 4 };
 
 // Lanczos scalar slots in Context::scal
 enum Slot {

@@ -404,6 +404,7 @@ void launch_gemm(Context& c, const std::vector<GemmTask>& tasks, const std::vect
   const int occ = cfg == 2 ? 2 : 4;
   const int grid = (int)std::min<int64_t>((int64_t)tiles.size(), (int64_t)c.num_cu * occ);
   const int nt = (int)tiles.size();
+  const int ps = c.prof_begin();
   if (cfg == 0)
     hipLaunchKernelGGL((k_gemm_f64<32, 32>), dim3(grid), dim3(kThreads), 0, c.stream, dt, nt, dk, ds);
   else if (cfg == 1)
@@ -411,6 +412,17 @@ void launch_gemm(Context& c, const std::vector<GemmTask>& tasks, const std::vect
   else
     hipLaunchKernelGGL((k_gemm_f64<128, 128>), dim3(grid), dim3(kThreads), 0, c.stream, dt, nt, dk, ds);
   HIP_OK(hipGetLastError());
+  if (ps >= 0) {
+    double bytes = 0.0;  // compulsory: every operand block read once, C written once
+    for (const auto& t : tasks) {
+      bytes += 8.0 * t.M * (double)t.N;
+      for (int q = 0; q < t.nseg; ++q) {
+        const auto& g = segs[t.seg0 + q];
+        bytes += 8.0 * g.K * ((double)t.M + t.N);
+      }
+    }
+    c.prof_end(ps, KIND_GEMM_F64, gemm_flops(tasks, segs), bytes);
+  }
 }
 
 void launch_mix(Context& c, const std::vector<MixDst>& dsts, const std::vector<MixTerm>& terms) {
@@ -423,8 +435,17 @@ void launch_mix(Context& c, const std::vector<MixDst>& dsts, const std::vector<M
   auto* dd = (MixDst*)c.upload(dsts.data(), dsts.size() * sizeof(MixDst));
   auto* dt = (MixTerm*)c.upload(terms.data(), std::max<size_t>(1, terms.size()) * sizeof(MixTerm));
   const int grid = (int)std::min<size_t>(work.size(), (size_t)c.num_cu * 8);
+  const int ps = c.prof_begin();
   hipLaunchKernelGGL(k_mix, dim3(grid), dim3(kThreads), 0, c.stream, dw, (int)work.size(), dd, dt);
   HIP_OK(hipGetLastError());
+  if (ps >= 0) {
+    double bytes = 0.0, flops = 0.0;
+    for (const auto& d : dsts) {
+      bytes += 8.0 * d.n * (1.0 + d.nt);
+      flops += 2.0 * d.n * d.nt;
+    }
+    c.prof_end(ps, KIND_MIX, flops, bytes);
+  }
 }
 
 void launch_copy(Context& c, const std::vector<CopyOp>& ops) {
@@ -438,8 +459,14 @@ void launch_copy(Context& c, const std::vector<CopyOp>& ops) {
   auto* dw = (CopyWork*)c.upload(work.data(), work.size() * sizeof(CopyWork));
   auto* dops = (CopyOp*)c.upload(ops.data(), ops.size() * sizeof(CopyOp));
   const int grid = (int)std::min<size_t>(work.size(), (size_t)c.num_cu * 8);
+  const int ps = c.prof_begin();
   hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kThreads), 0, c.stream, dw, (int)work.size(), dops);
   HIP_OK(hipGetLastError());
+  if (ps >= 0) {
+    double bytes = 0.0;
+    for (const auto& o : ops) bytes += 8.0 * o.rows * (double)o.cols * (o.src ? 2.0 : 1.0);
+    c.prof_end(ps, KIND_COPY, 0.0, bytes);
+  }
 }
 
 void launch_fill(Context& c, double* p, int64_t n, double v) {

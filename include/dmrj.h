@@ -58,6 +58,18 @@ int dmrj_version(void);
 int dmrj_ctx_memstats(dmrj_ctx* ctx, int64_t* out3);
 /* force a GEMM tile shape (0: 32x32, 1: 64x64, 2: 128x128, -1: automatic) */
 int dmrj_ctx_set_gemm_cfg(dmrj_ctx* ctx, int cfg);
+/* SVD backend of dmrj_svd_trunc2 / dmrj_twostep: 0 (default) = batched
+ * one-sided Jacobi for sectors with min(R,C) <= 64 and one batched symmetric
+ * eigensolve of the augmented matrices [[0,M],[M^T,0]] for the larger ones;
+ * 1 = rocSOLVER gesdd per sector, 2 = gesvd per sector, 3 = batched Jacobi +
+ * gesvd. */
+int dmrj_ctx_set_svd_backend(dmrj_ctx* ctx, int mode);
+/* per-kernel HIP-event timing on the context stream (enable = 1 on) */
+int dmrj_ctx_profile(dmrj_ctx* ctx, int enable);
+/* kind 0: FP64 grouped GEMM, 1: MPO mix, 2: block copy, 3: SVD solver,
+ * 4: low-precision grouped GEMM.  out4 = [launches, total ms, algorithmic
+ * flops, compulsory HBM bytes]; reset = 1 clears the counters. */
+int dmrj_ctx_kernel_stats(dmrj_ctx* ctx, int kind, double* out4, int reset);
 /* accumulated algorithmic flops of H_eff applications and env updates */
 int dmrj_ctx_flops(dmrj_ctx* ctx, double* out2, int reset);
 

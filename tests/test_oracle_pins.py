@@ -223,3 +223,29 @@ def test_cfg1_golden_oracle_regression():
         assert abs(h["energy"] - gh["energy"]) <= 1e-12 * abs(gh["energy"])
         assert np.allclose(h["storeD"], gh["storeD"], rtol=1e-9, atol=1e-12)
     assert abs(E - g["ed_energy"]) < 1e-7
+
+
+def test_block_sparse_heff_matches_dense():
+    """The sector-blocked CPU H_eff (bench baseline) equals the dense oracle."""
+    from oracle import blocks as ob
+    rng = np.random.default_rng(11)
+    qphys = np.array([[1], [-1]])
+    mpo = om.makeMPO(om.heisenbergMPO(), 2, 4)
+    deltas = omps.mpo_deltas(mpo, qphys)
+    secs = [[(0, 1)], [(-1, 3), (1, 4)], [(-2, 5), (0, 9), (2, 6)], [(-1, 7), (1, 5)], [(0, 1)]]
+    qb = [np.array([[c] for (c, n) in s for _ in range(n)]) for s in secs]
+    A = []
+    for i in range(4):
+        ql, qr = qb[i], qb[i + 1]
+        T = np.zeros((len(ql), 2, len(qr)))
+        ok = (ql[:, None, None, 0] + qphys[None, :, None, 0]) == qr[None, None, :, 0]
+        T[ok] = rng.standard_normal(int(ok.sum()))
+        A.append(T)
+    Lenv = od.Lupdate(np.ones((1, 1, 1)), A[0], mpo[0])
+    Renv = od.Rupdate(np.ones((1, 1, 1)), A[3], mpo[3])
+    th = np.tensordot(A[1], A[2], axes=(2, 0))
+    plan = ob.Heff2Plan(Lenv, Renv, mpo[1], mpo[2], qb[1], qphys, qb[3], deltas[1], deltas[3])
+    y = plan.apply(th)
+    want = od.heff2(Lenv, Renv, th, mpo[1], mpo[2])
+    assert np.allclose(y, want, rtol=1e-12, atol=1e-12 * np.max(np.abs(want)))
+    assert plan.flops > 0
