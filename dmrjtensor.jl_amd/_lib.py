@@ -13,7 +13,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libdmrjhip.so")
 
-PREC = {"f64": 0, "bf16x3": 1, "f16s": 2}
+PREC = {"f64": 0, "bf16x3": 1, "f64c": 16, "bf16x3c": 17}
+CORRECT = 16
 
 _p = C.c_void_p
 _i64 = C.c_int64
@@ -32,6 +33,9 @@ SIGNATURES = {
     "dmrj_ctx_flops": [_p, _dp, C.c_int],
     "dmrj_ctx_profile": [_p, C.c_int],
     "dmrj_ctx_set_svd_backend": [_p, C.c_int],
+    "dmrj_ctx_set_env_precision": [_p, C.c_int],
+    "dmrj_ctx_last_residual": [_p, _dp],
+    "dmrj_selftest_mfma_bf16": [_p, C.POINTER(C.c_float)],
     "dmrj_ctx_kernel_stats": [_p, C.c_int, _dp, C.c_int],
     "dmrj_site_from_dense": [_p, C.c_int, _i64, _i64, _i64, _dp, _i32p, _i32p, _i32p, C.POINTER(_p)],
     "dmrj_site2_from_dense": [_p, C.c_int, _i64, _i64, _i64, _i64, _dp, _i32p, _i32p, _i32p, _i32p,

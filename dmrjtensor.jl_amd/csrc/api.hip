@@ -512,11 +512,24 @@ void rebase_plan(HeffPlan& p, const double* oldx, const double* newx, int64_t nx
     if (t.C >= oldy && t.C < oldy + ny) t.C = newy + (t.C - oldy);
 }
 
-void run_heff(Context& c, const HeffPlan& p) {
-  launch_gemm(c, p.tA, p.sA, -1);
-  launch_mix(c, p.md, p.mt);
-  launch_gemm(c, p.tC, p.sC, -1);
+void run_heff(Context& c, const HeffPlan& p, int prec = DMRJ_PREC_F64) {
+  if (prec == DMRJ_PREC_BF16X3) {
+    launch_gemm_lp(c, p.tA, p.sA, -1);
+    launch_mix(c, p.md, p.mt);
+    launch_gemm_lp(c, p.tC, p.sC, -1);
+  } else {
+    launch_gemm(c, p.tA, p.sA, -1);
+    launch_mix(c, p.md, p.mt);
+    launch_gemm(c, p.tC, p.sC, -1);
+  }
   c.flops_heff += p.flops_alg;
+}
+
+int base_prec(int precision) {
+  const int b = precision & 15;
+  if (b != DMRJ_PREC_F64 && b != DMRJ_PREC_BF16X3)
+    throw Error(DMRJ_ERR_ARG, "precision must be DMRJ_PREC_F64 or DMRJ_PREC_BF16X3 (optionally | DMRJ_PREC_CORRECT)");
+  return b;
 }
 
 // ======================================================== environment plans
@@ -723,9 +736,15 @@ EnvPlan plan_right(Context& c, const EnvMeta& R, const double* Rd, const SiteMet
 }
 
 void run_env(Context& c, EnvPlan& p) {
-  launch_gemm(c, p.t1, p.s1, -1);
-  launch_mix(c, p.md, p.mt);
-  launch_gemm(c, p.t3, p.s3, -1);
+  if (c.env_prec == DMRJ_PREC_BF16X3) {
+    launch_gemm_lp(c, p.t1, p.s1, -1);
+    launch_mix(c, p.md, p.mt);
+    launch_gemm_lp(c, p.t3, p.s3, -1);
+  } else {
+    launch_gemm(c, p.t1, p.s1, -1);
+    launch_mix(c, p.md, p.mt);
+    launch_gemm(c, p.t3, p.s3, -1);
+  }
   c.flops_env += p.flops;
   c.alloc.release(p.X);
   c.alloc.release(p.Y);
@@ -1100,7 +1119,9 @@ struct LanczosOut {
 };
 
 dmrj_tensor* lanczos2(Context& c, const Tensor& L, const Tensor& R, const Tensor& AL,
-                      const Tensor& AR, const MpoSite& W1, const MpoSite& W2) {
+                      const Tensor& AR, const MpoSite& W1, const MpoSite& W2, int precision) {
+  const int prec = base_prec(precision);
+  const bool correct = (precision & DMRJ_PREC_CORRECT) != 0;
   if (L.kind != 1 || R.kind != 2) throw Error(DMRJ_ERR_ARG, "lanczos2: bad environment handles");
   dmrj_tensor* th = make_theta(c, AL, AR);
   const SiteMeta& x = th->site;
@@ -1111,7 +1132,7 @@ dmrj_tensor* lanczos2(Context& c, const Tensor& L, const Tensor& R, const Tensor
   double *T1 = nullptr, *T3 = nullptr;
   HeffPlan p = plan_heff(c, x, th->data, y1, L.env, L.data, R.env, R.data, f, &T1, &T3);
   // H psi (#1)
-  run_heff(c, p);
+  run_heff(c, p, prec);
   {
     const double* xs[2] = {th->data, th->data};
     const double* ys[2] = {th->data, y1};
@@ -1127,7 +1148,7 @@ dmrj_tensor* lanczos2(Context& c, const Tensor& L, const Tensor& R, const Tensor
   launch_scale_dev(c, y1, n, c.scal + S_INVB);
   // H psi2 (#2)
   rebase_plan(p, th->data, y1, n, y1, y2, n);
-  run_heff(c, p);
+  run_heff(c, p, prec);
   {
     const double* xs[1] = {y1};
     const double* ys[1] = {y2};
@@ -1135,6 +1156,24 @@ dmrj_tensor* lanczos2(Context& c, const Tensor& L, const Tensor& R, const Tensor
   }
   launch_lz_scalar(c, 3);
   launch_comb_dev(c, y2, th->data, y1, n, c.scal + S_CA, c.scal + S_CB);
+  if (correct) {
+    // FP64 residual correction of the energy: Rayleigh quotient of the Ritz
+    // vector with an FP64 H_eff application; the residual norm
+    // |H AA - E AA| is kept in S_RES
+    rebase_plan(p, y1, y2, n, y2, y1, n);
+    run_heff(c, p, DMRJ_PREC_F64);
+    {
+      const double* xs[2] = {y2, y2};
+      const double* ys[2] = {y2, y1};
+      launch_dots(c, 2, xs, ys, n, c.scal + S_RR);
+    }
+    launch_lz_scalar(c, 4);
+    launch_axpy_dev(c, y1, y2, n, c.scal + S_NEGE);
+    {
+      const double* xs[1] = {y1};
+      launch_dots(c, 1, xs, xs, n, c.scal + S_RES);
+    }
+  }
   c.alloc.release(T1);
   c.alloc.release(T3);
   c.alloc.release(y1);
@@ -1243,6 +1282,28 @@ int dmrj_ctx_memstats(dmrj_ctx* c, int64_t* out3) {
     out3[0] = (int64_t)c->alloc.bytes_live;
     out3[1] = (int64_t)c->alloc.bytes_cached;
     out3[2] = (int64_t)c->alloc.peak;
+  });
+}
+
+int dmrj_ctx_set_env_precision(dmrj_ctx* c, int precision) {
+  return guard(c, [&] { c->env_prec = base_prec(precision); });
+}
+
+int dmrj_ctx_last_residual(dmrj_ctx* c, double* out) {
+  return guard(c, [&] {
+    if (!out) throw Error(DMRJ_ERR_ARG, "null pointer");
+    *out = std::sqrt(std::max(0.0, read_scalar(*c, S_RES)));
+  });
+}
+
+int dmrj_selftest_mfma_bf16(dmrj_ctx* c, float* out768) {
+  return guard(c, [&] {
+    if (!out768) throw Error(DMRJ_ERR_ARG, "null pointer");
+    float* d = (float*)c->alloc.alloc(768 * sizeof(float));
+    launch_mfma_bf16_probe(*c, d);
+    HIP_OK(hipMemcpyAsync(out768, d, 768 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    c->alloc.release(d);
   });
 }
 
@@ -1523,7 +1584,7 @@ int dmrj_heff2_apply(dmrj_ctx* c, const dmrj_tensor* L, const dmrj_mpo* W1, cons
   return guard(c, [&] {
     if (!L || !W1 || !W2 || !R || !x || !out || L->kind != 1 || R->kind != 2 || x->kind != 0 || !x->site.d1)
       throw Error(DMRJ_ERR_ARG, "heff2_apply: bad handles");
-    if (precision != DMRJ_PREC_F64) throw Error(DMRJ_ERR_ARG, "precision not available");
+    const int prec = base_prec(precision);
     auto* y = new_tensor(*c, 0);
     y->nq = x->nq;
     y->site = x->site;
@@ -1531,7 +1592,7 @@ int dmrj_heff2_apply(dmrj_ctx* c, const dmrj_tensor* L, const dmrj_mpo* W1, cons
     FusedMpo f = fuse_two(*W1, *W2);
     double *T1, *T3;
     HeffPlan p = plan_heff(*c, x->site, x->data, y->data, L->env, L->data, R->env, R->data, f, &T1, &T3);
-    run_heff(*c, p);
+    run_heff(*c, p, prec);
     c->alloc.release(T1);
     c->alloc.release(T3);
     *out = y;
@@ -1543,7 +1604,7 @@ int dmrj_heff1_apply(dmrj_ctx* c, const dmrj_tensor* L, const dmrj_mpo* W, const
   return guard(c, [&] {
     if (!L || !W || !R || !x || !out || L->kind != 1 || R->kind != 2 || x->kind != 0 || x->site.d1)
       throw Error(DMRJ_ERR_ARG, "heff1_apply: bad handles");
-    if (precision != DMRJ_PREC_F64) throw Error(DMRJ_ERR_ARG, "precision not available");
+    const int prec = base_prec(precision);
     auto* y = new_tensor(*c, 0);
     y->nq = x->nq;
     y->site = x->site;
@@ -1551,7 +1612,7 @@ int dmrj_heff1_apply(dmrj_ctx* c, const dmrj_tensor* L, const dmrj_mpo* W, const
     FusedMpo f = fuse_one(*W);
     double *T1, *T3;
     HeffPlan p = plan_heff(*c, x->site, x->data, y->data, L->env, L->data, R->env, R->data, f, &T1, &T3);
-    run_heff(*c, p);
+    run_heff(*c, p, prec);
     c->alloc.release(T1);
     c->alloc.release(T3);
     *out = y;
@@ -1563,8 +1624,7 @@ int dmrj_lanczos2(dmrj_ctx* c, const dmrj_tensor* L, const dmrj_tensor* R, const
                   dmrj_tensor** AA, double* energy) {
   return guard(c, [&] {
     if (!L || !R || !psiL || !psiR || !W1 || !W2 || !AA) throw Error(DMRJ_ERR_ARG, "null pointer");
-    if (precision != DMRJ_PREC_F64) throw Error(DMRJ_ERR_ARG, "precision not available");
-    *AA = lanczos2(*c, *L, *R, *psiL, *psiR, *W1, *W2);
+    *AA = lanczos2(*c, *L, *R, *psiL, *psiR, *W1, *W2, precision);
     const double e = read_scalar(*c, S_E);
     if (energy) *energy = e;
   });
@@ -1601,7 +1661,6 @@ int dmrj_twostep(dmrj_ctx* c, const dmrj_tensor* L, const dmrj_tensor* R, dmrj_t
                  int64_t* kept, double* svals, int64_t cap) {
   return guard(c, [&] {
     if (!L || !R || !psiL || !psiR || !W1 || !W2 || !newL || !newR) throw Error(DMRJ_ERR_ARG, "null pointer");
-    if (precision != DMRJ_PREC_F64) throw Error(DMRJ_ERR_ARG, "precision not available");
     dmrj_tensor* oc = dir > 0 ? psiL : psiR;
     {
       const double* xs[1] = {oc->data};
@@ -1609,7 +1668,7 @@ int dmrj_twostep(dmrj_ctx* c, const dmrj_tensor* L, const dmrj_tensor* R, dmrj_t
       launch_lz_scalar(*c, 0);
       launch_scale_dev(*c, oc->data, oc->nelem(), c->scal + S_INVOC);
     }
-    dmrj_tensor* AA = lanczos2(*c, *L, *R, *psiL, *psiR, *W1, *W2);
+    dmrj_tensor* AA = lanczos2(*c, *L, *R, *psiL, *psiR, *W1, *W2, precision);
     HIP_OK(hipMemcpyAsync(c->hscal, c->scal, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     SvdResult r;
     try {
@@ -1671,6 +1730,8 @@ int dmrj_axpy(dmrj_ctx* c, double a, const dmrj_tensor* x, dmrj_tensor* y) {
 int dmrj_selftest_gemm(dmrj_ctx* c, int cfg, int64_t M, int64_t N, int64_t K, int tA, int tB,
                        const double* A, const double* B, double* Cout) {
   return guard(c, [&] {
+    const int prec = (cfg >> 4) & 15;
+    cfg &= 15;
     if (!A || !B || !Cout || M <= 0 || N <= 0 || K < 0 || cfg < 0 || cfg > 2)
       throw Error(DMRJ_ERR_ARG, "selftest_gemm: bad arguments");
     const int64_t na = std::max<int64_t>(M * K, 1), nb = std::max<int64_t>(K * N, 1);
@@ -1681,7 +1742,10 @@ int dmrj_selftest_gemm(dmrj_ctx* c, int cfg, int64_t M, int64_t N, int64_t K, in
     HIP_OK(hipMemcpyAsync(dB, B, K * N * sizeof(double), hipMemcpyHostToDevice, c->stream));
     std::vector<GemmTask> t{{dC, M, (int)M, (int)N, 0, 1, 0.0}};
     std::vector<GemmSeg> s{{dA, dB, tA ? K : M, tB ? N : K, (int)K, (tA ? 1 : 0) | (tB ? 2 : 0), 1.0}};
-    launch_gemm(*c, t, s, cfg);
+    if (prec == DMRJ_PREC_BF16X3)
+      launch_gemm_lp(*c, t, s, cfg);
+    else
+      launch_gemm(*c, t, s, cfg);
     HIP_OK(hipMemcpyAsync(Cout, dC, M * N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
     c->release_transients();

@@ -195,6 +195,7 @@ struct Context {
   // eigensolver of the augmented matrix for large ones, 1 = gesdd,
   // 2 = gesvd, 3 = batched Jacobi for small sectors + gesvd
   int svd_backend = 0;
+  int env_prec = 0;  // precision of environment updates
   int last_gemm_cfg = -1;
   // statistics
   double flops_heff = 0.0, flops_env = 0.0;
@@ -248,6 +249,8 @@ void launch_comb_dev(Context& c, double* out, const double* a, const double* b, 
                      const double* ca, const double* cb);
 void launch_lz_scalar(Context& c, int stage);
 void launch_mfma_probe(Context& c, double* out);
+void launch_mfma_bf16_probe(Context& c, float* out);
+void launch_gemm_lp(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs, int cfg);
 int choose_gemm_cfg(const std::vector<GemmTask>& tasks, int num_cu);
 struct SmallSvdJob {
   const double* M;
@@ -278,6 +281,7 @@ enum Slot {
   S_TT = 0, S_TH = 1, S_PP = 2, S_PHP = 3, S_OC = 4,
   S_NRM = 8, S_A1 = 9, S_CAXPY = 10, S_B1 = 11, S_INVB = 12, S_FLAG = 13,
   S_A2 = 14, S_E = 15, S_CA = 16, S_CB = 17, S_INVOC = 18, S_CHK = 19,
+  S_RR = 20, S_RH = 21, S_NEGE = 22, S_RES = 23, S_ELP = 24,
   S_COUNT = 32
 };
 

@@ -303,6 +303,11 @@ __global__ void k_lz_scalar(double* s, int stage) {
     s[S_E] = lam;
     s[S_CA] = v1 / s[S_NRM];
     s[S_CB] = v2;
+  } else if (stage == 4) {     // FP64 Rayleigh quotient of the Ritz vector
+    s[S_ELP] = s[S_E];
+    const double e = s[S_RH] / s[S_RR];
+    s[S_E] = e;
+    s[S_NEGE] = -e;
   }
 }
 

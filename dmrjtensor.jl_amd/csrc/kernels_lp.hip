@@ -1,0 +1,241 @@
+// Split-precision grouped GEMM on the BF16 matrix cores of gfx950.
+//
+// Same task / K-segment / tile structure as the FP64 kernel; operands stay
+// FP64 in HBM.  Each element is split on load into a high and a low BF16 part
+// (x = hi + lo + O(2^-16 |x|)) and the product is formed as
+//     hi*hi + hi*lo + lo*hi
+// with v_mfma_f32_16x16x32_bf16 (FP32 accumulation), i.e. about 16 correct
+// bits per product.  A 32-deep K step of the tile is one MFMA per product.
+// LDS holds hi and lo of both operands, row-major in k with a 40-element
+// pitch (the 16 lanes of a 16-byte ds_read hit disjoint banks).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "internal.h"
+
+namespace dmrj {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kLpThreads = 256;
+constexpr int kLpBK = 32;
+constexpr int kLpPitch = 40;
+
+__device__ inline void split_bf16(double x, __bf16& hi, __bf16& lo) {
+  const float xf = (float)x;
+  hi = (__bf16)xf;
+  const double r = x - (double)(float)hi;
+  lo = (__bf16)(float)r;
+}
+
+template <int BM, int BN>
+__global__ __launch_bounds__(kLpThreads) void k_gemm_bf16x3(const GemmTile* __restrict__ tiles, int ntiles,
+                                                             const GemmTask* __restrict__ tasks,
+                                                             const GemmSeg* __restrict__ segs) {
+  constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
+  constexpr int AE = BM * kLpBK / kLpThreads, BE = BN * kLpBK / kLpThreads;
+  __shared__ __attribute__((aligned(16))) __bf16 sAh[2][BM * kLpPitch];
+  __shared__ __attribute__((aligned(16))) __bf16 sAl[2][BM * kLpPitch];
+  __shared__ __attribute__((aligned(16))) __bf16 sBh[2][BN * kLpPitch];
+  __shared__ __attribute__((aligned(16))) __bf16 sBl[2][BN * kLpPitch];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int lr = lane & 15, lk = lane >> 4;
+  int buf = 0;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const GemmTile tl = tiles[t];
+    const GemmTask tk = tasks[tl.task];
+    const int m0 = tl.m0, n0 = tl.n0;
+    f32x4_t acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    for (int sidx = 0; sidx < tk.nseg; ++sidx) {
+      const GemmSeg sg = segs[tk.seg0 + sidx];
+      const bool tA = sg.flags & 1, tB = sg.flags & 2;
+      const int K = sg.K;
+      if (K <= 0) continue;
+      double ra[AE], rb[BE];
+      auto load = [&](int k0) {
+#pragma unroll
+        for (int r = 0; r < AE; ++r) {
+          const int idx = tid + r * kLpThreads;
+          int m, k;
+          if (!tA) { m = idx % BM; k = idx / BM; } else { k = idx % kLpBK; m = idx / kLpBK; }
+          const int gm = m0 + m, gk = k0 + k;
+          double v = 0.0;
+          if (gm < tk.M && gk < K)
+            v = tA ? sg.A[(int64_t)gk + (int64_t)gm * sg.lda] : sg.A[(int64_t)gm + (int64_t)gk * sg.lda];
+          ra[r] = v * sg.alpha;
+        }
+#pragma unroll
+        for (int r = 0; r < BE; ++r) {
+          const int idx = tid + r * kLpThreads;
+          int n, k;
+          if (!tB) { k = idx % kLpBK; n = idx / kLpBK; } else { n = idx % BN; k = idx / BN; }
+          const int gn = n0 + n, gk = k0 + k;
+          double v = 0.0;
+          if (gn < tk.N && gk < K)
+            v = tB ? sg.B[(int64_t)gn + (int64_t)gk * sg.ldb] : sg.B[(int64_t)gk + (int64_t)gn * sg.ldb];
+          rb[r] = v;
+        }
+      };
+      auto store = [&](int b) {
+#pragma unroll
+        for (int r = 0; r < AE; ++r) {
+          const int idx = tid + r * kLpThreads;
+          int m, k;
+          if (!tA) { m = idx % BM; k = idx / BM; } else { k = idx % kLpBK; m = idx / kLpBK; }
+          __bf16 h, l;
+          split_bf16(ra[r], h, l);
+          sAh[b][m * kLpPitch + k] = h;
+          sAl[b][m * kLpPitch + k] = l;
+        }
+#pragma unroll
+        for (int r = 0; r < BE; ++r) {
+          const int idx = tid + r * kLpThreads;
+          int n, k;
+          if (!tB) { k = idx % kLpBK; n = idx / kLpBK; } else { n = idx % BN; k = idx / BN; }
+          __bf16 h, l;
+          split_bf16(rb[r], h, l);
+          sBh[b][n * kLpPitch + k] = h;
+          sBl[b][n * kLpPitch + k] = l;
+        }
+      };
+      load(0);
+      for (int k0 = 0; k0 < K; k0 += kLpBK) {
+        store(buf);
+        __syncthreads();
+        if (k0 + kLpBK < K) load(k0 + kLpBK);
+        bf16x8_t ah[FM], al[FM], bh[FN], bl[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+          const int row = wm * WM + i * 16 + lr;
+          ah[i] = *reinterpret_cast<const bf16x8_t*>(&sAh[buf][row * kLpPitch + lk * 8]);
+          al[i] = *reinterpret_cast<const bf16x8_t*>(&sAl[buf][row * kLpPitch + lk * 8]);
+        }
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          const int col = wn * WN + j * 16 + lr;
+          bh[j] = *reinterpret_cast<const bf16x8_t*>(&sBh[buf][col * kLpPitch + lk * 8]);
+          bl[j] = *reinterpret_cast<const bf16x8_t*>(&sBl[buf][col * kLpPitch + lk * 8]);
+        }
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) {
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+          }
+        buf ^= 1;
+      }
+    }
+    // accumulator item r of lane l: row 4*(l/16) + r, column l%16
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int col = n0 + wn * WN + j * 16 + lr;
+        if (col >= tk.N) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm * WM + i * 16 + lk * 4 + r;
+          if (row < tk.M) {
+            double* p = tk.C + (int64_t)row + (int64_t)col * tk.ldc;
+            double v = (double)acc[i][j][r];
+            if (tk.beta != 0.0) v += tk.beta * *p;
+            *p = v;
+          }
+        }
+      }
+  }
+}
+
+// Lane-layout probe of v_mfma_f32_16x16x32_bf16 (one wave, in-bounds):
+//  mode 0: a = lane+1, b = 1; mode 1: a = 1, b = lane+1 (row / column sums)
+//  mode 2: A[i][0] = i+1, A[i][1] = 1, B[0][j] = 1, B[1][j] = 16(j+1) under
+//          A[i][k] <- lane i + 16(k/8) element k%8, B[k][j] <- lane j + 16(k/8)
+__global__ void k_mfma_bf16_probe(float* out) {
+  const int l = threadIdx.x;
+  const int r16 = l & 15, kb = l >> 4;
+  for (int mode = 0; mode < 3; ++mode) {
+    bf16x8_t a, b;
+    for (int e = 0; e < 8; ++e) {
+      float av, bv;
+      if (mode == 0) {
+        av = (float)(l + 1);
+        bv = 1.f;
+      } else if (mode == 1) {
+        av = 1.f;
+        bv = (float)(l + 1);
+      } else {
+        const int k = kb * 8 + e;
+        av = k == 0 ? (float)(r16 + 1) : (k == 1 ? 1.f : 0.f);
+        bv = k == 0 ? 1.f : (k == 1 ? 16.f * (r16 + 1) : 0.f);
+      }
+      a[e] = (__bf16)av;
+      b[e] = (__bf16)bv;
+    }
+    f32x4_t acc = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    for (int r = 0; r < 4; ++r) out[mode * 256 + l * 4 + r] = acc[r];
+  }
+}
+
+void launch_mfma_bf16_probe(Context& c, float* out) {
+  hipLaunchKernelGGL(k_mfma_bf16_probe, dim3(1), dim3(64), 0, c.stream, out);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) throw Error(-3, std::string("HIP: ") + hipGetErrorString(e));
+}
+
+void launch_gemm_lp(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs, int cfg) {
+  if (tasks.empty()) return;
+  if (cfg < 0) cfg = c.gemm_cfg_override >= 0 ? c.gemm_cfg_override : choose_gemm_cfg(tasks, c.num_cu);
+  const int bm = cfg == 0 ? 32 : (cfg == 1 ? 64 : 128);
+  const int bn = bm;
+  std::vector<std::pair<int64_t, int>> order;
+  for (size_t i = 0; i < tasks.size(); ++i) {
+    int64_t k = 0;
+    for (int s = 0; s < tasks[i].nseg; ++s) k += segs[tasks[i].seg0 + s].K;
+    order.push_back({-(k + 1), (int)i});
+  }
+  std::stable_sort(order.begin(), order.end());
+  std::vector<GemmTile> tiles;
+  for (auto& o : order) {
+    const GemmTask& t = tasks[o.second];
+    for (int n0 = 0; n0 < t.N; n0 += bn)
+      for (int m0 = 0; m0 < t.M; m0 += bm) tiles.push_back({o.second, m0, n0, cfg});
+  }
+  if (tiles.empty()) return;
+  auto* dt = (GemmTile*)c.upload(tiles.data(), tiles.size() * sizeof(GemmTile));
+  auto* dk = (GemmTask*)c.upload(tasks.data(), tasks.size() * sizeof(GemmTask));
+  auto* ds = (GemmSeg*)c.upload(segs.data(), std::max<size_t>(1, segs.size()) * sizeof(GemmSeg));
+  const int occ = cfg == 2 ? 2 : 4;
+  const int grid = (int)std::min<int64_t>((int64_t)tiles.size(), (int64_t)c.num_cu * occ);
+  const int nt = (int)tiles.size();
+  const int ps = c.prof_begin();
+  if (cfg == 0)
+    hipLaunchKernelGGL((k_gemm_bf16x3<32, 32>), dim3(grid), dim3(kLpThreads), 0, c.stream, dt, nt, dk, ds);
+  else if (cfg == 1)
+    hipLaunchKernelGGL((k_gemm_bf16x3<64, 64>), dim3(grid), dim3(kLpThreads), 0, c.stream, dt, nt, dk, ds);
+  else
+    hipLaunchKernelGGL((k_gemm_bf16x3<128, 128>), dim3(grid), dim3(kLpThreads), 0, c.stream, dt, nt, dk, ds);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) throw Error(-3, std::string("HIP: ") + hipGetErrorString(e));
+  if (ps >= 0) {
+    double bytes = 0.0;
+    for (const auto& t : tasks) {
+      bytes += 8.0 * t.M * (double)t.N;
+      for (int q = 0; q < t.nseg; ++q) bytes += 8.0 * segs[t.seg0 + q].K * ((double)t.M + t.N);
+    }
+    c.prof_end(ps, KIND_GEMM_LP, gemm_flops(tasks, segs), bytes);
+  }
+}
+
+}  // namespace dmrj

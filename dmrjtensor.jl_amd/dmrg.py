@@ -29,7 +29,7 @@ class algvars:
     reference's algvars)."""
 
     def __init__(self, psi, maxm=0, minm=2, sweeps=1, cutoff=0.0, goal=0.0, silent=True,
-                 SvNbond=None, precision="f64"):
+                 SvNbond=None, precision="f64", correction="none"):
         self.maxm = maxm
         self.minm = minm
         self.sweeps = sweeps
@@ -38,6 +38,12 @@ class algvars:
         self.silent = silent
         self.SvNbond = (len(psi) + 1) // 2 if SvNbond is None else SvNbond
         self.precision = precision
+        # FP64 energy correction of split-precision runs: "none", "bond" (every
+        # bond step) or "sweep" (the last bond step of each sweep, whose energy
+        # is the one a sweep reports)
+        self.correction = correction
+        self.correct_now = False
+        self.residual = None
         self.energy = 0.0
         self.lastenergy = 0.0
         self.truncerr = 0.0
@@ -89,19 +95,27 @@ def twostep(n, j, i, iL, iR, psi, mpo, Lenv, Renv, params):
     """One bond step (algs/DMRG.jl:362-405) through dmrj_twostep."""
     ctx = psi.ctx
     cap = max(8, 4 * (params.maxm if params.maxm > 0 else 4096))
+    prec = _lib.PREC[params.precision]
+    if params.precision != "f64" and (params.correction == "bond" or
+                                      (params.correction == "sweep" and params.correct_now)):
+        prec |= _lib.CORRECT
     sv = np.zeros(cap)
     nl, nr = C.c_void_p(), C.c_void_p()
     e, te = C.c_double(), C.c_double()
     kept = C.c_int64()
     ctx.call("dmrj_twostep", Lenv[iL].h, Renv[iR].h, psi[iL].h, psi[iR].h, mpo.device(ctx, iL),
              mpo.device(ctx, iR), 1 if j > 0 else -1, int(params.maxm), int(params.minm),
-             float(params.cutoff), _lib.PREC[params.precision], C.byref(nl), C.byref(nr), C.byref(e),
+             float(params.cutoff), prec, C.byref(nl), C.byref(nr), C.byref(e),
              C.byref(te), C.byref(kept), _lib.dptr(sv), cap)
     psi[iL].free()
     psi[iR].free()
     psi[iL] = DeviceTensor(ctx, nl)
     psi[iR] = DeviceTensor(ctx, nr)
     params.energy = e.value
+    if prec & _lib.CORRECT:
+        r = C.c_double()
+        ctx.call("dmrj_ctx_last_residual", C.byref(r))
+        params.residual = r.value
     params.truncerr = te.value
     params.biggestm = max(params.biggestm, int(kept.value))
     SvNcheck(i, j, sv[:min(cap, kept.value)], len(psi), params)
@@ -116,7 +130,8 @@ def dmrgcvg(n, timer, params):
         print(flush=True)
     params.history.append(dict(sweep=n, energy=float(params.energy), maxtrunc=float(params.maxtrunc),
                                biggestm=int(params.biggestm), entropy=float(params.entropy),
-                               storeD=list(params.storeD), time=timer))
+                               storeD=list(params.storeD), time=timer, precision=params.precision,
+                               residual=params.residual))
     params.biggestm = 0
     params.truncerr = params.maxtrunc
     params.maxtrunc = 0.0
@@ -144,9 +159,20 @@ def optmps(psi, mpo, params, stepfct=twostep, cvgfct=dmrgcvg, measfct=startval):
         j = -1
     else:
         j = 1
+    psi.ctx.call("dmrj_ctx_set_env_precision", _lib.PREC[params.precision] & 15)
+    try:
+        _sweeps(psi, mpo, params, stepfct, cvgfct, Lenv, Renv, Nsteps, j)
+    finally:
+        psi.ctx.call("dmrj_ctx_set_env_precision", 0)
+    return params.energy
+
+
+def _sweeps(psi, mpo, params, stepfct, cvgfct, Lenv, Renv, Nsteps, j):
+    Ns = len(psi)
     for n in range(1, params.sweeps + 1):
         t0 = time.perf_counter()
-        for _ in range(Nsteps):
+        for ns in range(Nsteps):
+            params.correct_now = ns == Nsteps - 1
             i = psi.oc
             iL, iR = (i, i + 1) if j > 0 else (i - 1, i)
             stepfct(n, j, i, iL, iR, psi, mpo, Lenv, Renv, params)
@@ -162,11 +188,10 @@ def optmps(psi, mpo, params, stepfct=twostep, cvgfct=dmrgcvg, measfct=startval):
         params.sweep_times.append(timer)
         if cvgfct(n, timer, params):
             break
-    return params.energy
 
 
 def dmrg(psi, mpo, m=0, minm=2, sweeps=1, cutoff=0.0, silent=True, goal=0.0, method="twosite",
-         SvNbond=None, precision="f64", params=None):
+         SvNbond=None, precision="f64", correction=None, params=None):
     """dmrg(psi, mpo; m, minm, sweeps, cutoff, silent, goal, method)
     (algs/DMRG.jl:549-589).  Optimises ``psi`` in place and returns the
     energy; the per-sweep report is in ``params.history`` when a params
@@ -177,6 +202,9 @@ def dmrg(psi, mpo, m=0, minm=2, sweeps=1, cutoff=0.0, silent=True, goal=0.0, met
         params = algvars(psi)
     params.maxm, params.minm, params.sweeps, params.cutoff = m, minm, sweeps, cutoff
     params.silent, params.goal, params.precision = silent, goal, precision
+    if precision not in ("f64", "bf16x3"):
+        raise ValueError("precision must be 'f64' or 'bf16x3'")
+    params.correction = correction if correction is not None else ("none" if precision == "f64" else "sweep")
     if SvNbond is not None:
         params.SvNbond = SvNbond
     return optmps(psi, mpo, params)

@@ -45,7 +45,12 @@ enum {
   DMRJ_ERR_NUMERIC = -5   /* NaN/Inf or non-symmetric input detected */
 };
 
-enum { DMRJ_PREC_F64 = 0, DMRJ_PREC_BF16X3 = 1, DMRJ_PREC_F16S = 2 };
+/* precision of the H_eff / environment contractions: FP64 MFMA, or BF16
+ * split precision (hi*hi + hi*lo + lo*hi on BF16 MFMA, FP32 accumulation);
+ * OR DMRJ_PREC_CORRECT into the precision of lanczos2 / twostep to replace
+ * the energy by the FP64 Rayleigh quotient of the Ritz vector (one FP64
+ * H_eff application; residual norm via dmrj_ctx_last_residual). */
+enum { DMRJ_PREC_F64 = 0, DMRJ_PREC_BF16X3 = 1, DMRJ_PREC_CORRECT = 16 };
 
 /* ---- context --------------------------------------------------------- */
 /* No counterpart in the reference (Julia holds no device state). */
@@ -64,6 +69,10 @@ int dmrj_ctx_set_gemm_cfg(dmrj_ctx* ctx, int cfg);
  * 1 = rocSOLVER gesdd per sector, 2 = gesvd per sector, 3 = batched Jacobi +
  * gesvd. */
 int dmrj_ctx_set_svd_backend(dmrj_ctx* ctx, int mode);
+/* precision of dmrj_env_left / dmrj_env_right (DMRJ_PREC_F64 by default) */
+int dmrj_ctx_set_env_precision(dmrj_ctx* ctx, int precision);
+/* |H AA - E AA| of the last lanczos2 / twostep run with DMRJ_PREC_CORRECT */
+int dmrj_ctx_last_residual(dmrj_ctx* ctx, double* out);
 /* per-kernel HIP-event timing on the context stream (enable = 1 on) */
 int dmrj_ctx_profile(dmrj_ctx* ctx, int enable);
 /* kind 0: FP64 grouped GEMM, 1: MPO mix, 2: block copy, 3: SVD solver,
@@ -166,7 +175,8 @@ int dmrj_normalize(dmrj_ctx* ctx, dmrj_tensor* x);
 
 /* ---- kernel self-tests (no counterpart in the reference) -------------- */
 /* One grouped-GEMM launch on host arrays: C[MxN] = op(A) op(B) with the tile
- * configuration cfg (0: 32x32, 1: 64x64, 2: 128x128).  A is stored KxM when
+ * configuration cfg & 15 (0: 32x32, 1: 64x64, 2: 128x128) and the precision
+ * cfg >> 4 (DMRJ_PREC_F64 or DMRJ_PREC_BF16X3).  A is stored KxM when
  * tA != 0 (else MxK), B is stored NxK when tB != 0 (else KxN), column-major,
  * leading dimension = stored row count. */
 int dmrj_selftest_gemm(dmrj_ctx* ctx, int cfg, int64_t M, int64_t N, int64_t K, int tA, int tB,
@@ -175,6 +185,8 @@ int dmrj_selftest_gemm(dmrj_ctx* ctx, int cfg, int64_t M, int64_t N, int64_t K, 
  * of three probe products (row sums, column sums, encoded product), 768
  * doubles. */
 int dmrj_selftest_mfma_f64(dmrj_ctx* ctx, double* out768);
+/* Same three probes for v_mfma_f32_16x16x32_bf16 (768 floats). */
+int dmrj_selftest_mfma_bf16(dmrj_ctx* ctx, float* out768);
 
 #ifdef __cplusplus
 }

@@ -94,3 +94,50 @@ def test_svd_backends_graded(pkg, ctx, backend, shape):
     rel_err = np.max(np.abs(sv[:n][big] - want[:n][big]) / want[:n][big])
     say(" backend", backend, "shape", shape, "abs/sigma_max %.2e" % abs_err, "rel(sigma>1e-6) %.2e" % rel_err)
     assert abs_err < 1e-14
+
+
+def test_mfma_bf16_lane_layout(pkg, ctx):
+    import ctypes as Ct
+    out = np.zeros(768, dtype=np.float32)
+    ctx.call("dmrj_selftest_mfma_bf16", out.ctypes.data_as(Ct.POINTER(Ct.c_float)))
+    rows = out[:256].reshape(64, 4)
+    cols = out[256:512].reshape(64, 4)
+    enc = out[512:].reshape(64, 4)
+    # operands: A[i][k] from lane i + 16(k/8) element k%8, B[k][j] from lane
+    # j + 16(k/8); result: item r of lane l holds D[4(l/16) + r][l%16]
+    wr = np.zeros((64, 4))
+    wc = np.zeros((64, 4))
+    we = np.zeros((64, 4))
+    for l in range(64):
+        for r in range(4):
+            i, j = 4 * (l // 16) + r, l % 16
+            wr[l, r] = 8 * (4 * i + 100)
+            wc[l, r] = 8 * (4 * j + 100)
+            we[l, r] = (i + 1) + 16 * (j + 1)
+    if not (np.array_equal(rows, wr) and np.array_equal(cols, wc) and np.array_equal(enc, we)):
+        say("bf16 probe rows:", rows.astype(int).tolist())
+        say("bf16 probe cols:", cols.astype(int).tolist())
+        say("bf16 probe enc:", enc.astype(int).tolist())
+    assert np.array_equal(rows, wr)
+    assert np.array_equal(cols, wc)
+    assert np.array_equal(enc, we)
+
+
+@pytest.mark.parametrize("cfg", [0, 1, 2])
+@pytest.mark.parametrize("tA,tB", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("M,N,K", [(16, 16, 32), (33, 70, 17), (130, 129, 200)])
+def test_grouped_gemm_bf16x3(pkg, ctx, cfg, tA, tB, M, N, K):
+    """Split-precision GEMM: error bounded by ~2^-16 of the |A||B| scale."""
+    rng = np.random.default_rng(M + 5 * N + 11 * K)
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((K, N))
+    As = np.asfortranarray(A.T if tA else A)
+    Bs = np.asfortranarray(B.T if tB else B)
+    C = np.zeros((M, N), order="F")
+    ctx.call("dmrj_selftest_gemm", cfg | (1 << 4), M, N, K, tA, tB, pkg._lib.dptr(As.ravel(order="F")),
+             pkg._lib.dptr(Bs.ravel(order="F")), C.ctypes.data_as(pkg._lib._dp))
+    want = A @ B
+    scale = np.abs(A) @ np.abs(B)
+    err = np.max(np.abs(C - want) / scale)
+    say(" bf16x3 cfg", cfg, tA, tB, M, N, K, "err/|A||B| %.2e" % err)
+    assert err < 6e-5
