@@ -4,7 +4,9 @@ Workload (the metric's configuration): open spin-1/2 Heisenberg chain,
 L=100, J=1, d=2, MPO bond dimension 5, U(1) Sz block-sparse tensors (total
 Sz = 0), seeded random product-state start (a permutation of 50 up / 50 down
 from numpy default_rng(0)), bond dimension ramped 128 -> 256 -> 512 -> 1024,
-two-vector Lanczos (Krylov 2), cutoff 1e-12, minm 2.
+two-vector Lanczos (Krylov 2), minm 2, SVD cutoff 0 by default so that the
+bond dimension is capped by m alone (with cutoff 1e-12 the bond dimension of
+this chain stalls near 100 from a product-state start).
 
 A "step" is one full DMRG sweep (2(L-1) two-site bond steps) at m = 1024.
 Warmup = the ramp sweeps plus --warmup sweeps at m = 1024; exactly --steps
@@ -46,6 +48,8 @@ def main():
     ap.add_argument("--L", type=int, default=100)
     ap.add_argument("--m", type=int, default=1024)
     ap.add_argument("--ramp", default="128,256,512")
+    ap.add_argument("--cutoff", type=float, default=0.0,
+                    help="SVD cutoff of the ramp and timed sweeps (0: bond dimension capped by m only)")
     ap.add_argument("--precision", default="f64")
     ap.add_argument("--max-grow", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
@@ -69,18 +73,18 @@ def main():
     params = pkg.algvars(psi)
     t_ramp = time.perf_counter()
     for mr in [int(x) for x in args.ramp.split(",") if x]:
-        pkg.dmrg(psi, mpo, m=mr, sweeps=1, cutoff=1e-12, params=params, precision=args.precision)
+        pkg.dmrg(psi, mpo, m=mr, sweeps=1, cutoff=args.cutoff, params=params, precision=args.precision)
         log("ramp m=%d E=%.12f maxm=%d t=%.2fs" % (mr, params.energy, params.history[-1]["biggestm"],
                                                    params.history[-1]["time"]))
     # keep sweeping at the target m until the bond dimension has grown to it
     for _ in range(args.max_grow):
         if params.history and params.history[-1]["biggestm"] >= int(0.95 * m):
             break
-        pkg.dmrg(psi, mpo, m=m, sweeps=1, cutoff=1e-12, params=params, precision=args.precision)
+        pkg.dmrg(psi, mpo, m=m, sweeps=1, cutoff=args.cutoff, params=params, precision=args.precision)
         log("grow m=%d E=%.12f maxm=%d t=%.2fs" % (m, params.energy, params.history[-1]["biggestm"],
                                                    params.history[-1]["time"]))
     for _ in range(args.warmup):
-        pkg.dmrg(psi, mpo, m=m, sweeps=1, cutoff=1e-12, params=params, precision=args.precision)
+        pkg.dmrg(psi, mpo, m=m, sweeps=1, cutoff=args.cutoff, params=params, precision=args.precision)
         log("warmup m=%d E=%.12f maxm=%d t=%.2fs" % (m, params.energy, params.history[-1]["biggestm"],
                                                      params.history[-1]["time"]))
     log("ramp+warmup %.1fs" % (time.perf_counter() - t_ramp))
@@ -91,7 +95,7 @@ def main():
     for k in ("gemm_f64", "mix", "copy", "svd"):
         ctx.kernel_stats(k, reset=True)
     t0 = time.perf_counter()
-    pkg.dmrg(psi, mpo, m=m, sweeps=args.steps, cutoff=1e-12, params=params, precision=args.precision)
+    pkg.dmrg(psi, mpo, m=m, sweeps=args.steps, cutoff=args.cutoff, params=params, precision=args.precision)
     ctx.sync()
     t1 = time.perf_counter()
     elapsed = t1 - t0
@@ -128,7 +132,7 @@ def main():
         "dtype": "f64" if args.precision == "f64" else args.precision,
         "data": "synthetic (seeded random product state, Heisenberg MPO)",
         "config": {"workload": "Heisenberg L=%d d=2 D=5 U(1) Sz=0, two-site DMRG sweep at m=%d "
-                               "(ramp %s), Krylov 2, cutoff 1e-12" % (L, m, args.ramp),
+                               "(ramp %s), Krylov 2, cutoff %g" % (L, m, args.ramp, args.cutoff),
                    "L": L, "m": m, "energy": params.energy,
                    "heff_flops_per_sweep": fl_heff / args.steps, "env_flops_per_sweep": fl_env / args.steps,
                    "sweep_s": sweep_s},

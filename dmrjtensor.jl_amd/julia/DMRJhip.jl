@@ -1,0 +1,285 @@
+#  DMRJhip.jl: Julia front-end of libdmrjhip (MI355X two-site DMRG hot path).
+#
+#  Keeps the reference's API for the two-site path (bakerte/DMRJtensor.jl):
+#    dmrg(psi, mpo; m, minm, sweeps, cutoff, silent, goal, method)  algs/DMRG.jl:549
+#    twostep / optmps / dmrgcvg / SvNcheck!                         algs/DMRG.jl:362,610,172
+#                                                                   algs/optimizeMPS.jl:759
+#    Lupdate / Rupdate / makeEnv                                    lib/MPO_MPS/Lupdate.jl:17,
+#                                                                   Rupdate.jl:17, lib/Env/makeEnv.jl:196
+#  The tensors live in HBM behind opaque handles; every method below is a
+#  `ccall` into include/dmrj.h.  Host arrays are column-major, the same memory
+#  order as Julia arrays, so no transposition happens at the boundary.
+#
+#  This file needs no package beyond Base.  Point DMRJHIP_LIB at the built
+#  shared library (dmrjtensor.jl_amd/lib/libdmrjhip.so by default).
+module DMRJhip
+
+export gpuMPS, gpuMPO, gpuEnv, dmrg, makeEnv, Lupdate, Rupdate, expect, twostep, free!
+
+const libdmrj = get(ENV, "DMRJHIP_LIB", joinpath(@__DIR__, "..", "lib", "libdmrjhip.so"))
+
+const PREC = Dict("f64" => Cint(0), "bf16x3" => Cint(1))
+const CORRECT = Cint(16)
+
+mutable struct Context
+  h::Ptr{Cvoid}
+end
+
+function check(ctx::Context, rc::Cint, what::String)
+  if rc != 0
+    msg = unsafe_string(ccall((:dmrj_last_error, libdmrj), Cstring, (Ptr{Cvoid},), ctx.h))
+    error("$what failed ($rc): $msg")
+  end
+  nothing
+end
+
+const _ctx = Ref{Union{Nothing,Context}}(nothing)
+
+function context(device::Integer=0)
+  if _ctx[] === nothing
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:dmrj_ctx_create, libdmrj), Cint, (Cint, Ref{Ptr{Cvoid}}), device, h)
+    rc == 0 || error("dmrj_ctx_create failed ($rc): no usable GPU")
+    _ctx[] = Context(h[])
+  end
+  return _ctx[]
+end
+
+"device tensor handle (MPS tensor, two-site tensor or environment)"
+mutable struct gpuTens
+  h::Ptr{Cvoid}
+end
+
+function free!(t::gpuTens)
+  if t.h != C_NULL
+    ctx = context()
+    ccall((:dmrj_tensor_free, libdmrj), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), ctx.h, t.h)
+    t.h = C_NULL
+  end
+  nothing
+end
+
+"charges as an Int32 [nq, n] matrix (column j = charge of basis state j)"
+charges(q::AbstractMatrix{<:Integer}) = Matrix{Int32}(q)
+
+function upload(A::Array{Float64,3}; qL=nothing, qP=nothing, qR=nothing)
+  ctx = context()
+  nq = qL === nothing ? 0 : size(qL, 1)
+  out = Ref{Ptr{Cvoid}}(C_NULL)
+  pl = nq == 0 ? C_NULL : pointer(charges(qL))
+  pp = nq == 0 ? C_NULL : pointer(charges(qP))
+  pr = nq == 0 ? C_NULL : pointer(charges(qR))
+  GC.@preserve qL qP qR A begin
+    rc = ccall((:dmrj_site_from_dense, libdmrj), Cint,
+               (Ptr{Cvoid}, Cint, Int64, Int64, Int64, Ptr{Float64}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32},
+                Ref{Ptr{Cvoid}}),
+               ctx.h, nq, size(A, 1), size(A, 2), size(A, 3), A, pl, pp, pr, out)
+  end
+  check(ctx, rc, "dmrj_site_from_dense")
+  return gpuTens(out[])
+end
+
+function info(t::gpuTens)
+  d = zeros(Int64, 9)
+  ccall((:dmrj_tensor_info, libdmrj), Cint, (Ptr{Cvoid}, Ptr{Int64}), t.h, d)
+  return d
+end
+
+function download(t::gpuTens)
+  ctx = context()
+  d = info(t)
+  dims = d[1] == 1 ? (d[2], d[3], d[4], d[5]) : (d[2], d[3], d[4])
+  out = zeros(Float64, dims...)
+  rc = ccall((:dmrj_tensor_to_dense, libdmrj), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Ptr{Int32}, Ptr{Int32}),
+             ctx.h, t.h, out, C_NULL, C_NULL)
+  check(ctx, rc, "dmrj_tensor_to_dense")
+  return out
+end
+
+"matrixproductstate on the GPU (lib/types.jl:84)"
+mutable struct gpuMPS
+  A::Vector{gpuTens}
+  oc::Int
+  nq::Int
+end
+Base.getindex(psi::gpuMPS, i::Integer) = psi.A[i]
+Base.setindex!(psi::gpuMPS, v::gpuTens, i::Integer) = (psi.A[i] = v)
+Base.length(psi::gpuMPS) = length(psi.A)
+
+"Upload a dense (canonical, oc-centred) MPS; optional charges per bond (sorted)."
+function gpuMPS(A::Vector{Array{Float64,3}}; oc::Integer=1, qbond=nothing, qphys=nothing)
+  if qbond === nothing
+    return gpuMPS([upload(a) for a in A], oc, 0)
+  end
+  nq = size(qphys, 1)
+  return gpuMPS([upload(A[i]; qL=qbond[i], qP=qphys, qR=qbond[i+1]) for i = 1:length(A)], oc, nq)
+end
+
+"matrixproductoperator on the GPU (lib/types.jl:99); deltas = MPO link charges"
+struct gpuMPO
+  W::Vector{Ptr{Cvoid}}
+end
+Base.getindex(mpo::gpuMPO, i::Integer) = mpo.W[i]
+Base.length(mpo::gpuMPO) = length(mpo.W)
+
+function gpuMPO(H::Vector{Array{Float64,4}}; qphys=nothing, deltas=nothing)
+  ctx = context()
+  nq = qphys === nothing ? 0 : size(qphys, 1)
+  W = Ptr{Cvoid}[]
+  for i = 1:length(H)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    w = H[i]
+    qp = nq == 0 ? C_NULL : pointer(charges(qphys))
+    dl = nq == 0 ? C_NULL : pointer(charges(deltas[i]))
+    dr = nq == 0 ? C_NULL : pointer(charges(deltas[i+1]))
+    GC.@preserve w qphys deltas begin
+      rc = ccall((:dmrj_mpo_create, libdmrj), Cint,
+                 (Ptr{Cvoid}, Cint, Int64, Int64, Int64, Ptr{Float64}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32},
+                  Ref{Ptr{Cvoid}}),
+                 ctx.h, nq, size(w, 1), size(w, 2), size(w, 4), w, qp, dl, dr, out)
+    end
+    check(ctx, rc, "dmrj_mpo_create")
+    push!(W, out[])
+  end
+  return gpuMPO(W)
+end
+
+const gpuEnv = Vector{Union{Nothing,gpuTens}}
+
+function boundary(left::Bool, nq::Integer, q=nothing)
+  ctx = context()
+  out = Ref{Ptr{Cvoid}}(C_NULL)
+  qq = nq == 0 ? C_NULL : pointer(Vector{Int32}(q))
+  rc = ccall((:dmrj_env_boundary, libdmrj), Cint, (Ptr{Cvoid}, Cint, Cint, Ptr{Int32}, Ref{Ptr{Cvoid}}),
+             ctx.h, left ? 1 : 0, nq, qq, out)
+  check(ctx, rc, "dmrj_env_boundary")
+  return gpuTens(out[])
+end
+
+"Lupdate (lib/MPO_MPS/Lupdate.jl:17-26)"
+function Lupdate(Lenv::gpuTens, A::gpuTens, W::Ptr{Cvoid})
+  ctx = context()
+  out = Ref{Ptr{Cvoid}}(C_NULL)
+  rc = ccall((:dmrj_env_left, libdmrj), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{Ptr{Cvoid}}),
+             ctx.h, Lenv.h, A.h, W, out)
+  check(ctx, rc, "dmrj_env_left")
+  return gpuTens(out[])
+end
+
+"Rupdate (lib/MPO_MPS/Rupdate.jl:17-26)"
+function Rupdate(Renv::gpuTens, A::gpuTens, W::Ptr{Cvoid})
+  ctx = context()
+  out = Ref{Ptr{Cvoid}}(C_NULL)
+  rc = ccall((:dmrj_env_right, libdmrj), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{Ptr{Cvoid}}),
+             ctx.h, Renv.h, A.h, W, out)
+  check(ctx, rc, "dmrj_env_right")
+  return gpuTens(out[])
+end
+
+"makeEnv (lib/Env/makeEnv.jl:196-222); qtot = total charge of the state"
+function makeEnv(psi::gpuMPS, mpo::gpuMPO; qtot=nothing)
+  Ns = length(psi)
+  Lenv = gpuEnv(nothing, Ns)
+  Renv = gpuEnv(nothing, Ns)
+  Lenv[1] = boundary(true, psi.nq, psi.nq == 0 ? nothing : zeros(Int32, psi.nq))
+  Renv[Ns] = boundary(false, psi.nq, qtot)
+  for i = 1:psi.oc-1
+    Lenv[i+1] = Lupdate(Lenv[i], psi[i], mpo[i])
+  end
+  for i = Ns:-1:psi.oc+1
+    Renv[i-1] = Rupdate(Renv[i], psi[i], mpo[i])
+  end
+  return Lenv, Renv
+end
+
+mutable struct algvars
+  maxm::Int
+  minm::Int
+  sweeps::Int
+  cutoff::Float64
+  goal::Float64
+  silent::Bool
+  SvNbond::Int
+  precision::String
+  energy::Float64
+  lastenergy::Float64
+  truncerr::Float64
+  maxtrunc::Float64
+  biggestm::Int
+  entropy::Float64
+  storeD::Vector{Float64}
+end
+
+"twostep (algs/DMRG.jl:362-405) as one dmrj_twostep call"
+function twostep(n, j, i, iL, iR, psi::gpuMPS, mpo::gpuMPO, Lenv, Renv, params::algvars; correct::Bool=false)
+  ctx = context()
+  cap = max(8, 4 * (params.maxm > 0 ? params.maxm : 4096))
+  sv = zeros(Float64, cap)
+  nl = Ref{Ptr{Cvoid}}(C_NULL); nr = Ref{Ptr{Cvoid}}(C_NULL)
+  e = Ref{Float64}(0.0); te = Ref{Float64}(0.0); kept = Ref{Int64}(0)
+  prec = PREC[params.precision] | ((correct && params.precision != "f64") ? CORRECT : Cint(0))
+  rc = ccall((:dmrj_twostep, libdmrj), Cint,
+             (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Int64,
+              Int64, Float64, Cint, Ref{Ptr{Cvoid}}, Ref{Ptr{Cvoid}}, Ref{Float64}, Ref{Float64}, Ref{Int64},
+              Ptr{Float64}, Int64),
+             ctx.h, Lenv[iL].h, Renv[iR].h, psi[iL].h, psi[iR].h, mpo[iL], mpo[iR], j > 0 ? 1 : -1,
+             params.maxm, params.minm, params.cutoff, prec, nl, nr, e, te, kept, sv, cap)
+  check(ctx, rc, "dmrj_twostep")
+  free!(psi[iL]); free!(psi[iR])
+  psi[iL] = gpuTens(nl[]); psi[iR] = gpuTens(nr[])
+  params.energy = e[]
+  params.truncerr = te[]
+  params.maxtrunc = max(params.maxtrunc, te[])
+  params.biggestm = max(params.biggestm, kept[])
+  if i == min(max(2, params.SvNbond + 1), length(psi)) && j < 0
+    D = sort(sv[1:min(cap, kept[])], rev=true)
+    params.entropy = -sum(x -> x > 0 ? x^2 * log(x^2) : 0.0, D)
+    params.storeD = D[1:min(8, length(D))]
+  end
+  nothing
+end
+
+"dmrg (algs/DMRG.jl:549-589), method = \"twosite\" only"
+function dmrg(psi::gpuMPS, mpo::gpuMPO; m::Integer=0, minm::Integer=2, sweeps::Integer=1, cutoff::Float64=0.,
+              silent::Bool=false, goal::Float64=0., method::String="twosite", precision::String="f64",
+              qtot=nothing)
+  method == "twosite" || error("DMRG method ($method) not defined on the GPU path")
+  Ns = length(psi)
+  params = algvars(m, minm, sweeps, cutoff, goal, silent, cld(Ns, 2), precision, 0., 0., 0., 0., 0, 0., zeros(8))
+  Lenv, Renv = makeEnv(psi, mpo; qtot=qtot)
+  ccall((:dmrj_ctx_set_env_precision, libdmrj), Cint, (Ptr{Cvoid}, Cint), context().h, PREC[precision])
+  j = psi.oc == Ns ? -1 : 1
+  Nsteps = 2 * (Ns - 1)
+  for n = 1:sweeps
+    timer = -time()
+    for ns = 1:Nsteps
+      i = psi.oc
+      iL, iR = j > 0 ? (i, i + 1) : (i - 1, i)
+      twostep(n, j, i, iL, iR, psi, mpo, Lenv, Renv, params; correct=(ns == Nsteps))
+      if j > 0
+        Lenv[i+1] = Lupdate(Lenv[i], psi[i], mpo[i])
+      else
+        Renv[i-1] = Rupdate(Renv[i], psi[i], mpo[i])
+      end
+      psi.oc += j
+      if (psi.oc == Ns && j > 0) || (psi.oc == 1 && j < 0)
+        j *= -1
+      end
+    end
+    timer += time()
+    if !silent
+      println("Sweep $n (back and forth): $(round(timer, digits=5)) sec")
+      println("Largest truncation = $(params.maxtrunc), m = $(params.biggestm)")
+      println("Energy at sweep $n is $(params.energy)")
+    end
+    params.maxtrunc = 0.; params.biggestm = 0
+    if abs(params.energy - params.lastenergy) < goal
+      break
+    end
+    params.lastenergy = params.energy
+  end
+  ccall((:dmrj_ctx_set_env_precision, libdmrj), Cint, (Ptr{Cvoid}, Cint), context().h, 0)
+  return params.energy
+end
+
+end # module
