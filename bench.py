@@ -54,6 +54,8 @@ def main():
     ap.add_argument("--max-grow", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="only run one block-copy kernel over this many doubles (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -64,6 +66,11 @@ def main():
     import __graft_entry__ as ge
     pkg = ge.load_package()
     ctx = pkg.default_context()
+    if args.calibrate_copy:
+        ctx.call("dmrj_selftest_copy", int(args.calibrate_copy))
+        log("copy kernel: %d doubles read and written (%.1f MiB each way)"
+            % (args.calibrate_copy, args.calibrate_copy * 8 / 2 ** 20))
+        return
     L, m = args.L, args.m
     rng = np.random.default_rng(0)
     states = list(rng.permutation([0] * (L // 2) + [1] * (L - L // 2)))

@@ -64,6 +64,7 @@ SIGNATURES = {
     "dmrj_normalize": [_p, _p],
     "dmrj_selftest_gemm": [_p, C.c_int, _i64, _i64, _i64, C.c_int, C.c_int, _dp, _dp, _dp],
     "dmrj_selftest_mfma_f64": [_p, _dp],
+    "dmrj_selftest_copy": [_p, _i64],
 }
 
 _lib = None

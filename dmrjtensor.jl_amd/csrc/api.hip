@@ -1756,6 +1756,22 @@ int dmrj_selftest_gemm(dmrj_ctx* c, int cfg, int64_t M, int64_t N, int64_t K, in
   });
 }
 
+int dmrj_selftest_copy(dmrj_ctx* c, int64_t n) {
+  return guard(c, [&] {
+    if (n <= 0 || n % 4096 != 0 || n / 4096 > (1 << 30)) throw Error(DMRJ_ERR_ARG, "n must be a positive multiple of 4096");
+    double* a = (double*)c->alloc.alloc(n * sizeof(double));
+    double* b = (double*)c->alloc.alloc(n * sizeof(double));
+    launch_fill(*c, a, n, 1.0);
+    std::vector<CopyOp> ops{{a, b, 4096, 4096, 4096, (int)(n / 4096), nullptr, nullptr, 1.0}};
+    launch_copy(*c, ops);
+    HIP_OK(hipStreamSynchronize(c->stream));
+    c->release_transients();
+    c->stage_reset();
+    c->alloc.release(a);
+    c->alloc.release(b);
+  });
+}
+
 int dmrj_selftest_mfma_f64(dmrj_ctx* c, double* out768) {
   return guard(c, [&] {
     if (!out768) throw Error(DMRJ_ERR_ARG, "null pointer");

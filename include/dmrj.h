@@ -185,6 +185,10 @@ int dmrj_selftest_gemm(dmrj_ctx* ctx, int cfg, int64_t M, int64_t N, int64_t K, 
  * of three probe products (row sums, column sums, encoded product), 768
  * doubles. */
 int dmrj_selftest_mfma_f64(dmrj_ctx* ctx, double* out768);
+/* One block-copy kernel launch over n doubles (n a multiple of 4096):
+ * the calibration kernel of known traffic (8n bytes read, 8n written) for
+ * the FETCH_SIZE / WRITE_SIZE hardware counters. */
+int dmrj_selftest_copy(dmrj_ctx* ctx, int64_t n);
 /* Same three probes for v_mfma_f32_16x16x32_bf16 (768 floats). */
 int dmrj_selftest_mfma_bf16(dmrj_ctx* ctx, float* out768);
 
