@@ -14,11 +14,13 @@ from .network import (MPO, MPS, DeviceTensor, Lupdate, Rupdate, environment, exp
                       makeMPO, makeq, mpo_deltas, product_state, randMPS_normal, site2_from_dense,
                       site_from_dense)
 from .dmrg import algvars, dmrg, optmps, singlesite_update, startval, twostep
+from .tdvp import krylovTimeEvol1, krylovTimeEvol2, local_expect, tdvp_twosite
 
 __all__ = [
     "Context", "DMRJError", "default_context", "fermionOps", "heisenbergMPO", "hubbardMPO",
     "j1j2_bulk", "spinOps", "XXZ", "MPO", "MPS", "DeviceTensor", "Lupdate", "Rupdate",
     "environment", "expect", "makeEnv", "makeMPO", "makeq", "mpo_deltas", "product_state",
     "randMPS_normal", "site2_from_dense", "site_from_dense", "algvars", "dmrg", "optmps",
-    "singlesite_update", "startval", "twostep",
+    "singlesite_update", "startval", "twostep", "krylovTimeEvol1", "krylovTimeEvol2", "local_expect",
+    "tdvp_twosite",
 ]

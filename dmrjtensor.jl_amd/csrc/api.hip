@@ -1182,6 +1182,148 @@ dmrj_tensor* lanczos2(Context& c, const Tensor& L, const Tensor& R, const Tensor
   return th;
 }
 
+// ============================================================ Krylov exponential
+// Symmetric eigendecomposition of a small tridiagonal matrix (cyclic Jacobi,
+// FP64, host): T = Q diag(lam) Q^T, Q column-major k x k.
+void tridiag_eig(int k, const double* a, const double* b, std::vector<double>& lam, std::vector<double>& Q) {
+  std::vector<double> A((size_t)k * k, 0.0);
+  for (int i = 0; i < k; ++i) {
+    A[i + (size_t)i * k] = a[i];
+    if (i + 1 < k) A[i + (size_t)(i + 1) * k] = A[(i + 1) + (size_t)i * k] = b[i];
+  }
+  Q.assign((size_t)k * k, 0.0);
+  for (int i = 0; i < k; ++i) Q[i + (size_t)i * k] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < k; ++p)
+      for (int q = p + 1; q < k; ++q) off += A[p + (size_t)q * k] * A[p + (size_t)q * k];
+    if (off < 1e-300) break;
+    for (int p = 0; p < k; ++p)
+      for (int q = p + 1; q < k; ++q) {
+        const double apq = A[p + (size_t)q * k];
+        if (apq == 0.0) continue;
+        const double app = A[p + (size_t)p * k], aqq = A[q + (size_t)q * k];
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double cs = 1.0 / std::sqrt(t * t + 1.0), sn = t * cs;
+        for (int r = 0; r < k; ++r) {  // columns p, q
+          const double x = A[r + (size_t)p * k], y = A[r + (size_t)q * k];
+          A[r + (size_t)p * k] = cs * x - sn * y;
+          A[r + (size_t)q * k] = sn * x + cs * y;
+        }
+        for (int r = 0; r < k; ++r) {  // rows p, q
+          const double x = A[p + (size_t)r * k], y = A[q + (size_t)r * k];
+          A[p + (size_t)r * k] = cs * x - sn * y;
+          A[q + (size_t)r * k] = sn * x + cs * y;
+        }
+        for (int r = 0; r < k; ++r) {
+          const double x = Q[r + (size_t)p * k], y = Q[r + (size_t)q * k];
+          Q[r + (size_t)p * k] = cs * x - sn * y;
+          Q[r + (size_t)q * k] = sn * x + cs * y;
+        }
+      }
+  }
+  lam.resize(k);
+  for (int i = 0; i < k; ++i) lam[i] = A[i + (size_t)i * k];
+}
+
+// Lanczos with r vectors and full reorthogonalisation (FP64 dots) on x,
+// then out = sum_j c_j v_j with c = exp(pf T) e1 (TensorPACK lanczos +
+// exp(M, prefactor) as used by krylovTimeEvol, algs/time.jl:182-202).
+// Returns the lowest Ritz value in *energy and the Krylov dimension used.
+dmrj_tensor* krylov_expm(Context& c, const Tensor& L, const Tensor& R, const Tensor& x, const FusedMpo& f, int r,
+                         double pf, int prec, double* energy, int* kdim) {
+  if (r < 1 || r > 64) throw Error(DMRJ_ERR_ARG, "Krylov dimension must be 1..64");
+  const SiteMeta& m = x.site;
+  const int64_t n = std::max<int64_t>(m.size, 1);
+  double* V = (double*)c.alloc.alloc((size_t)n * r * sizeof(double));
+  double* w = (double*)c.alloc.alloc((size_t)n * sizeof(double));
+  double* sc = (double*)c.alloc.alloc(256 * sizeof(double));  // alpha[0..63], beta[64..127], work[128..]
+  launch_fill(c, sc, 256, 0.0);
+  // v0 = x / |x|
+  {
+    const double* xs[1] = {x.data};
+    launch_dots(c, 1, xs, xs, m.size, c.scal + S_OC);
+    launch_lz_scalar(c, 0);
+    HIP_OK(hipMemcpyAsync(V, x.data, m.size * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    launch_scale_dev(c, V, m.size, c.scal + S_INVOC);
+  }
+  double *T1 = nullptr, *T3 = nullptr;
+  HeffPlan p = plan_heff(c, m, V, w, L.env, L.data, R.env, R.data, f, &T1, &T3);
+  const double* px = V;
+  for (int j = 0; j < r; ++j) {
+    double* vj = V + (size_t)j * n;
+    rebase_plan(p, px, vj, n, w, w, n);
+    px = vj;
+    run_heff(c, p, prec);
+    // alpha_j = <v_j, w>;  w -= alpha_j v_j + beta_{j-1} v_{j-1}; full reorthogonalisation
+    {
+      const double* xs[1] = {vj};
+      const double* ys[1] = {w};
+      launch_dots(c, 1, xs, ys, m.size, sc + j);
+    }
+    if (j == r - 1) break;
+    for (int i = 0; i <= j; ++i) {  // two passes of classical Gram-Schmidt
+      for (int pass = 0; pass < 2; ++pass) {
+        const double* xs[1] = {V + (size_t)i * n};
+        const double* ys[1] = {w};
+        launch_dots(c, 1, xs, ys, m.size, sc + 128);
+        launch_scalar_op(c, sc, 0, 128, 129);
+        launch_axpy_dev(c, w, V + (size_t)i * n, m.size, sc + 129);
+      }
+    }
+    {
+      const double* xs[1] = {w};
+      launch_dots(c, 1, xs, xs, m.size, sc + 130);
+    }
+    // beta_j = |w|, v_{j+1} = w / beta_j (zero when the Krylov space is exhausted)
+    HIP_OK(hipMemcpyAsync(c.scal + S_PP, sc + 130, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    launch_lz_scalar(c, 2);
+    HIP_OK(hipMemcpyAsync(sc + 64 + j, c.scal + S_B1, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    double* vn = V + (size_t)(j + 1) * n;
+    HIP_OK(hipMemcpyAsync(vn, w, m.size * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    launch_scale_dev(c, vn, m.size, c.scal + S_INVB);
+  }
+  std::vector<double> hs(128);
+  HIP_OK(hipMemcpyAsync(hs.data(), sc, 128 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIP_OK(hipStreamSynchronize(c.stream));
+  c.release_transients();
+  c.stage_reset();
+  // effective dimension: stop at the first negligible beta
+  int k = r;
+  for (int j = 0; j + 1 < r; ++j) {
+    if (!std::isfinite(hs[64 + j]) || !std::isfinite(hs[j])) throw Error(DMRJ_ERR_NUMERIC, "Krylov: non-finite value");
+    if (hs[64 + j] <= 1e-10) {  // same threshold as the device-side breakdown flag
+      k = j + 1;
+      break;
+    }
+  }
+  std::vector<double> lam, Q;
+  tridiag_eig(k, hs.data(), hs.data() + 64, lam, Q);
+  std::vector<double> coef(k, 0.0);
+  double emin = lam[0];
+  for (int i = 0; i < k; ++i) emin = std::min(emin, lam[i]);
+  for (int i = 0; i < k; ++i) {
+    const double g = std::exp(pf * lam[i]) * Q[0 + (size_t)i * k];  // (Q^T e1)_i scaled
+    for (int jj = 0; jj < k; ++jj) coef[jj] += Q[jj + (size_t)i * k] * g;
+  }
+  auto* out = new dmrj_tensor();
+  out->kind = 0;
+  out->nq = x.nq;
+  out->site = m;
+  alloc_data(c, *out);
+  double* dc = (double*)c.upload(coef.data(), coef.size() * sizeof(double));
+  launch_combine(c, out->data, V, n, m.size, dc, k);
+  c.alloc.release(T1);
+  c.alloc.release(T3);
+  c.alloc.release(V);
+  c.alloc.release(w);
+  c.alloc.release(sc);
+  if (energy) *energy = emin;
+  if (kdim) *kdim = k;
+  return out;
+}
+
 double read_scalar(Context& c, int slot) {
   HIP_OK(hipMemcpyAsync(c.hscal, c.scal, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   HIP_OK(hipStreamSynchronize(c.stream));
@@ -1627,6 +1769,26 @@ int dmrj_lanczos2(dmrj_ctx* c, const dmrj_tensor* L, const dmrj_tensor* R, const
     *AA = lanczos2(*c, *L, *R, *psiL, *psiR, *W1, *W2, precision);
     const double e = read_scalar(*c, S_E);
     if (energy) *energy = e;
+  });
+}
+
+int dmrj_krylov_expm2(dmrj_ctx* c, const dmrj_tensor* L, const dmrj_tensor* R, const dmrj_tensor* AA,
+                      const dmrj_mpo* W1, const dmrj_mpo* W2, int krylov, double prefactor, int precision,
+                      dmrj_tensor** out, double* energy, int* kdim) {
+  return guard(c, [&] {
+    if (!L || !R || !AA || !W1 || !W2 || !out || L->kind != 1 || R->kind != 2 || AA->kind != 0 || !AA->site.d1)
+      throw Error(DMRJ_ERR_ARG, "krylov_expm2: bad handles");
+    *out = krylov_expm(*c, *L, *R, *AA, fuse_two(*W1, *W2), krylov, prefactor, base_prec(precision), energy, kdim);
+  });
+}
+
+int dmrj_krylov_expm1(dmrj_ctx* c, const dmrj_tensor* L, const dmrj_tensor* R, const dmrj_tensor* A,
+                      const dmrj_mpo* W, int krylov, double prefactor, int precision, dmrj_tensor** out,
+                      double* energy, int* kdim) {
+  return guard(c, [&] {
+    if (!L || !R || !A || !W || !out || L->kind != 1 || R->kind != 2 || A->kind != 0 || A->site.d1)
+      throw Error(DMRJ_ERR_ARG, "krylov_expm1: bad handles");
+    *out = krylov_expm(*c, *L, *R, *A, fuse_one(*W), krylov, prefactor, base_prec(precision), energy, kdim);
   });
 }
 

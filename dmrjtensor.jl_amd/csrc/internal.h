@@ -248,6 +248,8 @@ void launch_scale_dev(Context& c, double* x, int64_t n, const double* coef);
 void launch_comb_dev(Context& c, double* out, const double* a, const double* b, int64_t n,
                      const double* ca, const double* cb);
 void launch_lz_scalar(Context& c, int stage);
+void launch_scalar_op(Context& c, double* slots, int op, int src, int dst);
+void launch_combine(Context& c, double* out, const double* V, int64_t ld, int64_t n, const double* coef, int k);
 void launch_mfma_probe(Context& c, double* out);
 void launch_mfma_bf16_probe(Context& c, float* out);
 void launch_gemm_lp(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs, int cfg);

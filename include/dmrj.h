@@ -146,6 +146,20 @@ int dmrj_lanczos2(dmrj_ctx* ctx, const dmrj_tensor* L, const dmrj_tensor* R,
                   const dmrj_tensor* psiL, const dmrj_tensor* psiR, const dmrj_mpo* W1,
                   const dmrj_mpo* W2, int precision, dmrj_tensor** AA, double* energy);
 
+/* ---- Krylov time evolution (two-site TDVP, algs/time.jl:182-202,288-362) -- */
+/* krylovTimeEvol: Lanczos with `krylov` vectors (full FP64
+ * reorthogonalisation) on AA, tridiagonal T, out = sum_j [exp(prefactor T)]_{j1}
+ * v_j with v_1 = AA/|AA|; energy = lowest Ritz value, kdim = Krylov
+ * dimension used (smaller when the space is exhausted).  Real prefactor
+ * (imaginary-time evolution). */
+int dmrj_krylov_expm2(dmrj_ctx* ctx, const dmrj_tensor* L, const dmrj_tensor* R, const dmrj_tensor* AA,
+                      const dmrj_mpo* W1, const dmrj_mpo* W2, int krylov, double prefactor, int precision,
+                      dmrj_tensor** out, double* energy, int* kdim);
+/* Same for a one-site tensor with singlesite_update (algs/optimizeMPS.jl:503). */
+int dmrj_krylov_expm1(dmrj_ctx* ctx, const dmrj_tensor* L, const dmrj_tensor* R, const dmrj_tensor* A,
+                      const dmrj_mpo* W, int krylov, double prefactor, int precision, dmrj_tensor** out,
+                      double* energy, int* kdim);
+
 /* ---- truncated SVD ---------------------------------------------------- */
 /* fsvd!(AA,[[1,2],[3,4]]; m, minm, cutoff, mag=1.) and the split of
  * algs/DMRG.jl:387-395: dir > 0 -> (U, D*V); dir < 0 -> (U*D, V).

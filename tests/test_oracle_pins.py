@@ -249,3 +249,49 @@ def test_block_sparse_heff_matches_dense():
     want = od.heff2(Lenv, Renv, th, mpo[1], mpo[2])
     assert np.allclose(y, want, rtol=1e-12, atol=1e-12 * np.max(np.abs(want)))
     assert plan.flops > 0
+
+
+def test_krylov_expm_matches_dense_expm():
+    """With the Krylov dimension equal to the space dimension the Krylov
+    propagator is exact: compare with scipy's expm of the dense H_eff."""
+    import scipy.linalg as sla
+    from oracle import tdvp as ot
+    rng = np.random.default_rng(4)
+    L = 4
+    mpo = om.makeMPO(om.heisenbergMPO(), 2, L)
+    psi = omps.seeded_normal_mps(L, 2, 4, seed=2)
+    omps.move(psi, 2)
+    Lenv, Renv = od.makeEnv(psi, mpo)
+    AA = np.tensordot(psi[2], psi[3], axes=(2, 0))
+    shape = AA.shape
+    n = AA.size
+    H = np.zeros((n, n))
+    for k in range(n):
+        e = np.zeros(n)
+        e[k] = 1.0
+        H[:, k] = od.heff2(Lenv[2], Renv[3], e.reshape(shape), mpo[1], mpo[2]).ravel()
+    assert np.allclose(H, H.T, atol=1e-12)
+    pf = -0.3
+    x = AA.ravel() / np.linalg.norm(AA)
+    want = sla.expm(pf * H) @ x
+    got, e0 = ot.krylov_expm(lambda t: od.heff2(Lenv[2], Renv[3], t, mpo[1], mpo[2]), AA, n, pf)
+    assert np.allclose(got.ravel(), want, rtol=1e-10, atol=1e-12)
+    assert np.isclose(e0, np.linalg.eigvalsh(H)[0], rtol=1e-12)
+
+
+def test_tdvp_imaginary_time_approaches_ground_state():
+    """Imaginary-time two-site TDVP (real prefactor) lowers the energy toward
+    the exact ground state of the L=8 Heisenberg chain."""
+    from oracle import tdvp as ot
+    L = 8
+    mpo = om.makeMPO(om.heisenbergMPO(), 2, L)
+    psi = omps.product_mps([i % 2 for i in range(L)], 2)
+    E0 = np.linalg.eigvalsh(om.fullH(mpo))[0]
+    Lenv = Renv = None
+    energies = []
+    for _ in range(30):
+        E, Lenv, Renv = ot.tdvp_twosite(psi, mpo, m=16, cutoff=1e-12, r=6, prefactor=-2.0 * (L - 1),
+                                        Lenv=Lenv, Renv=Renv)
+        energies.append(od.expect(psi, mpo) / od.expect(psi, [np.eye(2)[None, :, :, None]] * L))
+    assert energies[-1] < energies[0]
+    assert abs(energies[-1] - E0) < 1e-6
