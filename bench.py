@@ -54,6 +54,12 @@ def main():
     ap.add_argument("--max-grow", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--graph", action="store_true",
+                    help="capture each Lanczos step into a hipGraph (kernels inside graphs are not timed "
+                         "individually, so the roofline then covers the environment-update GEMMs only)")
+    ap.add_argument("--no-autotune", action="store_true", help="fixed tile-shape heuristic instead of the autotuner")
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_k_gemm_f64.json"),
+                    help="HBM bytes per k_gemm_f64 launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run one block-copy kernel over this many doubles (counter calibration)")
     args = ap.parse_args()
@@ -71,6 +77,10 @@ def main():
         log("copy kernel: %d doubles read and written (%.1f MiB each way)"
             % (args.calibrate_copy, args.calibrate_copy * 8 / 2 ** 20))
         return
+    if args.no_autotune:
+        ctx.set_autotune(0)
+    if args.graph:
+        ctx.set_graph(1)
     L, m = args.L, args.m
     rng = np.random.default_rng(0)
     states = list(rng.permutation([0] * (L // 2) + [1] * (L - L // 2)))
@@ -99,15 +109,20 @@ def main():
     ctx.profile(True)
     ctx.sync()
     ctx.flops(reset=True)
-    for k in ("gemm_f64", "mix", "copy", "svd"):
+    for k in ("gemm_f64", "mix", "copy", "svd", "gemm_lp", "graph"):
         ctx.kernel_stats(k, reset=True)
+    tune0 = ctx.autotune_stats()
+    ctx.marker(1)       # kernel-trace cut: start of the timed region
     t0 = time.perf_counter()
     pkg.dmrg(psi, mpo, m=m, sweeps=args.steps, cutoff=args.cutoff, params=params, precision=args.precision)
     ctx.sync()
     t1 = time.perf_counter()
+    ctx.marker(2)       # end of the timed region
+    ctx.sync()
+    tune1 = ctx.autotune_stats()
     elapsed = t1 - t0
     fl_heff, fl_env = ctx.flops()
-    stats = {k: ctx.kernel_stats(k) for k in ("gemm_f64", "mix", "copy", "svd")}
+    stats = {k: ctx.kernel_stats(k) for k in ("gemm_f64", "mix", "copy", "svd", "gemm_lp", "graph")}
     ctx.profile(False)
     for h in params.history[-args.steps:]:
         log("timed sweep E=%.12f maxtrunc=%.3e m=%d t=%.3fs" % (h["energy"], h["maxtrunc"], h["biggestm"],
@@ -122,6 +137,14 @@ def main():
                         stats["mix"]["bytes"] / max(1e-9, stats["mix"]["ms"] * 1e-3) / 1e9,
                         stats["copy"]["ms"] / 1e3, stats["svd"]["ms"] / 1e3))
 
+    traffic, traffic_note = None, None
+    if os.path.exists(args.traffic_file) and args.precision == "f64":
+        with open(args.traffic_file) as f:
+            tr = json.load(f)
+        if tr.get("L") == L and tr.get("m") == m and tr.get("kernel") == "k_gemm_f64":
+            traffic = tr["fetch_bytes_per_launch"] + tr["write_bytes_per_launch"]
+            traffic_note = ("rocprofv3 FETCH_SIZE + WRITE_SIZE per k_gemm_f64 launch in the timed region, "
+                            "scaled by the copy-kernel calibration (%s)" % os.path.basename(args.traffic_file))
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(pkg, psi, mpo, params, args.cpu_seconds)
@@ -145,9 +168,14 @@ def main():
                    "sweep_s": sweep_s},
         "roofline": {"bound": "mfma", "kernel": "k_gemm_f64 (FP64 grouped GEMM, MFMA 16x16x4)",
                      "achieved": round(g_tflops, 3), "peak": PEAK_FP64, "unit": "TFLOP/s",
-                     "frac": round(g_tflops / PEAK_FP64, 4), "traffic": None,
+                     "frac": round(g_tflops / PEAK_FP64, 4), "traffic": traffic,
+                     "traffic_note": traffic_note,
                      "avg_launch_ms": round(g_avg_ms, 5), "launches": g["launches"],
-                     "flops_per_launch": g["flops"] / max(1, g["launches"])},
+                     "flops_per_launch": g["flops"] / max(1, g["launches"]),
+                     "compulsory_bytes_per_launch": g["bytes"] / max(1, g["launches"])},
+        "autotune": {"keys": tune1["keys"], "tuned_in_timed_region": tune1["tuned"] - tune0["tuned"],
+                     "hits_in_timed_region": tune1["hits"] - tune0["hits"]},
+        "graph": ctx.graph_stats() if args.graph else None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

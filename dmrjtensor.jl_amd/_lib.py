@@ -30,6 +30,11 @@ SIGNATURES = {
     "dmrj_ctx_sync": [_p],
     "dmrj_ctx_memstats": [_p, _i64p],
     "dmrj_ctx_set_gemm_cfg": [_p, C.c_int],
+    "dmrj_ctx_set_autotune": [_p, C.c_int],
+    "dmrj_ctx_autotune_stats": [_p, _dp],
+    "dmrj_ctx_set_graph": [_p, C.c_int],
+    "dmrj_ctx_marker": [_p, C.c_int],
+    "dmrj_ctx_graph_stats": [_p, _dp],
     "dmrj_ctx_flops": [_p, _dp, C.c_int],
     "dmrj_ctx_profile": [_p, C.c_int],
     "dmrj_ctx_set_svd_backend": [_p, C.c_int],
@@ -149,7 +154,7 @@ class Context:
     def profile(self, enable=True):
         self.call("dmrj_ctx_profile", 1 if enable else 0)
 
-    KINDS = {"gemm_f64": 0, "mix": 1, "copy": 2, "svd": 3, "gemm_lp": 4}
+    KINDS = {"gemm_f64": 0, "mix": 1, "copy": 2, "svd": 3, "gemm_lp": 4, "graph": 5}
 
     def kernel_stats(self, kind, reset=False):
         out = (C.c_double * 4)()
@@ -161,6 +166,29 @@ class Context:
 
     def set_gemm_cfg(self, cfg):
         self.call("dmrj_ctx_set_gemm_cfg", int(cfg))
+
+    def set_autotune(self, mode):
+        """1 = autotune GEMM tile shapes per block-size histogram, 0 = heuristic, 2 = clear cache."""
+        self.call("dmrj_ctx_set_autotune", int(mode))
+
+    def autotune_stats(self):
+        out = np.zeros(5)
+        self.call("dmrj_ctx_autotune_stats", dptr(out))
+        return dict(keys=int(out[0]), hits=int(out[1]), tuned=int(out[2]), untunable=int(out[3]),
+                    tune_ms=float(out[4]))
+
+    def marker(self, ident):
+        """Enqueue the empty k_region_marker kernel (cuts kernel traces)."""
+        self.call("dmrj_ctx_marker", int(ident))
+
+    def set_graph(self, mode):
+        """1 = capture each Lanczos step into a hipGraph, 0 = direct launches."""
+        self.call("dmrj_ctx_set_graph", int(mode))
+
+    def graph_stats(self):
+        out = np.zeros(2)
+        self.call("dmrj_ctx_graph_stats", dptr(out))
+        return dict(instantiations=int(out[0]), updates=int(out[1]))
 
     def close(self):
         if getattr(self, "h", None):

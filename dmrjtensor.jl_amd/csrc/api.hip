@@ -512,17 +512,101 @@ void rebase_plan(HeffPlan& p, const double* oldx, const double* newx, int64_t nx
     if (t.C >= oldy && t.C < oldy + ny) t.C = newy + (t.C - oldy);
 }
 
-void run_heff(Context& c, const HeffPlan& p, int prec = DMRJ_PREC_F64) {
-  if (prec == DMRJ_PREC_BF16X3) {
-    launch_gemm_lp(c, p.tA, p.sA, -1);
-    launch_mix(c, p.md, p.mt);
-    launch_gemm_lp(c, p.tC, p.sC, -1);
-  } else {
-    launch_gemm(c, p.tA, p.sA, -1);
-    launch_mix(c, p.md, p.mt);
-    launch_gemm(c, p.tC, p.sC, -1);
+// One H_eff application with its descriptor tables on the device.
+struct HeffLaunch {
+  GemmLaunch a, c;
+  MixLaunch m;
+  bool lp = false;
+  double fa = 0, fc = 0, ba = 0, bc = 0, flops_alg = 0;
+};
+
+HeffLaunch prepare_heff(Context& c, const HeffPlan& p, int prec) {
+  HeffLaunch h;
+  h.lp = prec == DMRJ_PREC_BF16X3;
+  h.a = prepare_gemm(c, p.tA, p.sA, p.tA.empty() ? 0 : pick_gemm_cfg(c, p.tA, p.sA, h.lp), h.lp);
+  h.m = prepare_mix(c, p.md, p.mt);
+  h.c = prepare_gemm(c, p.tC, p.sC, p.tC.empty() ? 0 : pick_gemm_cfg(c, p.tC, p.sC, h.lp), h.lp);
+  h.fa = gemm_flops(p.tA, p.sA);
+  h.fc = gemm_flops(p.tC, p.sC);
+  h.ba = gemm_bytes(p.tA, p.sA);
+  h.bc = gemm_bytes(p.tC, p.sC);
+  h.flops_alg = p.flops_alg;
+  return h;
+}
+
+// Issues the three kernels; with per-kernel timing unless the stream is
+// being captured into a graph.
+void issue_heff(Context& c, const HeffLaunch& h, bool timed) {
+  const int kind = h.lp ? KIND_GEMM_LP : KIND_GEMM_F64;
+  auto gemm = [&](const GemmLaunch& g, double f, double b) {
+    if (g.ntiles <= 0) return;
+    const int ps = timed ? c.prof_begin() : -1;
+    if (h.lp)
+      issue_gemm_lp(c, g);
+    else
+      issue_gemm_f64(c, g);
+    if (ps >= 0) c.prof_end(ps, kind, f, b);
+  };
+  gemm(h.a, h.fa, h.ba);
+  if (h.m.nwork > 0) {
+    const int ps = timed ? c.prof_begin() : -1;
+    issue_mix(c, h.m);
+    if (ps >= 0) c.prof_end(ps, KIND_MIX, h.m.flops, h.m.bytes);
   }
-  c.flops_heff += p.flops_alg;
+  gemm(h.c, h.fc, h.bc);
+  c.flops_heff += h.flops_alg;
+}
+
+void run_heff(Context& c, const HeffPlan& p, int prec = DMRJ_PREC_F64) {
+  issue_heff(c, prepare_heff(c, p, prec), true);
+}
+
+// Captures body() on the context stream into a graph and launches it; the
+// executable graph of the previous call is updated in place when the
+// topology matches (same kernels, new arguments), else re-instantiated.
+template <class F>
+void run_captured(Context& c, F&& body, double flops) {
+  HIP_OK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+  try {
+    body();
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    if (hipStreamEndCapture(c.stream, &g) == hipSuccess && g) hipGraphDestroy(g);
+    throw;
+  }
+  hipGraph_t g = nullptr;
+  HIP_OK(hipStreamEndCapture(c.stream, &g));
+  bool ok = false;
+  if (c.graph_exec) {
+    // the previous launch of this executable graph must have finished before
+    // its kernel arguments are rewritten
+    HIP_OK(hipEventSynchronize(c.graph_done));
+    hipGraphNode_t err_node = nullptr;
+    hipGraphExecUpdateResult res;
+    ok = hipGraphExecUpdate(c.graph_exec, g, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+    if (!ok) {
+      (void)hipGetLastError();
+      hipGraphExecDestroy(c.graph_exec);
+      c.graph_exec = nullptr;
+    } else {
+      c.graph_updates += 1;
+    }
+  }
+  if (!ok) {
+    hipError_t e = hipGraphInstantiate(&c.graph_exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      hipGraphDestroy(g);
+      c.graph_exec = nullptr;
+      throw Error(DMRJ_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    c.graph_instantiations += 1;
+  }
+  const int ps = c.prof_begin();
+  hipError_t e = hipGraphLaunch(c.graph_exec, c.stream);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) throw Error(DMRJ_ERR_DEVICE, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+  HIP_OK(hipEventRecord(c.graph_done, c.stream));
+  if (ps >= 0) c.prof_end(ps, KIND_GRAPH, flops, 0.0);
 }
 
 int base_prec(int precision) {
@@ -1131,49 +1215,62 @@ dmrj_tensor* lanczos2(Context& c, const Tensor& L, const Tensor& R, const Tensor
   double* y2 = (double*)c.alloc.alloc(std::max<int64_t>(n, 1) * sizeof(double));
   double *T1 = nullptr, *T3 = nullptr;
   HeffPlan p = plan_heff(c, x, th->data, y1, L.env, L.data, R.env, R.data, f, &T1, &T3);
-  // H psi (#1)
-  run_heff(c, p, prec);
-  {
-    const double* xs[2] = {th->data, th->data};
-    const double* ys[2] = {th->data, y1};
-    launch_dots(c, 2, xs, ys, n, c.scal + S_TT);
-  }
-  launch_lz_scalar(c, 1);
-  launch_axpy_dev(c, y1, th->data, n, c.scal + S_CAXPY);  // y1 := psi2
-  {
-    const double* xs[1] = {y1};
-    launch_dots(c, 1, xs, xs, n, c.scal + S_PP);
-  }
-  launch_lz_scalar(c, 2);
-  launch_scale_dev(c, y1, n, c.scal + S_INVB);
-  // H psi2 (#2)
+  // descriptor tables of every H_eff application of the step go to the
+  // device first, so the step itself is a fixed kernel sequence
+  HeffLaunch h1 = prepare_heff(c, p, prec);          // H psi
   rebase_plan(p, th->data, y1, n, y1, y2, n);
-  run_heff(c, p, prec);
-  {
-    const double* xs[1] = {y1};
-    const double* ys[1] = {y2};
-    launch_dots(c, 1, xs, ys, n, c.scal + S_PHP);
-  }
-  launch_lz_scalar(c, 3);
-  launch_comb_dev(c, y2, th->data, y1, n, c.scal + S_CA, c.scal + S_CB);
+  HeffLaunch h2 = prepare_heff(c, p, prec);          // H psi2
+  HeffLaunch h3;
   if (correct) {
-    // FP64 residual correction of the energy: Rayleigh quotient of the Ritz
-    // vector with an FP64 H_eff application; the residual norm
-    // |H AA - E AA| is kept in S_RES
     rebase_plan(p, y1, y2, n, y2, y1, n);
-    run_heff(c, p, DMRJ_PREC_F64);
+    h3 = prepare_heff(c, p, DMRJ_PREC_F64);          // H AA' in FP64
+  }
+  const bool graph = c.graph_mode != 0;
+  auto step = [&]() {
+    issue_heff(c, h1, !graph);
     {
-      const double* xs[2] = {y2, y2};
-      const double* ys[2] = {y2, y1};
-      launch_dots(c, 2, xs, ys, n, c.scal + S_RR);
+      const double* xs[2] = {th->data, th->data};
+      const double* ys[2] = {th->data, y1};
+      launch_dots(c, 2, xs, ys, n, c.scal + S_TT);
     }
-    launch_lz_scalar(c, 4);
-    launch_axpy_dev(c, y1, y2, n, c.scal + S_NEGE);
+    launch_lz_scalar(c, 1);
+    launch_axpy_dev(c, y1, th->data, n, c.scal + S_CAXPY);  // y1 := psi2
     {
       const double* xs[1] = {y1};
-      launch_dots(c, 1, xs, xs, n, c.scal + S_RES);
+      launch_dots(c, 1, xs, xs, n, c.scal + S_PP);
     }
-  }
+    launch_lz_scalar(c, 2);
+    launch_scale_dev(c, y1, n, c.scal + S_INVB);
+    issue_heff(c, h2, !graph);
+    {
+      const double* xs[1] = {y1};
+      const double* ys[1] = {y2};
+      launch_dots(c, 1, xs, ys, n, c.scal + S_PHP);
+    }
+    launch_lz_scalar(c, 3);
+    launch_comb_dev(c, y2, th->data, y1, n, c.scal + S_CA, c.scal + S_CB);
+    if (correct) {
+      // FP64 residual correction of the energy: Rayleigh quotient of the Ritz
+      // vector with an FP64 H_eff application; the residual norm
+      // |H AA - E AA| is kept in S_RES
+      issue_heff(c, h3, !graph);
+      {
+        const double* xs[2] = {y2, y2};
+        const double* ys[2] = {y2, y1};
+        launch_dots(c, 2, xs, ys, n, c.scal + S_RR);
+      }
+      launch_lz_scalar(c, 4);
+      launch_axpy_dev(c, y1, y2, n, c.scal + S_NEGE);
+      {
+        const double* xs[1] = {y1};
+        launch_dots(c, 1, xs, xs, n, c.scal + S_RES);
+      }
+    }
+  };
+  if (graph)
+    run_captured(c, step, h1.flops_alg + h2.flops_alg + h3.flops_alg);
+  else
+    step();
   c.alloc.release(T1);
   c.alloc.release(T3);
   c.alloc.release(y1);
@@ -1385,6 +1482,7 @@ int dmrj_ctx_create(int device, dmrj_ctx** out) {
     HIP_OK(hipMalloc((void**)&c->partial, 4 * 256 * sizeof(double)));
     c->stage.cap = 8 << 20;
     HIP_OK(hipHostMalloc((void**)&c->stage.base, c->stage.cap, hipHostMallocDefault));
+    HIP_OK(hipEventCreateWithFlags(&c->graph_done, hipEventDisableTiming));
   } catch (const Error& e) {
     delete c;
     return e.code;
@@ -1398,6 +1496,8 @@ int dmrj_ctx_destroy(dmrj_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->release_transients();
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph_done) (void)hipEventDestroy(c->graph_done);
   if (c->rocblas) rocblas_destroy_handle((rocblas_handle)c->rocblas);
   if (c->scal) (void)hipFree(c->scal);
   if (c->partial) (void)hipFree(c->partial);
@@ -1458,8 +1558,53 @@ int dmrj_ctx_set_svd_backend(dmrj_ctx* c, int mode) {
 
 int dmrj_ctx_set_gemm_cfg(dmrj_ctx* c, int cfg) {
   return guard(c, [&] {
-    if (cfg < -1 || cfg > 2) throw Error(DMRJ_ERR_ARG, "gemm cfg must be -1..2");
+    if (cfg < -1 || cfg >= kNumGemmShapes) throw Error(DMRJ_ERR_ARG, "gemm cfg must be -1..4");
     c->gemm_cfg_override = cfg;
+  });
+}
+
+int dmrj_ctx_set_autotune(dmrj_ctx* c, int mode) {
+  return guard(c, [&] {
+    if (mode < 0 || mode > 2) throw Error(DMRJ_ERR_ARG, "autotune mode must be 0, 1 or 2");
+    if (mode == 2) {
+      c->tune_cache.clear();
+      c->tune_hits = c->tune_misses = c->tune_ms = c->tune_skipped = 0;
+      return;
+    }
+    c->autotune = mode;
+  });
+}
+
+int dmrj_ctx_marker(dmrj_ctx* c, int id) {
+  return guard(c, [&] {
+    if (id < 0) throw Error(DMRJ_ERR_ARG, "marker id must be >= 0");
+    launch_marker(*c, id);
+  });
+}
+
+int dmrj_ctx_set_graph(dmrj_ctx* c, int mode) {
+  return guard(c, [&] {
+    if (mode < 0 || mode > 1) throw Error(DMRJ_ERR_ARG, "graph mode must be 0 or 1");
+    c->graph_mode = mode;
+  });
+}
+
+int dmrj_ctx_graph_stats(dmrj_ctx* c, double* out2) {
+  return guard(c, [&] {
+    if (!out2) throw Error(DMRJ_ERR_ARG, "null output");
+    out2[0] = c->graph_instantiations;
+    out2[1] = c->graph_updates;
+  });
+}
+
+int dmrj_ctx_autotune_stats(dmrj_ctx* c, double* out5) {
+  return guard(c, [&] {
+    if (!out5) throw Error(DMRJ_ERR_ARG, "null output");
+    out5[0] = (double)c->tune_cache.size();
+    out5[1] = c->tune_hits;
+    out5[2] = c->tune_misses;
+    out5[3] = c->tune_skipped;
+    out5[4] = c->tune_ms;
   });
 }
 
@@ -1894,7 +2039,7 @@ int dmrj_selftest_gemm(dmrj_ctx* c, int cfg, int64_t M, int64_t N, int64_t K, in
   return guard(c, [&] {
     const int prec = (cfg >> 4) & 15;
     cfg &= 15;
-    if (!A || !B || !Cout || M <= 0 || N <= 0 || K < 0 || cfg < 0 || cfg > 2)
+    if (!A || !B || !Cout || M <= 0 || N <= 0 || K < 0 || cfg < 0 || cfg >= kNumGemmShapes)
       throw Error(DMRJ_ERR_ARG, "selftest_gemm: bad arguments");
     const int64_t na = std::max<int64_t>(M * K, 1), nb = std::max<int64_t>(K * N, 1);
     double* dA = (double*)c->alloc.alloc(na * sizeof(double));

@@ -1,0 +1,182 @@
+// Tile-shape selection for the persistent grouped GEMM.
+//
+// A contraction step hands the GEMM a list of tasks whose shapes follow the
+// quantum-number block sizes of the tensors involved.  The autotuner reduces
+// that list to a block-size-histogram key (flop-weighted mean log2 of M, N
+// and K in half-octave bins, the number of tasks and the total work in
+// octaves, the precision) and caches the fastest tile shape per key.  On a
+// miss it times every shape on the live operands with HIP events: the launch
+// is repeated, so this is done only when repeating it is harmless (beta = 0
+// everywhere and no output overlaps an input); otherwise the heuristic
+// choose_gemm_cfg decides.  Every shape computes each output element with the
+// same K order, so the choice changes timing only, never the result bits.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "dmrj.h"
+#include "internal.h"
+
+namespace dmrj {
+
+#define HIP_OK(x)                                                                    \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess) throw Error(-3, std::string("HIP: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+const TileShape kGemmShapes[kNumGemmShapes] = {{32, 32}, {64, 64}, {128, 128}, {64, 128}, {128, 64}};
+
+static int occupancy(Context& c, int cfg, bool lp) {
+  int& o = c.occ_cache[lp ? 1 : 0][cfg];
+  if (o == 0) o = lp ? gemm_occupancy_lp(cfg) : gemm_occupancy_f64(cfg);
+  return o;
+}
+
+GemmLaunch prepare_gemm(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs,
+                        int cfg, bool lp) {
+  if (cfg < 0 || cfg >= kNumGemmShapes) throw Error(DMRJ_ERR_ARG, "GEMM tile config out of range");
+  GemmLaunch g;
+  g.cfg = cfg;
+  const int bm = kGemmShapes[cfg].bm, bn = kGemmShapes[cfg].bn;
+  // tiles ordered by descending K so that the persistent grid balances
+  std::vector<std::pair<int64_t, int>> order;
+  order.reserve(tasks.size());
+  for (size_t i = 0; i < tasks.size(); ++i) {
+    int64_t k = 0;
+    for (int s = 0; s < tasks[i].nseg; ++s) k += segs[tasks[i].seg0 + s].K;
+    order.push_back({-(k + 1), (int)i});
+  }
+  std::stable_sort(order.begin(), order.end());
+  std::vector<GemmTile> tiles;
+  for (auto& o : order) {
+    const GemmTask& t = tasks[o.second];
+    for (int n0 = 0; n0 < t.N; n0 += bn)
+      for (int m0 = 0; m0 < t.M; m0 += bm) tiles.push_back({o.second, m0, n0, cfg});
+  }
+  if (tiles.empty()) return g;
+  g.tiles = (const GemmTile*)c.upload(tiles.data(), tiles.size() * sizeof(GemmTile));
+  g.tasks = (const GemmTask*)c.upload(tasks.data(), tasks.size() * sizeof(GemmTask));
+  g.segs = (const GemmSeg*)c.upload(segs.data(), std::max<size_t>(1, segs.size()) * sizeof(GemmSeg));
+  g.ntiles = (int)tiles.size();
+  // persistent grid: never more work-groups than can be resident at once
+  const int64_t resident = (int64_t)c.num_cu * occupancy(c, cfg, lp);
+  g.grid = (int)std::min<int64_t>((int64_t)tiles.size(), resident);
+  return g;
+}
+
+static uint64_t histogram_key(const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs, bool lp) {
+  double F = 0.0, lm = 0.0, ln = 0.0, lk = 0.0;
+  for (const auto& t : tasks) {
+    int64_t K = 0;
+    for (int s = 0; s < t.nseg; ++s) K += segs[t.seg0 + s].K;
+    const double f = (double)t.M * t.N * std::max<int64_t>(K, 1);
+    F += f;
+    lm += f * std::log2(std::max(t.M, 1));
+    ln += f * std::log2(std::max(t.N, 1));
+    lk += f * std::log2((double)std::max<int64_t>(K, 1));
+  }
+  auto bin = [](double v, int scale) { return (uint64_t)std::max(0, (int)std::lround(v * scale)) & 0xff; };
+  uint64_t key = 0;
+  if (F > 0) {
+    key |= bin(lm / F, 2);
+    key |= bin(ln / F, 2) << 8;
+    key |= bin(lk / F, 2) << 16;
+    key |= bin(std::log2(F), 1) << 24;
+  }
+  key |= bin(std::log2((double)tasks.size() + 1.0), 1) << 32;
+  key |= (uint64_t)(lp ? 1 : 0) << 40;
+  return key;
+}
+
+static bool repeatable(const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs) {
+  struct Range {
+    uintptr_t a, b;
+  };
+  std::vector<Range> in, out;
+  for (const auto& t : tasks) {
+    if (t.beta != 0.0) return false;
+    if (t.M <= 0 || t.N <= 0) continue;
+    out.push_back({(uintptr_t)t.C, (uintptr_t)(t.C + t.ldc * (t.N - 1) + t.M)});
+    for (int s = 0; s < t.nseg; ++s) {
+      const GemmSeg& g = segs[t.seg0 + s];
+      if (g.K <= 0) continue;
+      const bool tA = g.flags & 1, tB = g.flags & 2;
+      const int64_t ea = tA ? g.lda * (t.M - 1) + g.K : g.lda * (g.K - 1) + t.M;
+      const int64_t eb = tB ? g.ldb * (g.K - 1) + t.N : g.ldb * (t.N - 1) + g.K;
+      in.push_back({(uintptr_t)g.A, (uintptr_t)(g.A + ea)});
+      in.push_back({(uintptr_t)g.B, (uintptr_t)(g.B + eb)});
+    }
+  }
+  auto by_start = [](const Range& x, const Range& y) { return x.a < y.a; };
+  std::sort(in.begin(), in.end(), by_start);
+  // merge the input ranges, then test every output range against them
+  std::vector<Range> merged;
+  for (const auto& r : in) {
+    if (!merged.empty() && r.a <= merged.back().b)
+      merged.back().b = std::max(merged.back().b, r.b);
+    else
+      merged.push_back(r);
+  }
+  for (const auto& o : out) {
+    auto it = std::upper_bound(merged.begin(), merged.end(), Range{o.a, o.a}, by_start);
+    if (it != merged.end() && it->a < o.b) return false;
+    if (it != merged.begin() && std::prev(it)->b > o.a) return false;
+  }
+  return true;
+}
+
+int autotune_gemm(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs, bool lp) {
+  const uint64_t key = histogram_key(tasks, segs, lp);
+  auto it = c.tune_cache.find(key);
+  if (it != c.tune_cache.end()) {
+    c.tune_hits += 1;
+    return it->second;
+  }
+  if (!repeatable(tasks, segs)) {
+    c.tune_skipped += 1;
+    return choose_gemm_cfg(tasks, c.num_cu);
+  }
+  c.tune_misses += 1;
+  constexpr int kReps = 2;
+  hipEvent_t ev[kNumGemmShapes + 1];
+  for (auto& e : ev) HIP_OK(hipEventCreate(&e));
+  GemmLaunch g[kNumGemmShapes];
+  for (int cfg = 0; cfg < kNumGemmShapes; ++cfg) g[cfg] = prepare_gemm(c, tasks, segs, cfg, lp);
+  auto issue = [&](const GemmLaunch& x) {
+    if (lp)
+      issue_gemm_lp(c, x);
+    else
+      issue_gemm_f64(c, x);
+  };
+  issue(g[0]);  // warm the operands into the caches once
+  HIP_OK(hipEventRecord(ev[0], c.stream));
+  for (int cfg = 0; cfg < kNumGemmShapes; ++cfg) {
+    for (int r = 0; r < kReps; ++r) issue(g[cfg]);
+    HIP_OK(hipEventRecord(ev[cfg + 1], c.stream));
+  }
+  HIP_OK(hipEventSynchronize(ev[kNumGemmShapes]));
+  int best = choose_gemm_cfg(tasks, c.num_cu);
+  float best_ms = 1e30f, total = 0.f;
+  for (int cfg = 0; cfg < kNumGemmShapes; ++cfg) {
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, ev[cfg], ev[cfg + 1]));
+    total += ms;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = cfg;
+    }
+  }
+  for (auto& e : ev) HIP_OK(hipEventDestroy(e));
+  c.tune_ms += total;
+  c.tune_cache[key] = best;
+  return best;
+}
+
+int pick_gemm_cfg(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs, bool lp) {
+  if (c.gemm_cfg_override >= 0) return c.gemm_cfg_override;
+  return c.autotune ? autotune_gemm(c, tasks, segs, lp) : choose_gemm_cfg(tasks, c.num_cu);
+}
+
+}  // namespace dmrj

@@ -18,6 +18,7 @@
 
 #include <cstdint>
 #include <map>
+#include <unordered_map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -197,6 +198,17 @@ struct Context {
   int svd_backend = 0;
   int env_prec = 0;  // precision of environment updates
   int last_gemm_cfg = -1;
+  // grouped-GEMM autotuner: 1 = time the tile shapes per histogram key
+  int autotune = 1;
+  // 1 = each Lanczos step (both H_eff applications, inner products and the
+  // Ritz update) is captured into one hipGraph and launched as a unit
+  int graph_mode = 0;
+  hipGraphExec_t graph_exec = nullptr;
+  hipEvent_t graph_done = nullptr;
+  double graph_updates = 0, graph_instantiations = 0;
+  std::unordered_map<uint64_t, int> tune_cache;
+  int occ_cache[2][8] = {{0}};
+  double tune_hits = 0, tune_misses = 0, tune_ms = 0, tune_skipped = 0;
   // statistics
   double flops_heff = 0.0, flops_env = 0.0;
   // per-kernel-kind HIP-event timing on this context's stream
@@ -241,6 +253,7 @@ void launch_gemm(Context& c, const std::vector<GemmTask>& tasks, const std::vect
 void launch_mix(Context& c, const std::vector<MixDst>& dsts, const std::vector<MixTerm>& terms);
 void launch_copy(Context& c, const std::vector<CopyOp>& ops);
 void launch_fill(Context& c, double* p, int64_t n, double v);
+void launch_marker(Context& c, int id);
 void launch_dots(Context& c, int npairs, const double* const* xs, const double* const* ys,
                  int64_t n, double* out);
 void launch_axpy_dev(Context& c, double* y, const double* x, int64_t n, const double* coef);
@@ -254,6 +267,44 @@ void launch_mfma_probe(Context& c, double* out);
 void launch_mfma_bf16_probe(Context& c, float* out);
 void launch_gemm_lp(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs, int cfg);
 int choose_gemm_cfg(const std::vector<GemmTask>& tasks, int num_cu);
+
+// ---- grouped-GEMM tile shapes and the autotuner (autotune.hip) ----
+constexpr int kNumGemmShapes = 5;
+struct TileShape {
+  int bm, bn;
+};
+extern const TileShape kGemmShapes[kNumGemmShapes];
+// Descriptor tables of one grouped-GEMM launch, already on the device.
+struct GemmLaunch {
+  const GemmTile* tiles = nullptr;
+  const GemmTask* tasks = nullptr;
+  const GemmSeg* segs = nullptr;
+  int ntiles = 0, grid = 0, cfg = 0;
+};
+GemmLaunch prepare_gemm(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs,
+                        int cfg, bool lp);
+void issue_gemm_f64(Context& c, const GemmLaunch& g);
+void issue_gemm_lp(Context& c, const GemmLaunch& g);
+int gemm_occupancy_f64(int cfg);
+int gemm_occupancy_lp(int cfg);
+double gemm_flops(const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs);
+double gemm_bytes(const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs);
+// Tile shape for a task list: cached per block-size-histogram key; on a
+// miss every shape is timed on the live operands (only for launches that are
+// safe to repeat: beta = 0 and outputs disjoint from inputs).
+int autotune_gemm(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs, bool lp);
+// override > autotuner > heuristic
+int pick_gemm_cfg(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs, bool lp);
+// Descriptor tables of one mix launch, already on the device.
+struct MixLaunch {
+  const MixWork* work = nullptr;
+  const MixDst* dsts = nullptr;
+  const MixTerm* terms = nullptr;
+  int nwork = 0, grid = 0;
+  double flops = 0.0, bytes = 0.0;
+};
+MixLaunch prepare_mix(Context& c, const std::vector<MixDst>& dsts, const std::vector<MixTerm>& terms);
+void issue_mix(Context& c, const MixLaunch& m);
 struct SmallSvdJob {
   const double* M;
   double* U;
@@ -276,7 +327,7 @@ constexpr size_t kSmallSvdMaxLds = 60 * 1024;  // bytes of dynamic LDS per secto
 double gemm_flops(const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs);
 
 // kernel kinds for Context profiling
-enum KernelKind { KIND_GEMM_F64 = 0, KIND_MIX = 1, KIND_COPY = 2, KIND_SVD = 3, KIND_GEMM_LP = 4 };
+enum KernelKind { KIND_GEMM_F64 = 0, KIND_MIX = 1, KIND_COPY = 2, KIND_SVD = 3, KIND_GEMM_LP = 4, KIND_GRAPH = 5 };
 
 // Lanczos scalar slots in Context::scal
 enum Slot {

@@ -194,48 +194,50 @@ void launch_mfma_bf16_probe(Context& c, float* out) {
   if (e != hipSuccess) throw Error(-3, std::string("HIP: ") + hipGetErrorString(e));
 }
 
-void launch_gemm_lp(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs, int cfg) {
-  if (tasks.empty()) return;
-  if (cfg < 0) cfg = c.gemm_cfg_override >= 0 ? c.gemm_cfg_override : choose_gemm_cfg(tasks, c.num_cu);
-  const int bm = cfg == 0 ? 32 : (cfg == 1 ? 64 : 128);
-  const int bn = bm;
-  std::vector<std::pair<int64_t, int>> order;
-  for (size_t i = 0; i < tasks.size(); ++i) {
-    int64_t k = 0;
-    for (int s = 0; s < tasks[i].nseg; ++s) k += segs[tasks[i].seg0 + s].K;
-    order.push_back({-(k + 1), (int)i});
+template <int BM, int BN>
+static void gemm_lp_kernel_launch(Context& c, const GemmLaunch& g) {
+  hipLaunchKernelGGL((k_gemm_bf16x3<BM, BN>), dim3(g.grid), dim3(kLpThreads), 0, c.stream, g.tiles, g.ntiles,
+                     g.tasks, g.segs);
+}
+
+static const void* gemm_lp_kernel_ptr(int cfg) {
+  switch (cfg) {
+    case 0: return (const void*)k_gemm_bf16x3<32, 32>;
+    case 1: return (const void*)k_gemm_bf16x3<64, 64>;
+    case 2: return (const void*)k_gemm_bf16x3<128, 128>;
+    case 3: return (const void*)k_gemm_bf16x3<64, 128>;
+    default: return (const void*)k_gemm_bf16x3<128, 64>;
   }
-  std::stable_sort(order.begin(), order.end());
-  std::vector<GemmTile> tiles;
-  for (auto& o : order) {
-    const GemmTask& t = tasks[o.second];
-    for (int n0 = 0; n0 < t.N; n0 += bn)
-      for (int m0 = 0; m0 < t.M; m0 += bm) tiles.push_back({o.second, m0, n0, cfg});
+}
+
+int gemm_occupancy_lp(int cfg) {
+  int nb = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, gemm_lp_kernel_ptr(cfg), kLpThreads, 0);
+  if (e != hipSuccess) throw Error(-3, std::string("HIP: ") + hipGetErrorString(e));
+  return std::max(nb, 1);
+}
+
+void issue_gemm_lp(Context& c, const GemmLaunch& g) {
+  if (g.ntiles <= 0) return;
+  switch (g.cfg) {
+    case 0: gemm_lp_kernel_launch<32, 32>(c, g); break;
+    case 1: gemm_lp_kernel_launch<64, 64>(c, g); break;
+    case 2: gemm_lp_kernel_launch<128, 128>(c, g); break;
+    case 3: gemm_lp_kernel_launch<64, 128>(c, g); break;
+    default: gemm_lp_kernel_launch<128, 64>(c, g); break;
   }
-  if (tiles.empty()) return;
-  auto* dt = (GemmTile*)c.upload(tiles.data(), tiles.size() * sizeof(GemmTile));
-  auto* dk = (GemmTask*)c.upload(tasks.data(), tasks.size() * sizeof(GemmTask));
-  auto* ds = (GemmSeg*)c.upload(segs.data(), std::max<size_t>(1, segs.size()) * sizeof(GemmSeg));
-  const int occ = cfg == 2 ? 2 : 4;
-  const int grid = (int)std::min<int64_t>((int64_t)tiles.size(), (int64_t)c.num_cu * occ);
-  const int nt = (int)tiles.size();
-  const int ps = c.prof_begin();
-  if (cfg == 0)
-    hipLaunchKernelGGL((k_gemm_bf16x3<32, 32>), dim3(grid), dim3(kLpThreads), 0, c.stream, dt, nt, dk, ds);
-  else if (cfg == 1)
-    hipLaunchKernelGGL((k_gemm_bf16x3<64, 64>), dim3(grid), dim3(kLpThreads), 0, c.stream, dt, nt, dk, ds);
-  else
-    hipLaunchKernelGGL((k_gemm_bf16x3<128, 128>), dim3(grid), dim3(kLpThreads), 0, c.stream, dt, nt, dk, ds);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) throw Error(-3, std::string("HIP: ") + hipGetErrorString(e));
-  if (ps >= 0) {
-    double bytes = 0.0;
-    for (const auto& t : tasks) {
-      bytes += 8.0 * t.M * (double)t.N;
-      for (int q = 0; q < t.nseg; ++q) bytes += 8.0 * segs[t.seg0 + q].K * ((double)t.M + t.N);
-    }
-    c.prof_end(ps, KIND_GEMM_LP, gemm_flops(tasks, segs), bytes);
-  }
+}
+
+void launch_gemm_lp(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs, int cfg) {
+  if (tasks.empty()) return;
+  if (cfg < 0) cfg = pick_gemm_cfg(c, tasks, segs, true);
+  GemmLaunch g = prepare_gemm(c, tasks, segs, cfg, true);
+  if (g.ntiles == 0) return;
+  const int ps = c.prof_begin();
+  issue_gemm_lp(c, g);
+  if (ps >= 0) c.prof_end(ps, KIND_GEMM_LP, gemm_flops(tasks, segs), gemm_bytes(tasks, segs));
 }
 
 }  // namespace dmrj

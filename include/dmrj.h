@@ -63,6 +63,30 @@ int dmrj_version(void);
 int dmrj_ctx_memstats(dmrj_ctx* ctx, int64_t* out3);
 /* force a GEMM tile shape (0: 32x32, 1: 64x64, 2: 128x128, -1: automatic) */
 int dmrj_ctx_set_gemm_cfg(dmrj_ctx* ctx, int cfg);
+/* Grouped-GEMM tile-shape autotuner (no counterpart in the reference: it
+ * replaces the BLAS call's own blocking).  mode 1 (default) = time the five
+ * tile shapes (32x32, 64x64, 128x128, 64x128, 128x64) once per
+ * block-size-histogram key and reuse the fastest; 0 = fixed heuristic;
+ * 2 = clear the cache and counters.  The shape never changes result bits.
+ * A cfg set by dmrj_ctx_set_gemm_cfg takes precedence. */
+int dmrj_ctx_set_autotune(dmrj_ctx* ctx, int mode);
+/* out5 = {cached keys, cache hits, tuned misses, launches not tunable
+ *         (beta != 0 or aliasing), total tuning time in ms} */
+int dmrj_ctx_autotune_stats(dmrj_ctx* ctx, double* out5);
+/* Lanczos-step graphs (no counterpart in the reference).  mode 1: every
+ * dmrj_lanczos2 / dmrj_twostep Lanczos step (both H_eff applications, the
+ * inner products, the 2x2 Ritz solve on the device and the Ritz update, plus
+ * the FP64 correction when requested) is captured into one hipGraph and
+ * launched as a unit; consecutive steps update the executable graph in
+ * place when the kernel sequence matches.  0 (default) = direct launches.
+ * Results are identical in both modes. */
+int dmrj_ctx_set_graph(dmrj_ctx* ctx, int mode);
+/* out2 = {graph instantiations, in-place updates} */
+int dmrj_ctx_graph_stats(dmrj_ctx* ctx, double* out2);
+/* Enqueues k_region_marker (an empty one-wave kernel) on the context stream,
+ * so that a kernel trace can be cut to the region between two markers (no
+ * counterpart in the reference). */
+int dmrj_ctx_marker(dmrj_ctx* ctx, int id);
 /* SVD backend of dmrj_svd_trunc2 / dmrj_twostep: 0 (default) = batched
  * one-sided Jacobi for sectors with min(R,C) <= 64 and one batched symmetric
  * eigensolve of the augmented matrices [[0,M],[M^T,0]] for the larger ones;

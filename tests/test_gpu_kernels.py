@@ -35,7 +35,7 @@ def test_mfma_f64_lane_layout(pkg, ctx):
     assert np.array_equal(enc, want)
 
 
-@pytest.mark.parametrize("cfg", [0, 1, 2])
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("tA,tB", [(0, 0), (1, 0), (0, 1), (1, 1)])
 @pytest.mark.parametrize("M,N,K", [(1, 1, 1), (16, 16, 4), (33, 70, 17), (130, 129, 200), (5, 300, 0)])
 def test_grouped_gemm_f64(pkg, ctx, cfg, tA, tB, M, N, K):
@@ -123,7 +123,7 @@ def test_mfma_bf16_lane_layout(pkg, ctx):
     assert np.array_equal(enc, we)
 
 
-@pytest.mark.parametrize("cfg", [0, 1, 2])
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("tA,tB", [(0, 0), (1, 0), (0, 1), (1, 1)])
 @pytest.mark.parametrize("M,N,K", [(16, 16, 32), (33, 70, 17), (130, 129, 200)])
 def test_grouped_gemm_bf16x3(pkg, ctx, cfg, tA, tB, M, N, K):

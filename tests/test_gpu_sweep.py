@@ -109,3 +109,63 @@ def test_hubbard_L10_u1xu1_reference_constant(pkg, ctx):
         say(" sweep", h["sweep"], "%.15f" % h["energy"], "%.15f" % oh["energy"], "rel %.2e" % rel)
         assert rel <= 1e-12
     assert abs(E - (-25.380567747816)) < 1e-4
+
+
+def test_autotune_is_result_neutral(pkg, ctx):
+    """The grouped-GEMM autotuner picks tile shapes only: every shape sums K
+    in the same order, so per-sweep energies and spectra are bit-identical
+    with the tuner on and off."""
+    say("case autotune on/off U(1) L=32 m=96")
+    L = 32
+    qphys = np.array([[1], [-1]])
+    states = [i % 2 for i in range(L)]
+    qmpo = pkg.MPO(pkg.makeMPO(pkg.heisenbergMPO(), 2, L).H, qphys=qphys)
+    runs = []
+    for mode in (0, 1):
+        ctx.set_autotune(2)
+        ctx.set_autotune(mode)
+        psi = pkg.product_state(states, 2, qphys=qphys, ctx=ctx)
+        params = pkg.algvars(psi)
+        pkg.dmrg(psi, qmpo, m=96, sweeps=3, cutoff=1e-10, params=params)
+        st = ctx.autotune_stats()
+        say(" mode", mode, "E %.15f" % params.energy, st)
+        runs.append(params.history)
+        if mode == 1:
+            assert st["keys"] > 0 and st["tuned"] > 0
+            assert st["hits"] > 0
+    ctx.set_autotune(1)
+    for a, b in zip(*runs):
+        assert a["energy"] == b["energy"]
+        assert a["maxtrunc"] == b["maxtrunc"]
+        assert list(a["storeD"]) == list(b["storeD"])
+
+
+def test_graph_captured_lanczos_matches_direct(pkg, ctx):
+    """Lanczos steps captured into hipGraphs (instantiated once, then updated
+    in place bond after bond) give the same per-sweep energies, truncation
+    errors and spectra as direct launches, bit for bit, with the plain and
+    the FP64-corrected mixed-precision step."""
+    L = 24
+    qphys = np.array([[1], [-1]])
+    states = [i % 2 for i in range(L)]
+    qmpo = pkg.MPO(pkg.makeMPO(pkg.heisenbergMPO(), 2, L).H, qphys=qphys)
+    for precision, correction in (("f64", "none"), ("bf16x3", "bond")):
+        runs = []
+        for mode in (0, 1):
+            say("case graph mode", mode, precision, correction, "U(1) L=24 m=64")
+            ctx.set_graph(mode)
+            try:
+                psi = pkg.product_state(states, 2, qphys=qphys, ctx=ctx)
+                params = pkg.algvars(psi)
+                pkg.dmrg(psi, qmpo, m=64, sweeps=3, cutoff=1e-10, params=params, precision=precision,
+                         correction=correction)
+            finally:
+                ctx.set_graph(0)
+            runs.append(params.history)
+            say(" E %.15f" % params.energy, ctx.graph_stats())
+        st = ctx.graph_stats()
+        assert st["instantiations"] >= 1 and st["updates"] > 0
+        for a, b in zip(*runs):
+            assert a["energy"] == b["energy"]
+            assert a["maxtrunc"] == b["maxtrunc"]
+            assert list(a["storeD"]) == list(b["storeD"])
