@@ -47,6 +47,13 @@ SIGNATURES = {
                               C.POINTER(_p)],
     "dmrj_tensor_info": [_p, _i64p],
     "dmrj_tensor_to_dense": [_p, _p, _dp, _i32p, _i32p],
+    "dmrj_tensor_to_dense_z": [_p, _p, _dp, _i32p, _i32p],
+    "dmrj_site_from_dense_z": [_p, C.c_int, _i64, _i64, _i64, _dp, _i32p, _i32p, _i32p, C.POINTER(_p)],
+    "dmrj_site2_from_dense_z": [_p, C.c_int, _i64, _i64, _i64, _i64, _dp, _i32p, _i32p, _i32p, _i32p,
+                                C.POINTER(_p)],
+    "dmrj_tensor_is_complex": [_p],
+    "dmrj_tensor_to_complex": [_p, _p],
+    "dmrj_dotc": [_p, _p, _p, _dp],
     "dmrj_tensor_free": [_p, _p],
     "dmrj_tensor_copy": [_p, _p, C.POINTER(_p)],
     "dmrj_mpo_create": [_p, C.c_int, _i64, _i64, _i64, _dp, _i32p, _i32p, _i32p, C.POINTER(_p)],
@@ -58,9 +65,9 @@ SIGNATURES = {
     "dmrj_heff2_apply": [_p, _p, _p, _p, _p, _p, C.c_int, C.POINTER(_p)],
     "dmrj_heff1_apply": [_p, _p, _p, _p, _p, C.c_int, C.POINTER(_p)],
     "dmrj_lanczos2": [_p, _p, _p, _p, _p, _p, _p, C.c_int, C.POINTER(_p), _dp],
-    "dmrj_krylov_expm2": [_p, _p, _p, _p, _p, _p, C.c_int, C.c_double, C.c_int, C.POINTER(_p), _dp,
+    "dmrj_krylov_expm2": [_p, _p, _p, _p, _p, _p, C.c_int, C.c_double, C.c_double, C.c_int, C.POINTER(_p), _dp,
                           C.POINTER(C.c_int)],
-    "dmrj_krylov_expm1": [_p, _p, _p, _p, _p, C.c_int, C.c_double, C.c_int, C.POINTER(_p), _dp,
+    "dmrj_krylov_expm1": [_p, _p, _p, _p, _p, C.c_int, C.c_double, C.c_double, C.c_int, C.POINTER(_p), _dp,
                           C.POINTER(C.c_int)],
     "dmrj_svd_trunc2": [_p, _p, _i64, _i64, C.c_double, C.c_int, C.POINTER(_p), C.POINTER(_p), _dp,
                         _i64p, _dp, _i64],

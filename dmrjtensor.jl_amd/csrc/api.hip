@@ -279,7 +279,7 @@ dmrj_tensor* new_tensor(Context& c, int kind) {
 }
 
 void alloc_data(Context& c, Tensor& t) {
-  const int64_t n = t.nelem();
+  const int64_t n = t.nstore();
   t.data = (double*)c.alloc.alloc(std::max<int64_t>(n, 1) * sizeof(double));
 }
 
@@ -294,10 +294,12 @@ void check_finite_host(const double* p, int64_t n) {
 dmrj_tensor* site_from_dense(Context& c, int nq, int64_t mL, int64_t P, int64_t mR,
                              const double* A, const int32_t* qL,
                              const std::vector<Charge>& qp, const int32_t* qR, int d1, int d2,
-                             const std::vector<Charge>& qp1, const std::vector<Charge>& qp2) {
+                             const std::vector<Charge>& qp1, const std::vector<Charge>& qp2,
+                             const double* Aim = nullptr) {
   if (nq < 0 || nq > kMaxQ) throw Error(DMRJ_ERR_ARG, "nq must be 0, 1 or 2");
   if (mL <= 0 || P <= 0 || mR <= 0) throw Error(DMRJ_ERR_ARG, "bad dimensions");
   check_finite_host(A, mL * P * mR);
+  if (Aim) check_finite_host(Aim, mL * P * mR);
   auto* t = new_tensor(c, 0);
   t->nq = nq;
   SiteMeta& m = t->site;
@@ -310,7 +312,9 @@ dmrj_tensor* site_from_dense(Context& c, int nq, int64_t mL, int64_t P, int64_t 
   m.qp1 = qp1;
   m.qp2 = qp2;
   m.build();
-  std::vector<double> h(std::max<int64_t>(m.size, 1), 0.0);
+  t->cplx = Aim ? 1 : 0;
+  const int64_t plane = std::max<int64_t>(m.size, 1);
+  std::vector<double> h(plane * (Aim ? 2 : 1), 0.0);
   std::vector<char> used((size_t)(mL * P * mR), 0);
   for (const auto& b : m.blocks) {
     const int na = m.left.dim[b.a], nb = m.right.dim[b.b];
@@ -319,21 +323,22 @@ dmrj_tensor* site_from_dense(Context& c, int nq, int64_t mL, int64_t P, int64_t 
       for (int ra = 0; ra < na; ++ra) {
         const int64_t src = (oa + ra) + mL * (b.s + P * (ob + cb));
         h[b.off + ra + (int64_t)cb * na] = A[src];
+        if (Aim) h[plane + b.off + ra + (int64_t)cb * na] = Aim[src];
         used[src] = 1;
       }
   }
   double drop2 = 0.0, tot2 = 0.0;
   for (int64_t i = 0; i < mL * P * mR; ++i) {
-    tot2 += A[i] * A[i];
-    if (!used[i]) drop2 += A[i] * A[i];
+    const double a2 = A[i] * A[i] + (Aim ? Aim[i] * Aim[i] : 0.0);
+    tot2 += a2;
+    if (!used[i]) drop2 += a2;
   }
   if (drop2 > 1e-24 * std::max(1.0, tot2)) {
     delete t;
     throw Error(DMRJ_ERR_NUMERIC, "tensor has weight outside the charge-allowed blocks");
   }
   alloc_data(c, *t);
-  HIP_OK(hipMemcpyAsync(t->data, h.data(), std::max<int64_t>(m.size, 1) * sizeof(double),
-                        hipMemcpyHostToDevice, c.stream));
+  HIP_OK(hipMemcpyAsync(t->data, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
   HIP_OK(hipStreamSynchronize(c.stream));
   return t;
 }
@@ -404,7 +409,9 @@ struct HeffPlan {
 // allocated here (returned in the plan's pointers).
 HeffPlan plan_heff(Context& c, const SiteMeta& x, const double* xd, double* yd,
                    const EnvMeta& L, const double* Ld, const EnvMeta& R, const double* Rd,
-                   const FusedMpo& f, double** T1out, double** T3out) {
+                   const FusedMpo& f, double** T1out, double** T3out, int zmask = 0) {
+  // zmask: bit 0 = L complex, bit 1 = x complex, bit 2 = R complex
+  const int zf = zmask ? 2 : 1;
   if (!L.ket.same(x.left) || !L.bra.same(x.left))
     throw Error(DMRJ_ERR_ARG, "left environment does not match the tensor's left bond");
   if (!R.ket.same(x.right) || !R.bra.same(x.right))
@@ -434,8 +441,8 @@ HeffPlan plan_heff(Context& c, const SiteMeta& x, const double* xd, double* yd,
       t3pan[(size_t)ap * f.P + K] = p.t3;
       p.t3 += (int64_t)S.dim[ap] * R.panel_cols[bp];
     }
-  double* T1 = (double*)c.alloc.alloc(std::max<int64_t>(p.t1, 1) * sizeof(double));
-  double* T3 = (double*)c.alloc.alloc(std::max<int64_t>(p.t3, 1) * sizeof(double));
+  double* T1 = (double*)c.alloc.alloc(std::max<int64_t>(p.t1, 1) * zf * sizeof(double));
+  double* T3 = (double*)c.alloc.alloc(std::max<int64_t>(p.t3, 1) * zf * sizeof(double));
   *T1out = T1;
   *T3out = T3;
   // step A: T1(w,a) = L(ap,w) * Xpanel(a)
@@ -501,6 +508,20 @@ HeffPlan plan_heff(Context& c, const SiteMeta& x, const double* xd, double* yd,
   // middle step (4 D^2 d^3 mL mR for two sites); here: the mix terms counted
   // as dense over (w, J) per output element.
   p.flops_alg += p.flops_gemm;
+  if (zmask) {
+    BufMap bm;
+    bm.add(Ld, L.size, zmask & 1);
+    bm.add(xd, x.size, (zmask & 2) != 0);
+    bm.add(Rd, R.size, (zmask & 4) != 0);
+    bm.add(T1, p.t1, true);
+    bm.add(T3, p.t3, true);
+    bm.add(yd, x.size, true);
+    complexify_gemm(p.tA, p.sA, bm);
+    complexify_mix(p.md, p.mt, bm);
+    complexify_gemm(p.tC, p.sC, bm);
+    p.flops_gemm = gemm_flops(p.tA, p.sA) + gemm_flops(p.tC, p.sC);
+    p.flops_alg *= 4.0;  // one complex multiply-add = 8 real flops
+  }
   return p;
 }
 
@@ -629,7 +650,9 @@ struct EnvPlan {
 
 // Lupdate: Lout on A.right with D = W.DR.
 EnvPlan plan_left(Context& c, const EnvMeta& L, const double* Ld, const SiteMeta& A,
-                  const double* Ad, const MpoSite& W, EnvMeta& Lo, double* Lod) {
+                  const double* Ad, const MpoSite& W, EnvMeta& Lo, double* Lod, int zmask = 0) {
+  // zmask: bit 0 = L complex, bit 1 = A complex (then Lod holds two planes)
+  const int zf = zmask ? 2 : 1;
   if (!L.ket.same(A.left) || !L.bra.same(A.left))
     throw Error(DMRJ_ERR_ARG, "left environment does not match the MPS tensor");
   if (L.D != W.DL || A.P != W.d) throw Error(DMRJ_ERR_ARG, "MPO does not match environment/MPS");
@@ -663,8 +686,8 @@ EnvPlan plan_left(Context& c, const EnvMeta& L, const double* Ld, const SiteMeta
         ny += (int64_t)S.dim[ap] * T.dim[b];
       }
     }
-  p.X = (double*)c.alloc.alloc(std::max<int64_t>(nx, 1) * sizeof(double));
-  p.Y = (double*)c.alloc.alloc(std::max<int64_t>(ny, 1) * sizeof(double));
+  p.X = (double*)c.alloc.alloc(std::max<int64_t>(nx, 1) * zf * sizeof(double));
+  p.Y = (double*)c.alloc.alloc(std::max<int64_t>(ny, 1) * zf * sizeof(double));
   for (int w = 0; w < W.DL; ++w)
     for (int a = 0; a < nL; ++a) {
       const int64_t o = xoff[(size_t)w * nL + a];
@@ -719,18 +742,33 @@ EnvPlan plan_left(Context& c, const EnvMeta& L, const double* Ld, const SiteMeta
       if (ab < 0 || A.blocks[ab].b != bp) continue;
       const int64_t yo = yoff[((size_t)ap * A.P + k) * W.DR + u];
       if (yo < 0) continue;
-      p.s3.push_back({Ad + A.blocks[ab].off, p.Y + yo, S.dim[ap], S.dim[ap], S.dim[ap], 1, 1.0});
+      // bra tensor: enters complex-conjugated
+      p.s3.push_back({Ad + A.blocks[ab].off, p.Y + yo, S.dim[ap], S.dim[ap], S.dim[ap], 1 | kSegConjA, 1.0});
       t.nseg++;
     }
     p.t3.push_back(t);
   }
   p.flops += gemm_flops(p.t1, p.s1) + gemm_flops(p.t3, p.s3);
+  if (zmask) {
+    BufMap bm;
+    bm.add(Ld, L.size, zmask & 1);
+    bm.add(Ad, A.size, (zmask & 2) != 0);
+    bm.add(p.X, nx, true);
+    bm.add(p.Y, ny, true);
+    bm.add(Lod, Lo.size, true);
+    complexify_gemm(p.t1, p.s1, bm);
+    complexify_mix(p.md, p.mt, bm);
+    complexify_gemm(p.t3, p.s3, bm);
+    p.flops *= 4.0;
+  }
   return p;
 }
 
 // Rupdate: Rout (stored [bra, w, ket]) on A.left with D = W.DL.
 EnvPlan plan_right(Context& c, const EnvMeta& R, const double* Rd, const SiteMeta& A,
-                   const double* Ad, const MpoSite& W, EnvMeta& Ro, double* Rod) {
+                   const double* Ad, const MpoSite& W, EnvMeta& Ro, double* Rod, int zmask = 0) {
+  // zmask: bit 0 = R complex, bit 1 = A complex (then Rod holds two planes)
+  const int zf = zmask ? 2 : 1;
   if (!R.ket.same(A.right) || !R.bra.same(A.right))
     throw Error(DMRJ_ERR_ARG, "right environment does not match the MPS tensor");
   if (R.D != W.DR || A.P != W.d) throw Error(DMRJ_ERR_ARG, "MPO does not match environment/MPS");
@@ -760,8 +798,8 @@ EnvPlan plan_right(Context& c, const EnvMeta& R, const double* Rd, const SiteMet
       ypan[(size_t)ap * W.DL + w] = ny;
       ny += (int64_t)S.dim[ap] * A.panel_cols[a];
     }
-  p.X = (double*)c.alloc.alloc(std::max<int64_t>(nx, 1) * sizeof(double));
-  p.Y = (double*)c.alloc.alloc(std::max<int64_t>(ny, 1) * sizeof(double));
+  p.X = (double*)c.alloc.alloc(std::max<int64_t>(nx, 1) * zf * sizeof(double));
+  p.Y = (double*)c.alloc.alloc(std::max<int64_t>(ny, 1) * zf * sizeof(double));
   for (int ap = 0; ap < nL; ++ap)
     for (int k = 0; k < A.P; ++k) {
       const int64_t o = xpan[(size_t)ap * A.P + k];
@@ -769,7 +807,8 @@ EnvPlan plan_right(Context& c, const EnvMeta& R, const double* Rd, const SiteMet
       const SiteBlock& ab = A.blocks[A.block(ap, k)];
       const int bp = ab.b;
       p.t1.push_back({p.X + o, S.dim[ap], S.dim[ap], (int)R.panel_cols[bp], (int)p.s1.size(), 1, 0.0});
-      p.s1.push_back({Ad + ab.off, Rd + R.panel_off[bp], S.dim[ap], T.dim[bp], T.dim[bp], 0, 1.0});
+      // bra tensor: enters complex-conjugated
+      p.s1.push_back({Ad + ab.off, Rd + R.panel_off[bp], S.dim[ap], T.dim[bp], T.dim[bp], kSegConjA, 1.0});
     }
   for (int ap = 0; ap < nL; ++ap)
     for (int w = 0; w < W.DL; ++w) {
@@ -816,6 +855,18 @@ EnvPlan plan_right(Context& c, const EnvMeta& R, const double* Rd, const SiteMet
     p.t3.push_back(t);
   }
   p.flops += gemm_flops(p.t1, p.s1) + gemm_flops(p.t3, p.s3);
+  if (zmask) {
+    BufMap bm;
+    bm.add(Rd, R.size, zmask & 1);
+    bm.add(Ad, A.size, (zmask & 2) != 0);
+    bm.add(p.X, nx, true);
+    bm.add(p.Y, ny, true);
+    bm.add(Rod, Ro.size, true);
+    complexify_gemm(p.t1, p.s1, bm);
+    complexify_mix(p.md, p.mt, bm);
+    complexify_gemm(p.t3, p.s3, bm);
+    p.flops *= 4.0;
+  }
   return p;
 }
 
@@ -855,6 +906,7 @@ dmrj_tensor* make_theta(Context& c, const Tensor& AL, const Tensor& AR) {
   for (int j2 = 0; j2 < R.P; ++j2)
     for (int j1 = 0; j1 < L.P; ++j1) m.qp[j1 + L.P * j2] = L.qp[j1] + R.qp[j2];
   m.build();
+  t->cplx = (AL.cplx || AR.cplx) ? 1 : 0;
   alloc_data(c, *t);
   std::vector<GemmTask> tasks;
   std::vector<GemmSeg> segs;
@@ -873,6 +925,13 @@ dmrj_tensor* make_theta(Context& c, const Tensor& AL, const Tensor& AR) {
     }
     tasks.push_back(tk);
   }
+  if (t->cplx) {
+    BufMap bm;
+    bm.add(AL.data, L.size, AL.cplx != 0);
+    bm.add(AR.data, R.size, AR.cplx != 0);
+    bm.add(t->data, m.size, true);
+    complexify_gemm(tasks, segs, bm);
+  }
   launch_gemm(c, tasks, segs, -1);
   return t;
 }
@@ -888,6 +947,9 @@ struct SvdResult {
 SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, double cutoff, int dir) {
   const SiteMeta& x = AA.site;
   if (AA.kind != 0 || x.d1 == 0) throw Error(DMRJ_ERR_ARG, "svd_trunc2 needs a two-site tensor");
+  // complex tensors: every array below holds a real and an imaginary plane
+  const bool z = AA.cplx != 0;
+  const int zf = z ? 2 : 1;
   const int d1 = x.d1, d2 = x.d2;
   const Space& S = x.left;
   const Space& T = x.right;
@@ -935,9 +997,9 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
     Sec& s = secs[si];
     if (s.R == 0 || s.C == 0) continue;
     s.kmin = (int)std::min(s.R, s.C);
-    s.M = (double*)c.alloc.alloc(s.R * s.C * sizeof(double));
-    s.U = (double*)c.alloc.alloc(s.R * s.kmin * sizeof(double));
-    s.Vt = (double*)c.alloc.alloc((int64_t)s.kmin * s.C * sizeof(double));
+    s.M = (double*)c.alloc.alloc(s.R * s.C * zf * sizeof(double));
+    s.U = (double*)c.alloc.alloc(s.R * s.kmin * zf * sizeof(double));
+    s.Vt = (double*)c.alloc.alloc((int64_t)s.kmin * s.C * zf * sizeof(double));
     s.s = (double*)c.alloc.alloc(s.kmin * sizeof(double));
     s.E = (double*)c.alloc.alloc(std::max(1, s.kmin) * sizeof(double));
     total_s += s.kmin;
@@ -949,13 +1011,16 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
         if (xb < 0 || x.blocks[xb].b != b) throw Error(DMRJ_ERR_STATE, "two-site block table inconsistent");
         gather.push_back({AA.data + x.blocks[xb].off, s.M + s.roff[r] + s.coff[q] * s.R, S.dim[a], s.R,
                           S.dim[a], T.dim[b], nullptr, nullptr, 1.0});
+        if (z)
+          gather.push_back({AA.data + x.size + x.blocks[xb].off, s.M + s.R * s.C + s.roff[r] + s.coff[q] * s.R,
+                            S.dim[a], s.R, S.dim[a], T.dim[b], nullptr, nullptr, 1.0});
       }
   }
   launch_copy(c, gather);
   {
     // never hand non-finite data to the solver
     const double* xs[1] = {AA.data};
-    launch_dots(c, 1, xs, xs, x.size, c.scal + S_CHK);
+    launch_dots(c, 1, xs, xs, AA.nstore(), c.scal + S_CHK);
     double chk = 0.0;
     HIP_OK(hipMemcpyAsync(&c.hscal[S_CHK], c.scal + S_CHK, sizeof(double), hipMemcpyDeviceToHost, c.stream));
     HIP_OK(hipStreamSynchronize(c.stream));
@@ -982,7 +1047,7 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
   std::vector<char> is_small(secs.size(), 0);
   for (int si : order) {
     Sec& s = secs[si];
-    if (s.kmin == 0 || s.kmin > 64 || c.svd_backend == 1 || c.svd_backend == 2) continue;
+    if (z || s.kmin == 0 || s.kmin > 64 || c.svd_backend == 1 || c.svd_backend == 2) continue;
     const size_t l = small_svd_lds((int)s.R, (int)s.C);
     if (l > kSmallSvdMaxLds) continue;
     is_small[si] = 1;
@@ -1000,7 +1065,41 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
   double* augE = nullptr;
   rocblas_int* auginfo = nullptr;
   int naug = 0;
-  if (c.svd_backend == 0) {
+  if (z) {
+    // complex: every sector through one batched Hermitian eigensolve of
+    // [[0, M], [M^H, 0]] (eigenvalues +-sigma)
+    std::vector<int> big;
+    int N = 0;
+    for (int si : order) {
+      Sec& s = secs[si];
+      if (s.kmin == 0) continue;
+      big.push_back(si);
+      N = std::max<int>(N, (int)(s.R + s.C));
+    }
+    if (!big.empty()) {
+      naug = (int)big.size();
+      const int64_t NN = (int64_t)N * N;
+      aug = (double*)c.alloc.alloc(2 * NN * naug * sizeof(double));
+      augW = (double*)c.alloc.alloc((int64_t)N * naug * sizeof(double));
+      augE = (double*)c.alloc.alloc((int64_t)N * naug * sizeof(double));
+      auginfo = (rocblas_int*)c.alloc.alloc(sizeof(rocblas_int) * naug);
+      launch_fill(c, aug, 2 * NN * naug, 0.0);
+      std::vector<AugJobZ> jobs;
+      for (int b = 0; b < naug; ++b) {
+        Sec& s = secs[big[b]];
+        jobs.push_back({s.M, aug + 2 * b * NN, augW + (int64_t)b * N, s.U, s.s, s.Vt, (int)s.R, (int)s.C, N, s.kmin});
+        is_small[big[b]] = 1;  // handled
+        const double n2 = (double)N;
+        svd_flops += 4.0 * 9.0 * n2 * n2 * n2;
+      }
+      launch_aug_place_z(c, jobs);
+      rocblas_status st = rocsolver_zheevd_strided_batched(hb, rocblas_evect_original, rocblas_fill_upper, N,
+                                                           (rocblas_double_complex*)aug, N, NN, augW, N, augE, N,
+                                                           auginfo, naug);
+      if (st != rocblas_status_success) throw Error(DMRJ_ERR_SOLVER, "rocsolver_zheevd_strided_batched failed");
+      launch_aug_extract_z(c, jobs);
+    }
+  } else if (c.svd_backend == 0) {
     std::vector<int> big;
     int N = 0;
     for (int si : order) {
@@ -1078,7 +1177,7 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
   c.alloc.release(augE);
   c.alloc.release(auginfo);
   for (int i = 0; i < naug; ++i)
-    if (hainfo[i] != 0) throw Error(DMRJ_ERR_SOLVER, "rocsolver_dsyevd did not converge");
+    if (hainfo[i] != 0) throw Error(DMRJ_ERR_SOLVER, "rocsolver symmetric eigensolver did not converge");
   c.release_transients();
   c.stage_reset();
   for (int i = 0; i < nsvd; ++i)
@@ -1161,6 +1260,7 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
   tr->site.P = d2;
   tr->site.qp = x.qp2;
   tr->site.build();
+  tl->cplx = tr->cplx = z ? 1 : 0;
   alloc_data(c, *tl);
   alloc_data(c, *tr);
   std::vector<CopyOp> split;
@@ -1172,6 +1272,9 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
       if (s.rows[r].first == b.a && s.rows[r].second == b.s) ro = s.roff[r];
     split.push_back({s.U + ro, tl->data + b.off, s.R, S.dim[b.a], S.dim[b.a], s.keep, nullptr,
                      dir < 0 ? dscal + soff[qidx[Q]] : nullptr, 1.0});
+    if (z)
+      split.push_back({s.U + s.R * s.kmin + ro, tl->data + tl->site.size + b.off, s.R, S.dim[b.a], S.dim[b.a],
+                       s.keep, nullptr, dir < 0 ? dscal + soff[qidx[Q]] : nullptr, 1.0});
   }
   for (const SiteBlock& b : tr->site.blocks) {
     const Charge Q = mid.q[b.a];
@@ -1181,6 +1284,9 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
       if (s.cols[q].first == b.s && s.cols[q].second == b.b) co = s.coff[q];
     split.push_back({s.Vt + co * s.kmin, tr->data + b.off, s.kmin, s.keep, s.keep, T.dim[b.b],
                      dir > 0 ? dscal + soff[qidx[Q]] : nullptr, nullptr, 1.0});
+    if (z)
+      split.push_back({s.Vt + (int64_t)s.kmin * s.C + co * s.kmin, tr->data + tr->site.size + b.off, s.kmin, s.keep,
+                       s.keep, T.dim[b.b], dir > 0 ? dscal + soff[qidx[Q]] : nullptr, nullptr, 1.0});
   }
   launch_copy(c, split);
   for (auto& s : secs) {
@@ -1207,6 +1313,7 @@ dmrj_tensor* lanczos2(Context& c, const Tensor& L, const Tensor& R, const Tensor
   const int prec = base_prec(precision);
   const bool correct = (precision & DMRJ_PREC_CORRECT) != 0;
   if (L.kind != 1 || R.kind != 2) throw Error(DMRJ_ERR_ARG, "lanczos2: bad environment handles");
+  if (L.cplx || R.cplx || AL.cplx || AR.cplx) throw Error(DMRJ_ERR_ARG, "lanczos2 takes real tensors");
   dmrj_tensor* th = make_theta(c, AL, AR);
   const SiteMeta& x = th->site;
   const int64_t n = x.size;
@@ -1329,57 +1436,78 @@ void tridiag_eig(int k, const double* a, const double* b, std::vector<double>& l
 // exp(M, prefactor) as used by krylovTimeEvol, algs/time.jl:182-202).
 // Returns the lowest Ritz value in *energy and the Krylov dimension used.
 dmrj_tensor* krylov_expm(Context& c, const Tensor& L, const Tensor& R, const Tensor& x, const FusedMpo& f, int r,
-                         double pf, int prec, double* energy, int* kdim) {
+                         double pf_re, double pf_im, int prec, double* energy, int* kdim) {
   if (r < 1 || r > 64) throw Error(DMRJ_ERR_ARG, "Krylov dimension must be 1..64");
   const SiteMeta& m = x.site;
   const int64_t n = std::max<int64_t>(m.size, 1);
-  double* V = (double*)c.alloc.alloc((size_t)n * r * sizeof(double));
-  double* w = (double*)c.alloc.alloc((size_t)n * sizeof(double));
+  // Krylov vectors are complex when the start vector or an environment is;
+  // a complex prefactor alone only makes the final combination complex
+  const bool zv = x.cplx || L.cplx || R.cplx;
+  const bool zout = zv || pf_im != 0.0;
+  const int64_t nv = zv ? 2 * n : n;         // stride of one Krylov vector
+  const int64_t ne = zv ? 2 * m.size : m.size;  // stored elements of one vector
+  double* V = (double*)c.alloc.alloc((size_t)nv * r * sizeof(double));
+  double* w = (double*)c.alloc.alloc((size_t)nv * sizeof(double));
   double* sc = (double*)c.alloc.alloc(256 * sizeof(double));  // alpha[0..63], beta[64..127], work[128..]
   launch_fill(c, sc, 256, 0.0);
   // v0 = x / |x|
   {
     const double* xs[1] = {x.data};
-    launch_dots(c, 1, xs, xs, m.size, c.scal + S_OC);
+    launch_dots(c, 1, xs, xs, x.nstore(), c.scal + S_OC);
     launch_lz_scalar(c, 0);
-    HIP_OK(hipMemcpyAsync(V, x.data, m.size * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-    launch_scale_dev(c, V, m.size, c.scal + S_INVOC);
+    if (zv && !x.cplx) launch_fill(c, V + n, m.size, 0.0);
+    HIP_OK(hipMemcpyAsync(V, x.data, x.nstore() * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    launch_scale_dev(c, V, ne, c.scal + S_INVOC);
   }
   double *T1 = nullptr, *T3 = nullptr;
-  HeffPlan p = plan_heff(c, m, V, w, L.env, L.data, R.env, R.data, f, &T1, &T3);
+  const int zmask = zv ? ((L.cplx ? 1 : 0) | 2 | (R.cplx ? 4 : 0)) : 0;
+  HeffPlan p = plan_heff(c, m, V, w, L.env, L.data, R.env, R.data, f, &T1, &T3, zmask);
   const double* px = V;
   for (int j = 0; j < r; ++j) {
-    double* vj = V + (size_t)j * n;
-    rebase_plan(p, px, vj, n, w, w, n);
+    double* vj = V + (size_t)j * nv;
+    rebase_plan(p, px, vj, nv, w, w, nv);
     px = vj;
     run_heff(c, p, prec);
-    // alpha_j = <v_j, w>;  w -= alpha_j v_j + beta_{j-1} v_{j-1}; full reorthogonalisation
+    // alpha_j = Re <v_j, w> (real for a Hermitian H_eff)
     {
       const double* xs[1] = {vj};
       const double* ys[1] = {w};
-      launch_dots(c, 1, xs, ys, m.size, sc + j);
+      launch_dots(c, 1, xs, ys, ne, sc + j);
     }
     if (j == r - 1) break;
-    for (int i = 0; i <= j; ++i) {  // two passes of classical Gram-Schmidt
+    // w -= <v_i, w> v_i for all i <= j: two passes of classical Gram-Schmidt
+    // (this removes alpha_j v_j and beta_{j-1} v_{j-1} as well)
+    for (int i = 0; i <= j; ++i) {
+      const double* vi = V + (size_t)i * nv;
       for (int pass = 0; pass < 2; ++pass) {
-        const double* xs[1] = {V + (size_t)i * n};
-        const double* ys[1] = {w};
-        launch_dots(c, 1, xs, ys, m.size, sc + 128);
-        launch_scalar_op(c, sc, 0, 128, 129);
-        launch_axpy_dev(c, w, V + (size_t)i * n, m.size, sc + 129);
+        if (zv) {
+          const double* xr[1] = {vi};
+          const double* yr[1] = {w};
+          launch_dots(c, 1, xr, yr, ne, sc + 140);              // Re <vi, w>
+          const double* xc[2] = {vi, vi + n};
+          const double* yc[2] = {w + n, w};
+          launch_dots(c, 2, xc, yc, m.size, sc + 141);          // vi_r.w_i, vi_i.w_r
+          launch_caxpy_dev(c, w, vi, m.size, sc + 140);
+        } else {
+          const double* xs[1] = {vi};
+          const double* ys[1] = {w};
+          launch_dots(c, 1, xs, ys, m.size, sc + 128);
+          launch_scalar_op(c, sc, 0, 128, 129);
+          launch_axpy_dev(c, w, vi, m.size, sc + 129);
+        }
       }
     }
     {
       const double* xs[1] = {w};
-      launch_dots(c, 1, xs, xs, m.size, sc + 130);
+      launch_dots(c, 1, xs, xs, ne, sc + 130);
     }
     // beta_j = |w|, v_{j+1} = w / beta_j (zero when the Krylov space is exhausted)
     HIP_OK(hipMemcpyAsync(c.scal + S_PP, sc + 130, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
     launch_lz_scalar(c, 2);
     HIP_OK(hipMemcpyAsync(sc + 64 + j, c.scal + S_B1, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-    double* vn = V + (size_t)(j + 1) * n;
-    HIP_OK(hipMemcpyAsync(vn, w, m.size * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-    launch_scale_dev(c, vn, m.size, c.scal + S_INVB);
+    double* vn = V + (size_t)(j + 1) * nv;
+    HIP_OK(hipMemcpyAsync(vn, w, ne * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    launch_scale_dev(c, vn, ne, c.scal + S_INVB);
   }
   std::vector<double> hs(128);
   HIP_OK(hipMemcpyAsync(hs.data(), sc, 128 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
@@ -1397,20 +1525,45 @@ dmrj_tensor* krylov_expm(Context& c, const Tensor& L, const Tensor& R, const Ten
   }
   std::vector<double> lam, Q;
   tridiag_eig(k, hs.data(), hs.data() + 64, lam, Q);
-  std::vector<double> coef(k, 0.0);
+  // c = Q exp(pf Lambda) Q^T e1, pf complex
+  std::vector<double> cre(k, 0.0), cim(k, 0.0);
   double emin = lam[0];
   for (int i = 0; i < k; ++i) emin = std::min(emin, lam[i]);
   for (int i = 0; i < k; ++i) {
-    const double g = std::exp(pf * lam[i]) * Q[0 + (size_t)i * k];  // (Q^T e1)_i scaled
-    for (int jj = 0; jj < k; ++jj) coef[jj] += Q[jj + (size_t)i * k] * g;
+    const double mag = std::exp(pf_re * lam[i]) * Q[0 + (size_t)i * k];
+    const double gr = mag * std::cos(pf_im * lam[i]), gi = mag * std::sin(pf_im * lam[i]);
+    for (int jj = 0; jj < k; ++jj) {
+      cre[jj] += Q[jj + (size_t)i * k] * gr;
+      cim[jj] += Q[jj + (size_t)i * k] * gi;
+    }
   }
   auto* out = new dmrj_tensor();
   out->kind = 0;
   out->nq = x.nq;
   out->site = m;
+  out->cplx = zout ? 1 : 0;
   alloc_data(c, *out);
-  double* dc = (double*)c.upload(coef.data(), coef.size() * sizeof(double));
-  launch_combine(c, out->data, V, n, m.size, dc, k);
+  if (!zv) {
+    double* dcr = (double*)c.upload(cre.data(), k * sizeof(double));
+    launch_combine(c, out->data, V, n, m.size, dcr, k);
+    if (zout) {
+      double* dci = (double*)c.upload(cim.data(), k * sizeof(double));
+      launch_combine(c, out->data + m.size, V, n, m.size, dci, k);
+    }
+  } else {
+    // planes of the Krylov basis: plane 2j = Re v_j, plane 2j+1 = Im v_j, stride n
+    std::vector<double> a(2 * k), b(2 * k);
+    for (int jj = 0; jj < k; ++jj) {
+      a[2 * jj] = cre[jj];
+      a[2 * jj + 1] = -cim[jj];
+      b[2 * jj] = cim[jj];
+      b[2 * jj + 1] = cre[jj];
+    }
+    double* da = (double*)c.upload(a.data(), a.size() * sizeof(double));
+    double* db = (double*)c.upload(b.data(), b.size() * sizeof(double));
+    launch_combine(c, out->data, V, n, m.size, da, 2 * k);
+    launch_combine(c, out->data + m.size, V, n, m.size, db, 2 * k);
+  }
   c.alloc.release(T1);
   c.alloc.release(T3);
   c.alloc.release(V);
@@ -1660,6 +1813,56 @@ int dmrj_site2_from_dense(dmrj_ctx* c, int nq, int64_t mL, int64_t d1, int64_t d
   });
 }
 
+static void split_planes(const double* z, int64_t n, std::vector<double>& re, std::vector<double>& im) {
+  re.resize(n);
+  im.resize(n);
+  for (int64_t i = 0; i < n; ++i) {
+    re[i] = z[2 * i];
+    im[i] = z[2 * i + 1];
+  }
+}
+
+int dmrj_site_from_dense_z(dmrj_ctx* c, int nq, int64_t mL, int64_t d, int64_t mR, const double* Az,
+                           const int32_t* qL, const int32_t* qP, const int32_t* qR, dmrj_tensor** out) {
+  return guard(c, [&] {
+    if (!Az || !out || mL <= 0 || d <= 0 || mR <= 0) throw Error(DMRJ_ERR_ARG, "bad arguments");
+    std::vector<double> re, im;
+    split_planes(Az, mL * d * mR, re, im);
+    *out = site_from_dense(*c, nq, mL, d, mR, re.data(), qL, charges_vec(qP, nq, d), qR, 0, 0, {}, {}, im.data());
+  });
+}
+
+int dmrj_site2_from_dense_z(dmrj_ctx* c, int nq, int64_t mL, int64_t d1, int64_t d2, int64_t mR,
+                            const double* Az, const int32_t* qL, const int32_t* qP1, const int32_t* qP2,
+                            const int32_t* qR, dmrj_tensor** out) {
+  return guard(c, [&] {
+    if (!Az || !out || mL <= 0 || d1 <= 0 || d2 <= 0 || mR <= 0) throw Error(DMRJ_ERR_ARG, "bad arguments");
+    auto q1 = charges_vec(qP1, nq, d1), q2 = charges_vec(qP2, nq, d2);
+    std::vector<Charge> qp(d1 * d2);
+    for (int64_t j2 = 0; j2 < d2; ++j2)
+      for (int64_t j1 = 0; j1 < d1; ++j1) qp[j1 + d1 * j2] = q1[j1] + q2[j2];
+    std::vector<double> re, im;
+    split_planes(Az, mL * d1 * d2 * mR, re, im);
+    *out = site_from_dense(*c, nq, mL, d1 * d2, mR, re.data(), qL, qp, qR, (int)d1, (int)d2, q1, q2, im.data());
+  });
+}
+
+int dmrj_tensor_is_complex(const dmrj_tensor* t) { return t ? (t->cplx ? 1 : 0) : DMRJ_ERR_ARG; }
+
+int dmrj_tensor_to_complex(dmrj_ctx* c, dmrj_tensor* t) {
+  return guard(c, [&] {
+    if (!t) throw Error(DMRJ_ERR_ARG, "null pointer");
+    if (t->cplx) return;
+    const int64_t n = std::max<int64_t>(t->nelem(), 1);
+    double* d = (double*)c->alloc.alloc(2 * n * sizeof(double));
+    HIP_OK(hipMemcpyAsync(d, t->data, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    launch_fill(*c, d + n, n, 0.0);
+    c->alloc.release(t->data);
+    t->data = d;
+    t->cplx = 1;
+  });
+}
+
 int dmrj_tensor_info(const dmrj_tensor* t, int64_t* d) {
   if (!t || !d) return DMRJ_ERR_ARG;
   for (int i = 0; i < 9; ++i) d[i] = 0;
@@ -1695,52 +1898,69 @@ int dmrj_tensor_info(const dmrj_tensor* t, int64_t* d) {
   return DMRJ_OK;
 }
 
-int dmrj_tensor_to_dense(dmrj_ctx* c, const dmrj_tensor* t, double* out, int32_t* qfirst, int32_t* qlast) {
-  return guard(c, [&] {
-    if (!t || !out) throw Error(DMRJ_ERR_ARG, "null pointer");
-    const int64_t n = t->nelem();
-    std::vector<double> h(std::max<int64_t>(n, 1));
-    if (n > 0) HIP_OK(hipMemcpyAsync(h.data(), t->data, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_OK(hipStreamSynchronize(c->stream));
-    c->release_transients();
-    c->stage_reset();
-    if (t->kind == 0) {
-      const SiteMeta& m = t->site;
-      const int64_t mL = m.left.total, mR = m.right.total, P = m.P;
-      std::fill(out, out + mL * P * mR, 0.0);
+// Dense export; S = 1: real output, S = 2: interleaved complex output.
+static void to_dense_impl(dmrj_ctx* c, const dmrj_tensor* t, double* out, int32_t* qfirst, int32_t* qlast, int S) {
+  const int64_t n = t->nelem();
+  const int planes = t->cplx ? 2 : 1;
+  std::vector<double> h(std::max<int64_t>(n * planes, 1));
+  if (n > 0) HIP_OK(hipMemcpyAsync(h.data(), t->data, n * planes * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  c->release_transients();
+  c->stage_reset();
+  const int np = std::min(planes, S);
+  if (t->kind == 0) {
+    const SiteMeta& m = t->site;
+    const int64_t mL = m.left.total, mR = m.right.total, P = m.P;
+    std::fill(out, out + mL * P * mR * S, 0.0);
+    for (int p = 0; p < np; ++p)
       for (const auto& b : m.blocks) {
         const int na = m.left.dim[b.a], nb = m.right.dim[b.b];
         const int64_t oa = m.left.off[b.a], ob = m.right.off[b.b];
         for (int cb = 0; cb < nb; ++cb)
           for (int ra = 0; ra < na; ++ra)
-            out[(oa + ra) + mL * (b.s + P * (ob + cb))] = h[b.off + ra + (int64_t)cb * na];
+            out[((oa + ra) + mL * (b.s + P * (ob + cb))) * S + p] = h[p * n + b.off + ra + (int64_t)cb * na];
       }
-      space_to_list(m.left, t->nq, qfirst);
-      space_to_list(m.right, t->nq, qlast);
-    } else {
-      const EnvMeta& m = t->env;
-      const int64_t nb = m.bra.total, nk = m.ket.total, D = m.D;
-      std::fill(out, out + nb * D * nk, 0.0);
+    space_to_list(m.left, t->nq, qfirst);
+    space_to_list(m.right, t->nq, qlast);
+  } else {
+    const EnvMeta& m = t->env;
+    const int64_t nb = m.bra.total, nk = m.ket.total, D = m.D;
+    std::fill(out, out + nb * D * nk * S, 0.0);
+    for (int p = 0; p < np; ++p)
       for (const auto& b : m.blocks) {
         const int nap = m.bra.dim[b.ap], na = m.ket.dim[b.a];
         const int64_t oap = m.bra.off[b.ap], oa = m.ket.off[b.a];
         for (int cc = 0; cc < na; ++cc)
           for (int r = 0; r < nap; ++r) {
-            const double v = h[b.off + r + (int64_t)cc * nap];
+            const double v = h[p * n + b.off + r + (int64_t)cc * nap];
             if (t->kind == 1)
-              out[(oap + r) + nb * (b.w + D * (oa + cc))] = v;   // Lenv[bra, w, ket]
+              out[((oap + r) + nb * (b.w + D * (oa + cc))) * S + p] = v;   // Lenv[bra, w, ket]
             else
-              out[(oa + cc) + nk * (b.w + D * (oap + r))] = v;   // Renv[ket, u, bra]
+              out[((oa + cc) + nk * (b.w + D * (oap + r))) * S + p] = v;   // Renv[ket, u, bra]
           }
       }
-      if (t->kind == 1) {
-        space_to_list(m.bra, t->nq, qfirst);
-        space_to_list(m.ket, t->nq, qlast);
-      } else {
-        space_to_list(m.ket, t->nq, qfirst);
-        space_to_list(m.bra, t->nq, qlast);
-      }
+    if (t->kind == 1) {
+      space_to_list(m.bra, t->nq, qfirst);
+      space_to_list(m.ket, t->nq, qlast);
+    } else {
+      space_to_list(m.ket, t->nq, qfirst);
+      space_to_list(m.bra, t->nq, qlast);
     }
+  }
+}
+
+int dmrj_tensor_to_dense(dmrj_ctx* c, const dmrj_tensor* t, double* out, int32_t* qfirst, int32_t* qlast) {
+  return guard(c, [&] {
+    if (!t || !out) throw Error(DMRJ_ERR_ARG, "null pointer");
+    if (t->cplx) throw Error(DMRJ_ERR_ARG, "tensor is complex: export it with dmrj_tensor_to_dense_z");
+    to_dense_impl(c, t, out, qfirst, qlast, 1);
+  });
+}
+
+int dmrj_tensor_to_dense_z(dmrj_ctx* c, const dmrj_tensor* t, double* out, int32_t* qfirst, int32_t* qlast) {
+  return guard(c, [&] {
+    if (!t || !out) throw Error(DMRJ_ERR_ARG, "null pointer");
+    to_dense_impl(c, t, out, qfirst, qlast, 2);
   });
 }
 
@@ -1759,8 +1979,9 @@ int dmrj_tensor_copy(dmrj_ctx* c, const dmrj_tensor* t, dmrj_tensor** out) {
     n->nq = t->nq;
     n->site = t->site;
     n->env = t->env;
+    n->cplx = t->cplx;
     alloc_data(*c, *n);
-    HIP_OK(hipMemcpyAsync(n->data, t->data, std::max<int64_t>(t->nelem(), 1) * sizeof(double),
+    HIP_OK(hipMemcpyAsync(n->data, t->data, std::max<int64_t>(t->nstore(), 1) * sizeof(double),
                           hipMemcpyDeviceToDevice, c->stream));
     *out = n;
   });
@@ -1831,8 +2052,10 @@ int dmrj_env_left(dmrj_ctx* c, const dmrj_tensor* Lin, const dmrj_tensor* A, con
     lo.delta = W->dR;
     lo.build();
     t->env = lo;
+    const int zm = (Lin->cplx ? 1 : 0) | (A->cplx ? 2 : 0);
+    t->cplx = zm ? 1 : 0;
     alloc_data(*c, *t);
-    EnvPlan p = plan_left(*c, Lin->env, Lin->data, A->site, A->data, *W, t->env, t->data);
+    EnvPlan p = plan_left(*c, Lin->env, Lin->data, A->site, A->data, *W, t->env, t->data, zm);
     run_env(*c, p);
     *Lout = t;
   });
@@ -1852,8 +2075,10 @@ int dmrj_env_right(dmrj_ctx* c, const dmrj_tensor* Rin, const dmrj_tensor* A, co
     ro.delta = W->dL;
     ro.build();
     t->env = ro;
+    const int zm = (Rin->cplx ? 1 : 0) | (A->cplx ? 2 : 0);
+    t->cplx = zm ? 1 : 0;
     alloc_data(*c, *t);
-    EnvPlan p = plan_right(*c, Rin->env, Rin->data, A->site, A->data, *W, t->env, t->data);
+    EnvPlan p = plan_right(*c, Rin->env, Rin->data, A->site, A->data, *W, t->env, t->data, zm);
     run_env(*c, p);
     *Rout = t;
   });
@@ -1875,10 +2100,12 @@ int dmrj_heff2_apply(dmrj_ctx* c, const dmrj_tensor* L, const dmrj_mpo* W1, cons
     auto* y = new_tensor(*c, 0);
     y->nq = x->nq;
     y->site = x->site;
+    const int zm = (L->cplx ? 1 : 0) | (x->cplx ? 2 : 0) | (R->cplx ? 4 : 0);
+    y->cplx = zm ? 1 : 0;
     alloc_data(*c, *y);
     FusedMpo f = fuse_two(*W1, *W2);
     double *T1, *T3;
-    HeffPlan p = plan_heff(*c, x->site, x->data, y->data, L->env, L->data, R->env, R->data, f, &T1, &T3);
+    HeffPlan p = plan_heff(*c, x->site, x->data, y->data, L->env, L->data, R->env, R->data, f, &T1, &T3, zm);
     run_heff(*c, p, prec);
     c->alloc.release(T1);
     c->alloc.release(T3);
@@ -1895,10 +2122,12 @@ int dmrj_heff1_apply(dmrj_ctx* c, const dmrj_tensor* L, const dmrj_mpo* W, const
     auto* y = new_tensor(*c, 0);
     y->nq = x->nq;
     y->site = x->site;
+    const int zm = (L->cplx ? 1 : 0) | (x->cplx ? 2 : 0) | (R->cplx ? 4 : 0);
+    y->cplx = zm ? 1 : 0;
     alloc_data(*c, *y);
     FusedMpo f = fuse_one(*W);
     double *T1, *T3;
-    HeffPlan p = plan_heff(*c, x->site, x->data, y->data, L->env, L->data, R->env, R->data, f, &T1, &T3);
+    HeffPlan p = plan_heff(*c, x->site, x->data, y->data, L->env, L->data, R->env, R->data, f, &T1, &T3, zm);
     run_heff(*c, p, prec);
     c->alloc.release(T1);
     c->alloc.release(T3);
@@ -1918,22 +2147,24 @@ int dmrj_lanczos2(dmrj_ctx* c, const dmrj_tensor* L, const dmrj_tensor* R, const
 }
 
 int dmrj_krylov_expm2(dmrj_ctx* c, const dmrj_tensor* L, const dmrj_tensor* R, const dmrj_tensor* AA,
-                      const dmrj_mpo* W1, const dmrj_mpo* W2, int krylov, double prefactor, int precision,
-                      dmrj_tensor** out, double* energy, int* kdim) {
+                      const dmrj_mpo* W1, const dmrj_mpo* W2, int krylov, double prefactor_re,
+                      double prefactor_im, int precision, dmrj_tensor** out, double* energy, int* kdim) {
   return guard(c, [&] {
     if (!L || !R || !AA || !W1 || !W2 || !out || L->kind != 1 || R->kind != 2 || AA->kind != 0 || !AA->site.d1)
       throw Error(DMRJ_ERR_ARG, "krylov_expm2: bad handles");
-    *out = krylov_expm(*c, *L, *R, *AA, fuse_two(*W1, *W2), krylov, prefactor, base_prec(precision), energy, kdim);
+    *out = krylov_expm(*c, *L, *R, *AA, fuse_two(*W1, *W2), krylov, prefactor_re, prefactor_im, base_prec(precision),
+                       energy, kdim);
   });
 }
 
 int dmrj_krylov_expm1(dmrj_ctx* c, const dmrj_tensor* L, const dmrj_tensor* R, const dmrj_tensor* A,
-                      const dmrj_mpo* W, int krylov, double prefactor, int precision, dmrj_tensor** out,
-                      double* energy, int* kdim) {
+                      const dmrj_mpo* W, int krylov, double prefactor_re, double prefactor_im, int precision,
+                      dmrj_tensor** out, double* energy, int* kdim) {
   return guard(c, [&] {
     if (!L || !R || !A || !W || !out || L->kind != 1 || R->kind != 2 || A->kind != 0 || A->site.d1)
       throw Error(DMRJ_ERR_ARG, "krylov_expm1: bad handles");
-    *out = krylov_expm(*c, *L, *R, *A, fuse_one(*W), krylov, prefactor, base_prec(precision), energy, kdim);
+    *out = krylov_expm(*c, *L, *R, *A, fuse_one(*W), krylov, prefactor_re, prefactor_im, base_prec(precision), energy,
+                       kdim);
   });
 }
 
@@ -1956,9 +2187,9 @@ int dmrj_normalize(dmrj_ctx* c, dmrj_tensor* x) {
   return guard(c, [&] {
     if (!x) throw Error(DMRJ_ERR_ARG, "null pointer");
     const double* xs[1] = {x->data};
-    launch_dots(*c, 1, xs, xs, x->nelem(), c->scal + S_OC);
+    launch_dots(*c, 1, xs, xs, x->nstore(), c->scal + S_OC);
     launch_lz_scalar(*c, 0);
-    launch_scale_dev(*c, x->data, x->nelem(), c->scal + S_INVOC);
+    launch_scale_dev(*c, x->data, x->nstore(), c->scal + S_INVOC);
   });
 }
 
@@ -1968,6 +2199,8 @@ int dmrj_twostep(dmrj_ctx* c, const dmrj_tensor* L, const dmrj_tensor* R, dmrj_t
                  int64_t* kept, double* svals, int64_t cap) {
   return guard(c, [&] {
     if (!L || !R || !psiL || !psiR || !W1 || !W2 || !newL || !newR) throw Error(DMRJ_ERR_ARG, "null pointer");
+    if (L->cplx || R->cplx || psiL->cplx || psiR->cplx)
+      throw Error(DMRJ_ERR_ARG, "twostep: the DMRG step takes real tensors");
     dmrj_tensor* oc = dir > 0 ? psiL : psiR;
     {
       const double* xs[1] = {oc->data};
@@ -2000,11 +2233,48 @@ int dmrj_twostep(dmrj_ctx* c, const dmrj_tensor* L, const dmrj_tensor* R, dmrj_t
 int dmrj_dot(dmrj_ctx* c, const dmrj_tensor* x, const dmrj_tensor* y, double* out) {
   return guard(c, [&] {
     if (!x || !y || !out) throw Error(DMRJ_ERR_ARG, "null pointer");
-    if (x->nelem() != y->nelem()) throw Error(DMRJ_ERR_ARG, "dot: size mismatch");
+    if (x->nstore() != y->nstore()) throw Error(DMRJ_ERR_ARG, "dot: size mismatch");
     const double* xs[1] = {x->data};
     const double* ys[1] = {y->data};
-    launch_dots(*c, 1, xs, ys, x->nelem(), c->scal + S_TT);
+    launch_dots(*c, 1, xs, ys, x->nstore(), c->scal + S_TT);  // Re <x|y> for complex data
     *out = read_scalar(*c, S_TT);
+  });
+}
+
+int dmrj_dotc(dmrj_ctx* c, const dmrj_tensor* x, const dmrj_tensor* y, double* out2) {
+  return guard(c, [&] {
+    if (!x || !y || !out2) throw Error(DMRJ_ERR_ARG, "null pointer");
+    if (x->nelem() != y->nelem()) throw Error(DMRJ_ERR_ARG, "dotc: size mismatch");
+    const int64_t n = x->nelem();
+    const double* xr = x->data;
+    const double* yr = y->data;
+    // <x|y> = sum conj(x) y; a real operand has no imaginary plane
+    double re = 0.0, im = 0.0;
+    {
+      const double* xs[1] = {xr};
+      const double* ys[1] = {yr};
+      launch_dots(*c, 1, xs, ys, n, c->scal + S_TT);
+      re = read_scalar(*c, S_TT);
+    }
+    if (x->cplx && y->cplx) {
+      const double* xs[3] = {xr + n, xr, xr + n};
+      const double* ys[3] = {yr + n, yr + n, yr};
+      launch_dots(*c, 3, xs, ys, n, c->scal + S_A1);  // S_A1, S_CAXPY, S_B1
+      re += read_scalar(*c, S_A1);
+      im = read_scalar(*c, S_CAXPY) - read_scalar(*c, S_B1);
+    } else if (y->cplx) {
+      const double* xs[1] = {xr};
+      const double* ys[1] = {yr + n};
+      launch_dots(*c, 1, xs, ys, n, c->scal + S_A1);
+      im = read_scalar(*c, S_A1);
+    } else if (x->cplx) {
+      const double* xs[1] = {xr + n};
+      const double* ys[1] = {yr};
+      launch_dots(*c, 1, xs, ys, n, c->scal + S_A1);
+      im = -read_scalar(*c, S_A1);
+    }
+    out2[0] = re;
+    out2[1] = im;
   });
 }
 
@@ -2012,7 +2282,7 @@ int dmrj_norm(dmrj_ctx* c, const dmrj_tensor* x, double* out) {
   return guard(c, [&] {
     if (!x || !out) throw Error(DMRJ_ERR_ARG, "null pointer");
     const double* xs[1] = {x->data};
-    launch_dots(*c, 1, xs, xs, x->nelem(), c->scal + S_TT);
+    launch_dots(*c, 1, xs, xs, x->nstore(), c->scal + S_TT);
     *out = std::sqrt(read_scalar(*c, S_TT));
   });
 }
@@ -2021,16 +2291,16 @@ int dmrj_scale(dmrj_ctx* c, dmrj_tensor* x, double a) {
   return guard(c, [&] {
     if (!x) throw Error(DMRJ_ERR_ARG, "null pointer");
     double* tmp = (double*)c->upload(&a, sizeof(double));
-    launch_scale_dev(*c, x->data, x->nelem(), tmp);
+    launch_scale_dev(*c, x->data, x->nstore(), tmp);
   });
 }
 
 int dmrj_axpy(dmrj_ctx* c, double a, const dmrj_tensor* x, dmrj_tensor* y) {
   return guard(c, [&] {
     if (!x || !y) throw Error(DMRJ_ERR_ARG, "null pointer");
-    if (x->nelem() != y->nelem()) throw Error(DMRJ_ERR_ARG, "axpy: size mismatch");
+    if (x->nstore() != y->nstore()) throw Error(DMRJ_ERR_ARG, "axpy: size mismatch");
     double* tmp = (double*)c->upload(&a, sizeof(double));
-    launch_axpy_dev(*c, y->data, x->data, x->nelem(), tmp);
+    launch_axpy_dev(*c, y->data, x->data, x->nstore(), tmp);
   });
 }
 

@@ -241,11 +241,47 @@ struct MpoSite {
 struct Tensor {
   int kind = 0;  // 0 = site tensor, 1 = left env, 2 = right env
   int nq = 0;
+  int cplx = 0;  // 1: ComplexF64 held as two planes, real part then imaginary part
   SiteMeta site;
   EnvMeta env;
   double* data = nullptr;
-  int64_t nelem() const { return kind == 0 ? site.size : env.size; }
+  int64_t nelem() const { return kind == 0 ? site.size : env.size; }   // elements per plane
+  int64_t nstore() const { return nelem() * (cplx ? 2 : 1); }         // doubles stored
 };
+
+// ---- ComplexF64 on split planes (complex.hip) ----
+// A complex operand is stored as its real plane followed by its imaginary
+// plane.  A plan built for real data (pointers into real planes) is turned
+// into the complex plan by expanding every product into its real products;
+// operands that are real contribute no imaginary part.  GemmSeg flag bits
+// kSegConjA / kSegConjB mark operands that enter complex-conjugated (the bra
+// of an environment update); the GEMM kernels ignore them.
+constexpr int kSegConjA = 4, kSegConjB = 8;
+struct BufRef {
+  const double* base;
+  int64_t n;  // elements per plane
+  bool cplx;
+};
+struct BufMap {
+  std::vector<BufRef> b;
+  void add(const double* base, int64_t n, bool cplx) { b.push_back({base, n < 1 ? 1 : n, cplx}); }
+  const BufRef* find(const double* p) const;
+};
+void complexify_gemm(std::vector<GemmTask>& tasks, std::vector<GemmSeg>& segs, const BufMap& m);
+void complexify_mix(std::vector<MixDst>& dsts, std::vector<MixTerm>& terms, const BufMap& m);
+// w -= c v on complex vectors of n elements per plane, c = s[0] + i (s[1] - s[2])
+void launch_caxpy_dev(Context& c, double* w, const double* v, int64_t n, const double* s);
+struct AugJobZ {
+  const double* M;   // R x C, two planes, ld R
+  void* Z;           // N x N complex (interleaved), ld N
+  const double* W;   // eigenvalues, ascending
+  double* U;         // R x kmin, two planes, ld R
+  double* S;
+  double* Vt;        // kmin x C, two planes, ld kmin
+  int R, C, N, kmin;
+};
+void launch_aug_place_z(Context& c, const std::vector<AugJobZ>& jobs);
+void launch_aug_extract_z(Context& c, const std::vector<AugJobZ>& jobs);
 
 // ---- kernel launchers (kernels.hip) ----
 void launch_gemm(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs,

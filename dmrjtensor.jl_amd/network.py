@@ -49,15 +49,31 @@ class DeviceTensor:
     def nelem(self):
         return self.info()[5]
 
+    @property
+    def is_complex(self):
+        r = self.ctx.lib.dmrj_tensor_is_complex(self.h)
+        if r < 0:
+            raise _lib.DMRJError("dmrj_tensor_is_complex: null handle")
+        return bool(r)
+
+    def to_complex(self):
+        """Promote in place to ComplexF64 (zero imaginary part)."""
+        self.ctx.call("dmrj_tensor_to_complex", self.h)
+        return self
+
     def to_dense(self, nq=0):
         """Dense column-major copy with the reference's index order, plus the
-        charges of the first and last index (sector-sorted basis)."""
+        charges of the first and last index (sector-sorted basis).  Complex
+        tensors come back as complex128 arrays."""
         shp = self.shape
-        out = np.zeros(int(np.prod(shp)), dtype=np.float64)
+        z = self.is_complex
+        out = np.zeros(int(np.prod(shp)) * (2 if z else 1), dtype=np.float64)
         q0 = np.zeros((shp[0], max(nq, 1)), dtype=np.int32)
         q1 = np.zeros((shp[-1], max(nq, 1)), dtype=np.int32)
-        self.ctx.call("dmrj_tensor_to_dense", self.h, _lib.dptr(out),
+        self.ctx.call("dmrj_tensor_to_dense_z" if z else "dmrj_tensor_to_dense", self.h, _lib.dptr(out),
                       _lib.iptr(q0) if nq else None, _lib.iptr(q1) if nq else None)
+        if z:
+            out = out.view(np.complex128)
         arr = out.reshape(shp, order="F")
         return arr, (q0[:, :nq] if nq else None), (q1[:, :nq] if nq else None)
 
@@ -67,9 +83,16 @@ class DeviceTensor:
         return r.value
 
     def dot(self, other):
+        """Re <self|other>."""
         r = C.c_double()
         self.ctx.call("dmrj_dot", self.h, other.h, C.byref(r))
         return r.value
+
+    def dotc(self, other):
+        """<self|other> = sum conj(self) other (complex)."""
+        out = np.zeros(2)
+        self.ctx.call("dmrj_dotc", self.h, other.h, _lib.dptr(out))
+        return complex(out[0], out[1])
 
     def free(self):
         if self.h:
@@ -84,25 +107,31 @@ class DeviceTensor:
             pass
 
 
+def _host_buffer(A):
+    """Column-major float64 buffer; complex arrays as interleaved (re, im)."""
+    if np.iscomplexobj(A):
+        z = np.asfortranarray(np.asarray(A, dtype=np.complex128)).ravel(order="F")
+        return np.ascontiguousarray(z).view(np.float64), True
+    return np.asfortranarray(np.asarray(A, dtype=np.float64)).ravel(order="F"), False
+
+
 def site_from_dense(ctx, A, qL=None, qP=None, qR=None, nq=0):
-    A = np.asarray(A, dtype=np.float64)
-    mL, d, mR = A.shape
-    buf = np.asfortranarray(A).ravel(order="F")
+    mL, d, mR = np.shape(A)
+    buf, z = _host_buffer(A)
     h = C.c_void_p()
     ql = _lib.charges_arg(qL, mL, nq)
     qp = _lib.charges_arg(qP, d, nq)
     qr = _lib.charges_arg(qR, mR, nq)
-    ctx.call("dmrj_site_from_dense", nq, mL, d, mR, _lib.dptr(buf), _lib.iptr(ql), _lib.iptr(qp),
-             _lib.iptr(qr), C.byref(h))
+    ctx.call("dmrj_site_from_dense_z" if z else "dmrj_site_from_dense", nq, mL, d, mR, _lib.dptr(buf),
+             _lib.iptr(ql), _lib.iptr(qp), _lib.iptr(qr), C.byref(h))
     return DeviceTensor(ctx, h)
 
 
 def site2_from_dense(ctx, A, qL=None, qP1=None, qP2=None, qR=None, nq=0):
-    A = np.asarray(A, dtype=np.float64)
-    mL, d1, d2, mR = A.shape
-    buf = np.asfortranarray(A).ravel(order="F")
+    mL, d1, d2, mR = np.shape(A)
+    buf, z = _host_buffer(A)
     h = C.c_void_p()
-    ctx.call("dmrj_site2_from_dense", nq, mL, d1, d2, mR, _lib.dptr(buf),
+    ctx.call("dmrj_site2_from_dense_z" if z else "dmrj_site2_from_dense", nq, mL, d1, d2, mR, _lib.dptr(buf),
              _lib.iptr(_lib.charges_arg(qL, mL, nq)), _lib.iptr(_lib.charges_arg(qP1, d1, nq)),
              _lib.iptr(_lib.charges_arg(qP2, d2, nq)), _lib.iptr(_lib.charges_arg(qR, mR, nq)),
              C.byref(h))
@@ -434,4 +463,4 @@ def expect(psi, mpo):
         R = R2
     val, _, _ = R.to_dense(psi.nq)
     R.free()
-    return float(val.ravel()[0])
+    return float(np.real(val.ravel()[0]))

@@ -7,7 +7,8 @@ one call is one half-sweep.  Each bond: dmrj_theta2 -> dmrj_krylov_expm2
 (Krylov exponential of the two-site H_eff) -> dmrj_normalize ->
 dmrj_svd_trunc2 -> environment update -> dmrj_krylov_expm1 (one-site) ->
 dmrj_normalize of the new centre.  A real prefactor evolves in imaginary
-time.
+time; -i dt evolves in real time, and every tensor the evolution touches
+becomes ComplexF64 (real and imaginary planes on the device).
 """
 import ctypes as C
 
@@ -24,13 +25,15 @@ def _theta(ctx, a, b):
 
 
 def krylovTimeEvol2(Lenv, Renv, AA, mpo, iL, r, prefactor, precision="f64"):
-    """krylovTimeEvol with joinedtwosite_update (algs/time.jl:169-202)."""
+    """krylovTimeEvol with joinedtwosite_update (algs/time.jl:169-202);
+    complex prefactor (-i dt: real time)."""
     ctx = AA.ctx
     h = C.c_void_p()
     e = C.c_double()
     k = C.c_int()
+    pf = complex(prefactor)
     ctx.call("dmrj_krylov_expm2", Lenv.h, Renv.h, AA.h, mpo.device(ctx, iL), mpo.device(ctx, iL + 1), int(r),
-             float(prefactor), _lib.PREC[precision], C.byref(h), C.byref(e), C.byref(k))
+             pf.real, pf.imag, _lib.PREC[precision], C.byref(h), C.byref(e), C.byref(k))
     return DeviceTensor(ctx, h), e.value
 
 
@@ -40,7 +43,8 @@ def krylovTimeEvol1(Lenv, Renv, A, mpo, i, r, prefactor, precision="f64"):
     h = C.c_void_p()
     e = C.c_double()
     k = C.c_int()
-    ctx.call("dmrj_krylov_expm1", Lenv.h, Renv.h, A.h, mpo.device(ctx, i), int(r), float(prefactor),
+    pf = complex(prefactor)
+    ctx.call("dmrj_krylov_expm1", Lenv.h, Renv.h, A.h, mpo.device(ctx, i), int(r), pf.real, pf.imag,
              _lib.PREC[precision], C.byref(h), C.byref(e), C.byref(k))
     return DeviceTensor(ctx, h), e.value
 
@@ -64,11 +68,9 @@ def _svd(AA, m, minm, cutoff, direction):
 def tdvp_twosite(psi, mpo, m=0, minm=2, cutoff=0.0, r=2, prefactor=1.0, Lenv=None, Renv=None,
                  precision="f64"):
     """One half-sweep of two-site TDVP.  Returns (energy, Lenv, Renv); the
-    environments can be passed back in for the next half-sweep."""
-    if isinstance(prefactor, complex) and prefactor.imag != 0.0:
-        raise ValueError("complex prefactor (real-time evolution) needs complex tensors, "
-                         "which this build does not provide")
-    prefactor = float(np.real(prefactor))
+    environments can be passed back in for the next half-sweep.  A prefactor
+    -i dt evolves in real time: the tensors touched become ComplexF64."""
+    prefactor = complex(prefactor) if np.iscomplexobj(prefactor) else float(prefactor)
     Ns = len(psi)
     timestep = prefactor / (Ns - 1) / 2
     if Lenv is None or Renv is None:

@@ -130,6 +130,30 @@ int dmrj_tensor_to_dense(dmrj_ctx* ctx, const dmrj_tensor* t, double* out, int32
 int dmrj_tensor_free(dmrj_ctx* ctx, dmrj_tensor* t);
 int dmrj_tensor_copy(dmrj_ctx* ctx, const dmrj_tensor* t, dmrj_tensor** out);
 
+/* ---- ComplexF64 tensors (the TDVP path, algs/time.jl:288-362, evolves
+ * ComplexF64 MPS).  Host arrays are Julia's ComplexF64 layout: interleaved
+ * (re, im) pairs, column-major.  On the device a complex tensor keeps the
+ * block layout with its real plane followed by its imaginary plane.
+ * Environment updates, H_eff applications, dmrj_theta2, dmrj_svd_trunc2,
+ * the Krylov exponentials and the vector operations take complex tensors
+ * (mixing real and complex operands is allowed: a real operand has a zero
+ * imaginary part and the result is complex); dmrj_lanczos2 and
+ * dmrj_twostep (ground-state DMRG) take real tensors only.  dmrj_dot returns
+ * Re <x|y>; dmrj_dotc returns the complex <x|y> = sum conj(x) y. */
+int dmrj_site_from_dense_z(dmrj_ctx* ctx, int nq, int64_t mL, int64_t d, int64_t mR, const double* Az,
+                           const int32_t* qL, const int32_t* qP, const int32_t* qR, dmrj_tensor** out);
+int dmrj_site2_from_dense_z(dmrj_ctx* ctx, int nq, int64_t mL, int64_t d1, int64_t d2, int64_t mR,
+                            const double* Az, const int32_t* qL, const int32_t* qP1, const int32_t* qP2,
+                            const int32_t* qR, dmrj_tensor** out);
+/* interleaved complex export; a real tensor exports with zero imaginary parts */
+int dmrj_tensor_to_dense_z(dmrj_ctx* ctx, const dmrj_tensor* t, double* out, int32_t* qfirst, int32_t* qlast);
+/* 1 if complex, 0 if real, negative on a null handle */
+int dmrj_tensor_is_complex(const dmrj_tensor* t);
+/* in-place promotion to complex (zero imaginary plane); no-op if complex */
+int dmrj_tensor_to_complex(dmrj_ctx* ctx, dmrj_tensor* t);
+/* complex inner product, out2 = {Re, Im} of sum conj(x) y */
+int dmrj_dotc(dmrj_ctx* ctx, const dmrj_tensor* x, const dmrj_tensor* y, double* out2);
+
 /* ---- MPO ------------------------------------------------------------- */
 /* One MPO site W[w,j,k,u] (lib/MPO/makeMPO.jl:59-92; charges as
  * lib/MPO/makeqMPO.jl:23-93).  W column-major [DL, d, d, DR]; deltaL [DL][nq],
@@ -174,15 +198,16 @@ int dmrj_lanczos2(dmrj_ctx* ctx, const dmrj_tensor* L, const dmrj_tensor* R,
 /* krylovTimeEvol: Lanczos with `krylov` vectors (full FP64
  * reorthogonalisation) on AA, tridiagonal T, out = sum_j [exp(prefactor T)]_{j1}
  * v_j with v_1 = AA/|AA|; energy = lowest Ritz value, kdim = Krylov
- * dimension used (smaller when the space is exhausted).  Real prefactor
- * (imaginary-time evolution). */
+ * dimension used (smaller when the space is exhausted).  Complex prefactor
+ * prefactor_re + i prefactor_im: -i dt evolves in real time (complex output),
+ * a real prefactor in imaginary time. */
 int dmrj_krylov_expm2(dmrj_ctx* ctx, const dmrj_tensor* L, const dmrj_tensor* R, const dmrj_tensor* AA,
-                      const dmrj_mpo* W1, const dmrj_mpo* W2, int krylov, double prefactor, int precision,
-                      dmrj_tensor** out, double* energy, int* kdim);
+                      const dmrj_mpo* W1, const dmrj_mpo* W2, int krylov, double prefactor_re,
+                      double prefactor_im, int precision, dmrj_tensor** out, double* energy, int* kdim);
 /* Same for a one-site tensor with singlesite_update (algs/optimizeMPS.jl:503). */
 int dmrj_krylov_expm1(dmrj_ctx* ctx, const dmrj_tensor* L, const dmrj_tensor* R, const dmrj_tensor* A,
-                      const dmrj_mpo* W, int krylov, double prefactor, int precision, dmrj_tensor** out,
-                      double* energy, int* kdim);
+                      const dmrj_mpo* W, int krylov, double prefactor_re, double prefactor_im, int precision,
+                      dmrj_tensor** out, double* energy, int* kdim);
 
 /* ---- truncated SVD ---------------------------------------------------- */
 /* fsvd!(AA,[[1,2],[3,4]]; m, minm, cutoff, mag=1.) and the split of
