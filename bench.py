@@ -45,9 +45,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--L", type=int, default=100)
-    ap.add_argument("--m", type=int, default=1024)
-    ap.add_argument("--ramp", default="128,256,512")
+    ap.add_argument("--model", default="heisenberg", choices=["heisenberg", "hubbard", "j1j2"],
+                    help="heisenberg: chain L=100, U(1) Sz (default workload); hubbard: chain L=64, t=1, U=4, "
+                         "half filling, U(1)xU(1) (N, Sz); j1j2: width-4 cylinder, J2=0.5, U(1) Sz")
+    ap.add_argument("--L", type=int, default=None, help="sites (default 100 / 64 / 128 by model)")
+    ap.add_argument("--m", type=int, default=None, help="bond dimension (default 1024 / 2048 / 1024 by model)")
+    ap.add_argument("--ramp", default=None, help="ramp bond dimensions (default 128,256,512 / 256,512,1024)")
+    ap.add_argument("--svd-backend", type=int, default=None, help="dmrj_ctx_set_svd_backend mode")
     ap.add_argument("--cutoff", type=float, default=0.0,
                     help="SVD cutoff of the ramp and timed sweeps (0: bond dimension capped by m only)")
     ap.add_argument("--precision", default="f64")
@@ -81,12 +85,37 @@ def main():
         ctx.set_autotune(0)
     if args.graph:
         ctx.set_graph(1)
-    L, m = args.L, args.m
+    if args.svd_backend is not None:
+        ctx.set_svd_backend(args.svd_backend)
+    defaults = {"heisenberg": (100, 1024, "128,256,512"), "hubbard": (64, 2048, "256,512,1024"),
+                "j1j2": (128, 1024, "128,256,512")}[args.model]
+    L = args.L or defaults[0]
+    m = args.m or defaults[1]
+    if args.ramp is None:
+        args.ramp = defaults[2]
     rng = np.random.default_rng(0)
-    states = list(rng.permutation([0] * (L // 2) + [1] * (L - L // 2)))
-    qphys = np.array([[1], [-1]])
-    mpo = pkg.MPO(pkg.makeMPO(pkg.heisenbergMPO(), 2, L).H, qphys=qphys)
-    psi = pkg.product_state(states, 2, qphys=qphys, ctx=ctx)
+    if args.model == "hubbard":
+        # half filling, Sz = 0: a seeded permutation of L/2 up and L/2 down singly occupied sites
+        states = list(rng.permutation([1] * (L // 2) + [2] * (L - L // 2)))
+        qphys = np.array([[0, 0], [1, 1], [1, -1], [2, 0]])
+        mpo = pkg.MPO(pkg.makeMPO(pkg.hubbardMPO(), 4, L).H, qphys=qphys)
+        psi = pkg.product_state(states, 4, qphys=qphys, ctx=ctx)
+        desc = "Fermi-Hubbard L=%d t=1 U=4 mu=-2 d=4 D=6 U(1)xU(1) N=%d Sz=0" % (L, L)
+    elif args.model == "j1j2":
+        Ly = 4
+        Lx = L // Ly
+        states = list(rng.permutation([0] * (L // 2) + [1] * (L - L // 2)))
+        qphys = np.array([[1], [-1]])
+        bulk, _ = pkg.j1j2_bulk(Lx, Ly, J1=1.0, J2=0.5)
+        mpo = pkg.MPO(pkg.makeMPO(bulk, 2, L).H, qphys=qphys)
+        psi = pkg.product_state(states, 2, qphys=qphys, ctx=ctx)
+        desc = "J1-J2 Heisenberg cylinder %dx%d J2=0.5 d=2 U(1) Sz=0" % (Lx, Ly)
+    else:
+        states = list(rng.permutation([0] * (L // 2) + [1] * (L - L // 2)))
+        qphys = np.array([[1], [-1]])
+        mpo = pkg.MPO(pkg.makeMPO(pkg.heisenbergMPO(), 2, L).H, qphys=qphys)
+        psi = pkg.product_state(states, 2, qphys=qphys, ctx=ctx)
+        desc = "Heisenberg L=%d d=2 D=5 U(1) Sz=0" % L
     params = pkg.algvars(psi)
     t_ramp = time.perf_counter()
     for mr in [int(x) for x in args.ramp.split(",") if x]:
@@ -161,8 +190,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "f64" else args.precision,
         "data": "synthetic (seeded random product state, Heisenberg MPO)",
-        "config": {"workload": "Heisenberg L=%d d=2 D=5 U(1) Sz=0, two-site DMRG sweep at m=%d "
-                               "(ramp %s), Krylov 2, cutoff %g" % (L, m, args.ramp, args.cutoff),
+        "config": {"workload": "%s, two-site DMRG sweep at m=%d (ramp %s), Krylov 2, cutoff %g"
+                               % (desc, m, args.ramp, args.cutoff), "model": args.model,
                    "L": L, "m": m, "energy": params.energy,
                    "heff_flops_per_sweep": fl_heff / args.steps, "env_flops_per_sweep": fl_env / args.steps,
                    "sweep_s": sweep_s},

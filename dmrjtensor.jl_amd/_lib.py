@@ -13,7 +13,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libdmrjhip.so")
 
-PREC = {"f64": 0, "bf16x3": 1, "f64c": 16, "bf16x3c": 17}
+PREC = {"f64": 0, "bf16x3": 1, "f16x3": 2, "f64c": 16, "bf16x3c": 17, "f16x3c": 18}
 CORRECT = 16
 
 _p = C.c_void_p
@@ -161,7 +161,7 @@ class Context:
     def profile(self, enable=True):
         self.call("dmrj_ctx_profile", 1 if enable else 0)
 
-    KINDS = {"gemm_f64": 0, "mix": 1, "copy": 2, "svd": 3, "gemm_lp": 4, "graph": 5}
+    KINDS = {"gemm_f64": 0, "mix": 1, "copy": 2, "svd": 3, "gemm_lp": 4, "graph": 5, "gemm_f16": 6}
 
     def kernel_stats(self, kind, reset=False):
         out = (C.c_double * 4)()

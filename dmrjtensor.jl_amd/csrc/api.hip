@@ -537,16 +537,16 @@ void rebase_plan(HeffPlan& p, const double* oldx, const double* newx, int64_t nx
 struct HeffLaunch {
   GemmLaunch a, c;
   MixLaunch m;
-  bool lp = false;
+  int arith = 0;
   double fa = 0, fc = 0, ba = 0, bc = 0, flops_alg = 0;
 };
 
 HeffLaunch prepare_heff(Context& c, const HeffPlan& p, int prec) {
   HeffLaunch h;
-  h.lp = prec == DMRJ_PREC_BF16X3;
-  h.a = prepare_gemm(c, p.tA, p.sA, p.tA.empty() ? 0 : pick_gemm_cfg(c, p.tA, p.sA, h.lp), h.lp);
+  h.arith = prec;  // DMRJ_PREC_F64 / _BF16X3 / _F16X3 = GEMM arithmetic 0 / 1 / 2
+  h.a = prepare_gemm(c, p.tA, p.sA, p.tA.empty() ? 0 : pick_gemm_cfg(c, p.tA, p.sA, h.arith), h.arith);
   h.m = prepare_mix(c, p.md, p.mt);
-  h.c = prepare_gemm(c, p.tC, p.sC, p.tC.empty() ? 0 : pick_gemm_cfg(c, p.tC, p.sC, h.lp), h.lp);
+  h.c = prepare_gemm(c, p.tC, p.sC, p.tC.empty() ? 0 : pick_gemm_cfg(c, p.tC, p.sC, h.arith), h.arith);
   h.fa = gemm_flops(p.tA, p.sA);
   h.fc = gemm_flops(p.tC, p.sC);
   h.ba = gemm_bytes(p.tA, p.sA);
@@ -558,14 +558,11 @@ HeffLaunch prepare_heff(Context& c, const HeffPlan& p, int prec) {
 // Issues the three kernels; with per-kernel timing unless the stream is
 // being captured into a graph.
 void issue_heff(Context& c, const HeffLaunch& h, bool timed) {
-  const int kind = h.lp ? KIND_GEMM_LP : KIND_GEMM_F64;
+  const int kind = h.arith == 1 ? KIND_GEMM_LP : (h.arith == 2 ? KIND_GEMM_F16 : KIND_GEMM_F64);
   auto gemm = [&](const GemmLaunch& g, double f, double b) {
     if (g.ntiles <= 0) return;
     const int ps = timed ? c.prof_begin() : -1;
-    if (h.lp)
-      issue_gemm_lp(c, g);
-    else
-      issue_gemm_f64(c, g);
+    issue_gemm(c, g);
     if (ps >= 0) c.prof_end(ps, kind, f, b);
   };
   gemm(h.a, h.fa, h.ba);
@@ -632,8 +629,9 @@ void run_captured(Context& c, F&& body, double flops) {
 
 int base_prec(int precision) {
   const int b = precision & 15;
-  if (b != DMRJ_PREC_F64 && b != DMRJ_PREC_BF16X3)
-    throw Error(DMRJ_ERR_ARG, "precision must be DMRJ_PREC_F64 or DMRJ_PREC_BF16X3 (optionally | DMRJ_PREC_CORRECT)");
+  if (b != DMRJ_PREC_F64 && b != DMRJ_PREC_BF16X3 && b != DMRJ_PREC_F16X3)
+    throw Error(DMRJ_ERR_ARG,
+                "precision must be DMRJ_PREC_F64, _BF16X3 or _F16X3 (optionally | DMRJ_PREC_CORRECT)");
   return b;
 }
 
@@ -871,15 +869,9 @@ EnvPlan plan_right(Context& c, const EnvMeta& R, const double* Rd, const SiteMet
 }
 
 void run_env(Context& c, EnvPlan& p) {
-  if (c.env_prec == DMRJ_PREC_BF16X3) {
-    launch_gemm_lp(c, p.t1, p.s1, -1);
-    launch_mix(c, p.md, p.mt);
-    launch_gemm_lp(c, p.t3, p.s3, -1);
-  } else {
-    launch_gemm(c, p.t1, p.s1, -1);
-    launch_mix(c, p.md, p.mt);
-    launch_gemm(c, p.t3, p.s3, -1);
-  }
+  launch_gemm_arith(c, p.t1, p.s1, -1, c.env_prec);
+  launch_mix(c, p.md, p.mt);
+  launch_gemm_arith(c, p.t3, p.s3, -1, c.env_prec);
   c.flops_env += p.flops;
   c.alloc.release(p.X);
   c.alloc.release(p.Y);
@@ -961,6 +953,11 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
     int64_t R = 0, C = 0;
     double *M = nullptr, *U = nullptr, *Vt = nullptr, *s = nullptr, *E = nullptr;
     int kmin = 0, keep = 0;
+    // block-Jacobi sectors: working matrix, its column norms, sort order
+    double *X = nullptr, *Sn = nullptr;
+    int bj = 0, rowsX = 0, ncolsX = 0;
+    std::vector<int> perm;
+    std::vector<double> hn;
   };
   std::map<Charge, int> qidx;
   std::vector<Sec> secs;
@@ -1057,6 +1054,75 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
     svd_flops += 4.0 * mm * nn * nn + 8.0 * nn * nn * nn;
   }
   launch_svd_small(c, small, small_lds);
+  // large sectors, backend 4: batched block one-sided Jacobi on M (dir > 0,
+  // orthonormal U wanted) or on M^T (dir < 0, orthonormal V wanted); the
+  // other factor is one grouped GEMM with M after the truncation
+  int* bjflag = nullptr;
+  if (!z && c.svd_backend == 4) {
+    std::vector<BjJob> jobs;
+    for (int si : order) {
+      Sec& s = secs[si];
+      if (s.kmin == 0 || is_small[si]) continue;
+      const int rows = (int)(dir > 0 ? s.R : s.C), ncols = (int)(dir > 0 ? s.C : s.R);
+      int bw = 16;
+      while (bw > 4 && bj_lds_bytes(rows, bw) > c.lds_max) bw >>= 1;
+      if (bj_lds_bytes(rows, bw) > c.lds_max) continue;  // too tall for LDS: augmented solver below
+      s.X = (double*)c.alloc.alloc((int64_t)rows * ncols * sizeof(double));
+      s.Sn = (double*)c.alloc.alloc((int64_t)ncols * sizeof(double));
+      if (dir > 0)
+        HIP_OK(hipMemcpyAsync(s.X, s.M, (int64_t)rows * ncols * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+      else
+        launch_transpose(c, s.M, (int)s.R, (int)s.C, s.X);
+      int nblk = (ncols + bw - 1) / bw;
+      nblk += nblk & 1;
+      s.bj = 1;
+      s.rowsX = rows;
+      s.ncolsX = ncols;
+      is_small[si] = 1;  // handled
+      jobs.push_back({s.X, s.Sn, nullptr, rows, ncols, nblk, bw});
+      const double mm = (double)std::max(s.R, s.C), nn = (double)std::min(s.R, s.C);
+      svd_flops += 4.0 * mm * nn * nn + 8.0 * nn * nn * nn;
+    }
+    if (!jobs.empty()) {
+      const int nj = (int)jobs.size();
+      bjflag = (int*)c.alloc.alloc(nj * sizeof(int));
+      size_t lds = 0;
+      for (int j = 0; j < nj; ++j) {
+        jobs[j].flag = bjflag + j;
+        lds = std::max(lds, bj_lds_bytes(jobs[j].rows, jobs[j].bw));
+      }
+      auto* djobs = (BjJob*)c.upload(jobs.data(), nj * sizeof(BjJob));
+      std::vector<char> active(nj, 1);
+      std::vector<int> hflag(nj, 0);
+      constexpr int kBjMaxSweeps = 40;
+      int sweep = 0;
+      for (; sweep < kBjMaxSweeps; ++sweep) {
+        std::vector<int2> work;
+        int nsteps = 0;
+        for (int j = 0; j < nj; ++j) {
+          if (!active[j]) continue;
+          for (int k = 0; k < jobs[j].nblk / 2; ++k) work.push_back(make_int2(j, k));
+          nsteps = std::max(nsteps, jobs[j].nblk - 1);
+        }
+        if (work.empty()) break;
+        HIP_OK(hipMemsetAsync(bjflag, 0, nj * sizeof(int), c.stream));
+        launch_bj_sweep(c, djobs, work, nsteps, 8, lds);
+        HIP_OK(hipMemcpyAsync(hflag.data(), bjflag, nj * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        HIP_OK(hipStreamSynchronize(c.stream));
+        bool any = false;
+        for (int j = 0; j < nj; ++j) {
+          if (active[j] && hflag[j] == 0) active[j] = 0;  // a sweep without rotations: converged
+          any = any || active[j];
+        }
+        if (!any) break;
+      }
+      if (sweep == kBjMaxSweeps) throw Error(DMRJ_ERR_SOLVER, "block Jacobi SVD did not converge");
+      std::vector<int2> nw;
+      for (int j = 0; j < nj; ++j)
+        for (int col = 0; col < jobs[j].ncols; ++col) nw.push_back(make_int2(j, col));
+      launch_bj_norms(c, djobs, nw);
+    }
+  }
   // large sectors, default backend: one batched symmetric eigensolve of the
   // augmented matrices [[0, M], [M^T, 0]] (eigenvalues +-sigma, absolute
   // accuracy eps |M| like a bidiagonal SVD, no squaring of M)
@@ -1162,7 +1228,12 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
     for (int si : order) {
       Sec& s = secs[si];
       if (s.kmin == 0) continue;
-      HIP_OK(hipMemcpyAsync(hs.data() + o, s.s, s.kmin * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+      if (s.bj) {
+        s.hn.resize(s.ncolsX);
+        HIP_OK(hipMemcpyAsync(s.hn.data(), s.Sn, s.ncolsX * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+      } else {
+        HIP_OK(hipMemcpyAsync(hs.data() + o, s.s, s.kmin * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+      }
       o += s.kmin;
     }
   }
@@ -1182,6 +1253,22 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
   c.stage_reset();
   for (int i = 0; i < nsvd; ++i)
     if (hinfo[i] != 0) throw Error(DMRJ_ERR_SOLVER, "rocsolver SVD did not converge");
+  {
+    // block-Jacobi sectors: column norms sorted descending (stable), the
+    // largest kmin are the singular values
+    int64_t o = 0;
+    for (int si : order) {
+      Sec& s = secs[si];
+      if (s.kmin == 0) continue;
+      if (s.bj) {
+        s.perm.resize(s.ncolsX);
+        std::iota(s.perm.begin(), s.perm.end(), 0);
+        std::stable_sort(s.perm.begin(), s.perm.end(), [&](int a, int b) { return s.hn[a] > s.hn[b]; });
+        for (int l = 0; l < s.kmin; ++l) hs[o + l] = s.hn[s.perm[l]];
+      }
+      o += s.kmin;
+    }
+  }
   // global sort: sigma descending, ties by sector order then position
   struct Ent {
     double v;
@@ -1244,6 +1331,36 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
     }
   }
   mid.finish();
+  {
+    // block-Jacobi sectors: normalised kept columns, then the other factor
+    // U^T M = D V^T (dir > 0) or M V = U D (dir < 0) in one grouped GEMM
+    std::vector<BjGather> gat;
+    std::vector<GemmTask> tasks;
+    std::vector<GemmSeg> segs;
+    int64_t hoff = 0;
+    for (int si : order) {
+      Sec& s = secs[si];
+      if (s.bj && s.keep > 0) {
+        for (int k = 0; k < s.keep; ++k) {
+          const double* src = s.X + (int64_t)s.perm[k] * s.rowsX;
+          if (dir > 0)
+            gat.push_back({src, s.U + (int64_t)k * s.R, 1, hs[hoff + k], s.rowsX, 0});
+          else
+            gat.push_back({src, s.Vt + k, s.kmin, hs[hoff + k], s.rowsX, 1});
+        }
+        if (dir > 0) {
+          tasks.push_back({s.Vt, s.kmin, s.keep, (int)s.C, (int)segs.size(), 1, 0.0});
+          segs.push_back({s.U, s.M, s.R, s.R, (int)s.R, 1, 1.0});
+        } else {
+          tasks.push_back({s.U, s.R, (int)s.R, s.keep, (int)segs.size(), 1, 0.0});
+          segs.push_back({s.M, s.Vt, s.R, s.kmin, (int)s.C, 2, 1.0});
+        }
+      }
+      hoff += s.kmin;
+    }
+    launch_bj_gather(c, gat);
+    launch_gemm(c, tasks, segs, -1);
+  }
   double* dscal = (double*)c.alloc.alloc(std::max<size_t>(1, scal.size()) * sizeof(double));
   HIP_OK(hipMemcpyAsync(dscal, scal.data(), std::max<size_t>(1, scal.size()) * sizeof(double),
                         hipMemcpyHostToDevice, c.stream));
@@ -1270,8 +1387,12 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
     int64_t ro = -1;
     for (size_t r = 0; r < s.rows.size(); ++r)
       if (s.rows[r].first == b.a && s.rows[r].second == b.s) ro = s.roff[r];
-    split.push_back({s.U + ro, tl->data + b.off, s.R, S.dim[b.a], S.dim[b.a], s.keep, nullptr,
-                     dir < 0 ? dscal + soff[qidx[Q]] : nullptr, 1.0});
+    if (s.bj)  // U D from the GEMM (dir < 0) or orthonormal U (dir > 0)
+      split.push_back({s.U + ro, tl->data + b.off, s.R, S.dim[b.a], S.dim[b.a], s.keep, nullptr, nullptr,
+                       dir < 0 ? renorm : 1.0});
+    else
+      split.push_back({s.U + ro, tl->data + b.off, s.R, S.dim[b.a], S.dim[b.a], s.keep, nullptr,
+                       dir < 0 ? dscal + soff[qidx[Q]] : nullptr, 1.0});
     if (z)
       split.push_back({s.U + s.R * s.kmin + ro, tl->data + tl->site.size + b.off, s.R, S.dim[b.a], S.dim[b.a],
                        s.keep, nullptr, dir < 0 ? dscal + soff[qidx[Q]] : nullptr, 1.0});
@@ -1282,8 +1403,12 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
     int64_t co = -1;
     for (size_t q = 0; q < s.cols.size(); ++q)
       if (s.cols[q].first == b.s && s.cols[q].second == b.b) co = s.coff[q];
-    split.push_back({s.Vt + co * s.kmin, tr->data + b.off, s.kmin, s.keep, s.keep, T.dim[b.b],
-                     dir > 0 ? dscal + soff[qidx[Q]] : nullptr, nullptr, 1.0});
+    if (s.bj)  // D V^T from the GEMM (dir > 0) or orthonormal V^T (dir < 0)
+      split.push_back({s.Vt + co * s.kmin, tr->data + b.off, s.kmin, s.keep, s.keep, T.dim[b.b], nullptr, nullptr,
+                       dir > 0 ? renorm : 1.0});
+    else
+      split.push_back({s.Vt + co * s.kmin, tr->data + b.off, s.kmin, s.keep, s.keep, T.dim[b.b],
+                       dir > 0 ? dscal + soff[qidx[Q]] : nullptr, nullptr, 1.0});
     if (z)
       split.push_back({s.Vt + (int64_t)s.kmin * s.C + co * s.kmin, tr->data + tr->site.size + b.off, s.kmin, s.keep,
                        s.keep, T.dim[b.b], dir > 0 ? dscal + soff[qidx[Q]] : nullptr, nullptr, 1.0});
@@ -1295,7 +1420,10 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
     c.alloc.release(s.Vt);
     c.alloc.release(s.s);
     c.alloc.release(s.E);
+    c.alloc.release(s.X);
+    c.alloc.release(s.Sn);
   }
+  c.alloc.release(bjflag);
   c.alloc.release(info);
   c.alloc.release(dscal);
   res.L = tl;
@@ -1625,6 +1753,7 @@ int dmrj_ctx_create(int device, dmrj_ctx** out) {
     hipDeviceProp_t prop;
     HIP_OK(hipGetDeviceProperties(&prop, device));
     c->num_cu = prop.multiProcessorCount;
+    c->lds_max = prop.sharedMemPerBlock;
     rocblas_handle h;
     if (rocblas_create_handle(&h) != rocblas_status_success) throw Error(DMRJ_ERR_SOLVER, "rocblas handle");
     rocblas_set_stream(h, c->stream);
@@ -1704,7 +1833,7 @@ int dmrj_selftest_mfma_bf16(dmrj_ctx* c, float* out768) {
 
 int dmrj_ctx_set_svd_backend(dmrj_ctx* c, int mode) {
   return guard(c, [&] {
-    if (mode < 0 || mode > 3) throw Error(DMRJ_ERR_ARG, "svd backend must be 0..3");
+    if (mode < 0 || mode > 4) throw Error(DMRJ_ERR_ARG, "svd backend must be 0..4");
     c->svd_backend = mode;
   });
 }
@@ -2319,10 +2448,8 @@ int dmrj_selftest_gemm(dmrj_ctx* c, int cfg, int64_t M, int64_t N, int64_t K, in
     HIP_OK(hipMemcpyAsync(dB, B, K * N * sizeof(double), hipMemcpyHostToDevice, c->stream));
     std::vector<GemmTask> t{{dC, M, (int)M, (int)N, 0, 1, 0.0}};
     std::vector<GemmSeg> s{{dA, dB, tA ? K : M, tB ? N : K, (int)K, (tA ? 1 : 0) | (tB ? 2 : 0), 1.0}};
-    if (prec == DMRJ_PREC_BF16X3)
-      launch_gemm_lp(*c, t, s, cfg);
-    else
-      launch_gemm(*c, t, s, cfg);
+    if (prec > DMRJ_PREC_F16X3) throw Error(DMRJ_ERR_ARG, "selftest_gemm: bad precision");
+    launch_gemm_arith(*c, t, s, cfg, prec);
     HIP_OK(hipMemcpyAsync(Cout, dC, M * N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
     c->release_transients();

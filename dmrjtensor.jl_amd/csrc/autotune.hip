@@ -28,17 +28,19 @@ namespace dmrj {
 
 const TileShape kGemmShapes[kNumGemmShapes] = {{32, 32}, {64, 64}, {128, 128}, {64, 128}, {128, 64}};
 
-static int occupancy(Context& c, int cfg, bool lp) {
-  int& o = c.occ_cache[lp ? 1 : 0][cfg];
-  if (o == 0) o = lp ? gemm_occupancy_lp(cfg) : gemm_occupancy_f64(cfg);
+static int occupancy(Context& c, int cfg, int arith) {
+  int& o = c.occ_cache[arith][cfg];
+  if (o == 0) o = arith == 1 ? gemm_occupancy_lp(cfg) : (arith == 2 ? gemm_occupancy_f16(cfg) : gemm_occupancy_f64(cfg));
   return o;
 }
 
 GemmLaunch prepare_gemm(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs,
-                        int cfg, bool lp) {
+                        int cfg, int arith) {
   if (cfg < 0 || cfg >= kNumGemmShapes) throw Error(DMRJ_ERR_ARG, "GEMM tile config out of range");
+  if (arith < 0 || arith > 2) throw Error(DMRJ_ERR_ARG, "GEMM arithmetic out of range");
   GemmLaunch g;
   g.cfg = cfg;
+  g.arith = arith;
   const int bm = kGemmShapes[cfg].bm, bn = kGemmShapes[cfg].bn;
   // tiles ordered by descending K so that the persistent grid balances
   std::vector<std::pair<int64_t, int>> order;
@@ -60,13 +62,41 @@ GemmLaunch prepare_gemm(Context& c, const std::vector<GemmTask>& tasks, const st
   g.tasks = (const GemmTask*)c.upload(tasks.data(), tasks.size() * sizeof(GemmTask));
   g.segs = (const GemmSeg*)c.upload(segs.data(), std::max<size_t>(1, segs.size()) * sizeof(GemmSeg));
   g.ntiles = (int)tiles.size();
+  g.nseg = (int)segs.size();
   // persistent grid: never more work-groups than can be resident at once
-  const int64_t resident = (int64_t)c.num_cu * occupancy(c, cfg, lp);
+  const int64_t resident = (int64_t)c.num_cu * occupancy(c, cfg, arith);
   g.grid = (int)std::min<int64_t>((int64_t)tiles.size(), resident);
+  if (arith == 2) {
+    // per-segment operand maxima for the power-of-two scaling: work items
+    // (segment, operand, first column, last column) of the stored operand
+    std::vector<int4> mw;
+    std::vector<int> st(std::max<size_t>(1, segs.size()), 0);
+    for (size_t ti = 0; ti < tasks.size(); ++ti) {
+      const GemmTask& t = tasks[ti];
+      for (int q = 0; q < t.nseg; ++q) {
+        const int si = t.seg0 + q;
+        st[si] = (int)ti;
+        const GemmSeg& sg = segs[si];
+        if (sg.K <= 0 || t.M <= 0 || t.N <= 0) continue;
+        const bool tA = sg.flags & 1, tB = sg.flags & 2;
+        const int64_t ra = tA ? sg.K : t.M, ca = tA ? t.M : sg.K;
+        const int64_t rb = tB ? t.N : sg.K, cb = tB ? sg.K : t.N;
+        const int64_t stepa = std::max<int64_t>(1, 16384 / std::max<int64_t>(ra, 1));
+        const int64_t stepb = std::max<int64_t>(1, 16384 / std::max<int64_t>(rb, 1));
+        for (int64_t c0 = 0; c0 < ca; c0 += stepa) mw.push_back(make_int4(si, 0, (int)c0, (int)std::min(ca, c0 + stepa)));
+        for (int64_t c0 = 0; c0 < cb; c0 += stepb) mw.push_back(make_int4(si, 1, (int)c0, (int)std::min(cb, c0 + stepb)));
+      }
+    }
+    g.segtask = (const int*)c.upload(st.data(), st.size() * sizeof(int));
+    g.segmax = (double*)c.alloc.alloc(std::max<size_t>(2, 2 * segs.size()) * sizeof(double));
+    c.transient.push_back(g.segmax);  // lives until the next synchronisation point
+    if (!mw.empty()) g.maxwork = (const int4*)c.upload(mw.data(), mw.size() * sizeof(int4));
+    g.nmaxwork = (int)mw.size();
+  }
   return g;
 }
 
-static uint64_t histogram_key(const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs, bool lp) {
+static uint64_t histogram_key(const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs, int arith) {
   double F = 0.0, lm = 0.0, ln = 0.0, lk = 0.0;
   for (const auto& t : tasks) {
     int64_t K = 0;
@@ -86,7 +116,7 @@ static uint64_t histogram_key(const std::vector<GemmTask>& tasks, const std::vec
     key |= bin(std::log2(F), 1) << 24;
   }
   key |= bin(std::log2((double)tasks.size() + 1.0), 1) << 32;
-  key |= (uint64_t)(lp ? 1 : 0) << 40;
+  key |= (uint64_t)arith << 40;
   return key;
 }
 
@@ -127,8 +157,8 @@ static bool repeatable(const std::vector<GemmTask>& tasks, const std::vector<Gem
   return true;
 }
 
-int autotune_gemm(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs, bool lp) {
-  const uint64_t key = histogram_key(tasks, segs, lp);
+int autotune_gemm(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs, int arith) {
+  const uint64_t key = histogram_key(tasks, segs, arith);
   auto it = c.tune_cache.find(key);
   if (it != c.tune_cache.end()) {
     c.tune_hits += 1;
@@ -143,13 +173,8 @@ int autotune_gemm(Context& c, const std::vector<GemmTask>& tasks, const std::vec
   hipEvent_t ev[kNumGemmShapes + 1];
   for (auto& e : ev) HIP_OK(hipEventCreate(&e));
   GemmLaunch g[kNumGemmShapes];
-  for (int cfg = 0; cfg < kNumGemmShapes; ++cfg) g[cfg] = prepare_gemm(c, tasks, segs, cfg, lp);
-  auto issue = [&](const GemmLaunch& x) {
-    if (lp)
-      issue_gemm_lp(c, x);
-    else
-      issue_gemm_f64(c, x);
-  };
+  for (int cfg = 0; cfg < kNumGemmShapes; ++cfg) g[cfg] = prepare_gemm(c, tasks, segs, cfg, arith);
+  auto issue = [&](const GemmLaunch& x) { issue_gemm(c, x); };
   issue(g[0]);  // warm the operands into the caches once
   HIP_OK(hipEventRecord(ev[0], c.stream));
   for (int cfg = 0; cfg < kNumGemmShapes; ++cfg) {
@@ -174,9 +199,32 @@ int autotune_gemm(Context& c, const std::vector<GemmTask>& tasks, const std::vec
   return best;
 }
 
-int pick_gemm_cfg(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs, bool lp) {
+int pick_gemm_cfg(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs, int arith) {
   if (c.gemm_cfg_override >= 0) return c.gemm_cfg_override;
-  return c.autotune ? autotune_gemm(c, tasks, segs, lp) : choose_gemm_cfg(tasks, c.num_cu);
+  return c.autotune ? autotune_gemm(c, tasks, segs, arith) : choose_gemm_cfg(tasks, c.num_cu);
+}
+
+void issue_gemm(Context& c, const GemmLaunch& g) {
+  if (g.arith == 1)
+    issue_gemm_lp(c, g);
+  else if (g.arith == 2)
+    issue_gemm_f16(c, g);
+  else
+    issue_gemm_f64(c, g);
+}
+
+void launch_gemm_arith(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs, int cfg,
+                       int arith) {
+  if (tasks.empty()) return;
+  if (cfg < 0) cfg = pick_gemm_cfg(c, tasks, segs, arith);
+  if (arith == 0) c.last_gemm_cfg = cfg;
+  GemmLaunch g = prepare_gemm(c, tasks, segs, cfg, arith);
+  if (g.ntiles == 0) return;
+  const int ps = c.prof_begin();
+  issue_gemm(c, g);
+  if (ps >= 0)
+    c.prof_end(ps, arith == 1 ? KIND_GEMM_LP : (arith == 2 ? KIND_GEMM_F16 : KIND_GEMM_F64), gemm_flops(tasks, segs),
+               gemm_bytes(tasks, segs));
 }
 
 }  // namespace dmrj

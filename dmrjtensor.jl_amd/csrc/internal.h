@@ -190,6 +190,7 @@ struct Context {
   double* hscal = nullptr;      // pinned mirror
   double* partial = nullptr;    // reduction partials
   int num_cu = 256;
+  size_t lds_max = 65536;  // dynamic LDS per work-group (device property)
   // tuning: preferred GEMM tile config per operation (-1 = automatic)
   int gemm_cfg_override = -1;
   // SVD backend: 0 = batched Jacobi for small sectors + batched symmetric
@@ -207,7 +208,7 @@ struct Context {
   hipEvent_t graph_done = nullptr;
   double graph_updates = 0, graph_instantiations = 0;
   std::unordered_map<uint64_t, int> tune_cache;
-  int occ_cache[2][8] = {{0}};
+  int occ_cache[3][8] = {{0}};
   double tune_hits = 0, tune_misses = 0, tune_ms = 0, tune_skipped = 0;
   // statistics
   double flops_heff = 0.0, flops_env = 0.0;
@@ -281,6 +282,27 @@ struct AugJobZ {
   int R, C, N, kmin;
 };
 void launch_aug_place_z(Context& c, const std::vector<AugJobZ>& jobs);
+
+// ---- batched block one-sided Jacobi for large SVD sectors (svd_jacobi.hip) ----
+struct BjJob {
+  double* X;      // rows x ncols working matrix, ld rows, orthogonalised in place
+  double* S;      // ncols column norms (output of launch_bj_norms)
+  int* flag;      // set to 1 by a round that rotated
+  int rows, ncols, nblk, bw;  // nblk even (an empty block pads an odd count)
+};
+struct BjGather {
+  const double* src;  // column of X
+  double* dst;
+  int64_t ldd;        // trans: stride between consecutive elements of dst
+  double sigma;       // divisor
+  int rows, trans;
+};
+size_t bj_lds_bytes(int rows, int bw);
+void launch_bj_sweep(Context& c, const BjJob* djobs, const std::vector<int2>& work, int nsteps, int inner_max,
+                     size_t lds);
+void launch_bj_norms(Context& c, const BjJob* djobs, const std::vector<int2>& work);
+void launch_bj_gather(Context& c, const std::vector<BjGather>& g);
+void launch_transpose(Context& c, const double* M, int R, int C, double* X);
 void launch_aug_extract_z(Context& c, const std::vector<AugJobZ>& jobs);
 
 // ---- kernel launchers (kernels.hip) ----
@@ -311,26 +333,38 @@ struct TileShape {
 };
 extern const TileShape kGemmShapes[kNumGemmShapes];
 // Descriptor tables of one grouped-GEMM launch, already on the device.
+// arith: 0 = FP64 MFMA, 1 = BF16x3 split, 2 = FP16x3 split with per-segment
+// power-of-two scaling (the values of DMRJ_PREC_F64 / _BF16X3 / _F16X3).
 struct GemmLaunch {
   const GemmTile* tiles = nullptr;
   const GemmTask* tasks = nullptr;
   const GemmSeg* segs = nullptr;
-  int ntiles = 0, grid = 0, cfg = 0;
+  int ntiles = 0, grid = 0, cfg = 0, arith = 0, nseg = 0;
+  double* segmax = nullptr;          // FP16x3: 2 per segment, max |alpha A| and max |B|
+  const int4* maxwork = nullptr;     // FP16x3: work items of the max pre-pass
+  const int* segtask = nullptr;      // FP16x3: task of every segment
+  int nmaxwork = 0;
 };
 GemmLaunch prepare_gemm(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs,
-                        int cfg, bool lp);
+                        int cfg, int arith);
 void issue_gemm_f64(Context& c, const GemmLaunch& g);
 void issue_gemm_lp(Context& c, const GemmLaunch& g);
+void issue_gemm_f16(Context& c, const GemmLaunch& g);
+void issue_gemm(Context& c, const GemmLaunch& g);
 int gemm_occupancy_f64(int cfg);
 int gemm_occupancy_lp(int cfg);
+int gemm_occupancy_f16(int cfg);
 double gemm_flops(const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs);
 double gemm_bytes(const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs);
 // Tile shape for a task list: cached per block-size-histogram key; on a
 // miss every shape is timed on the live operands (only for launches that are
 // safe to repeat: beta = 0 and outputs disjoint from inputs).
-int autotune_gemm(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs, bool lp);
+int autotune_gemm(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs, int arith);
 // override > autotuner > heuristic
-int pick_gemm_cfg(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs, bool lp);
+int pick_gemm_cfg(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs, int arith);
+// one grouped GEMM in the given arithmetic (tile shape picked when cfg < 0)
+void launch_gemm_arith(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs, int cfg,
+                       int arith);
 // Descriptor tables of one mix launch, already on the device.
 struct MixLaunch {
   const MixWork* work = nullptr;
@@ -363,7 +397,8 @@ constexpr size_t kSmallSvdMaxLds = 60 * 1024;  // bytes of dynamic LDS per secto
 double gemm_flops(const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs);
 
 // kernel kinds for Context profiling
-enum KernelKind { KIND_GEMM_F64 = 0, KIND_MIX = 1, KIND_COPY = 2, KIND_SVD = 3, KIND_GEMM_LP = 4, KIND_GRAPH = 5 };
+enum KernelKind { KIND_GEMM_F64 = 0, KIND_MIX = 1, KIND_COPY = 2, KIND_SVD = 3, KIND_GEMM_LP = 4, KIND_GRAPH = 5,
+                  KIND_GEMM_F16 = 6 };
 
 // Lanczos scalar slots in Context::scal
 enum Slot {

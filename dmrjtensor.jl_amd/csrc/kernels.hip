@@ -441,14 +441,7 @@ void issue_gemm_f64(Context& c, const GemmLaunch& g) {
 
 void launch_gemm(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs,
                  int cfg) {
-  if (tasks.empty()) return;
-  if (cfg < 0) cfg = pick_gemm_cfg(c, tasks, segs, false);
-  c.last_gemm_cfg = cfg;
-  GemmLaunch g = prepare_gemm(c, tasks, segs, cfg, false);
-  if (g.ntiles == 0) return;
-  const int ps = c.prof_begin();
-  issue_gemm_f64(c, g);
-  if (ps >= 0) c.prof_end(ps, KIND_GEMM_F64, gemm_flops(tasks, segs), gemm_bytes(tasks, segs));
+  launch_gemm_arith(c, tasks, segs, cfg, 0);
 }
 
 MixLaunch prepare_mix(Context& c, const std::vector<MixDst>& dsts, const std::vector<MixTerm>& terms) {

@@ -8,6 +8,14 @@
 // bits per product.  A 32-deep K step of the tile is one MFMA per product.
 // LDS holds hi and lo of both operands, row-major in k with a 40-element
 // pitch (the 16 lanes of a 16-byte ds_read hit disjoint banks).
+//
+// The FP16 form (k_gemm_f16x3) does the same with FP16 halves on
+// v_mfma_f32_16x16x32_f16 (11-bit significands: about 22 correct bits per
+// product instead of 16) and per-block scaling: every K-segment's operands
+// are scaled by powers of two that bring max |alpha A| and max |B| of that
+// segment to [2^13, 2^14) before the split (k_seg_absmax measures them right
+// before the GEMM), the segment's FP32 partial sum is unscaled exactly by
+// 2^-(eA+eB) and added to the tile's FP32 total.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -157,6 +165,239 @@ __global__ __launch_bounds__(kLpThreads) void k_gemm_bf16x3(const GemmTile* __re
   }
 }
 
+// ------------------------------------------------------------------ FP16x3
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+
+__device__ inline void split_f16(double x, _Float16& hi, _Float16& lo) {
+  hi = (_Float16)(float)x;
+  lo = (_Float16)(float)(x - (double)hi);
+}
+
+// 2^e max in [2^13, 2^14): FP16 (max 65504) holds the scaled hi part
+__device__ inline int f16_scale_exp(double mx) {
+  if (!(mx > 0.0) || !isfinite(mx)) return 0;
+  int e;
+  (void)frexp(mx, &e);  // mx = f 2^e, f in [0.5, 1)
+  return 14 - e;
+}
+
+// work item (segment, operand 0 = A / 1 = B, first column, end column) of the
+// operand as stored (column-major with its leading dimension)
+__global__ __launch_bounds__(256) void k_seg_absmax(const GemmTask* __restrict__ tasks, const GemmSeg* __restrict__ segs,
+                                                    const int* __restrict__ seg_task, const int4* __restrict__ work,
+                                                    int nwork, double* __restrict__ segmax) {
+  __shared__ double red[4];
+  for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+    const int4 w = work[wi];
+    const GemmSeg sg = segs[w.x];
+    const GemmTask tk = tasks[seg_task[w.x]];
+    const bool tA = sg.flags & 1, tB = sg.flags & 2;
+    const double* base = w.y == 0 ? sg.A : sg.B;
+    const int64_t ld = w.y == 0 ? sg.lda : sg.ldb;
+    const int rows = w.y == 0 ? (tA ? sg.K : tk.M) : (tB ? tk.N : sg.K);
+    const int ncol = w.w - w.z;
+    double m = 0.0;
+    for (int64_t e = threadIdx.x; e < (int64_t)rows * ncol; e += 256) {
+      const int r = (int)(e % rows), cc = w.z + (int)(e / rows);
+      m = fmax(m, fabs(base[(int64_t)r + (int64_t)cc * ld]));
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double v = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+      if (w.y == 0) v *= fabs(sg.alpha);
+      // non-negative doubles order like their bit patterns
+      atomicMax(reinterpret_cast<unsigned long long*>(segmax + 2 * w.x + w.y), (unsigned long long)__double_as_longlong(v));
+    }
+    __syncthreads();
+  }
+}
+
+template <int BM, int BN>
+__global__ __launch_bounds__(kLpThreads) void k_gemm_f16x3(const GemmTile* __restrict__ tiles, int ntiles,
+                                                            const GemmTask* __restrict__ tasks,
+                                                            const GemmSeg* __restrict__ segs,
+                                                            const double* __restrict__ segmax) {
+  constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
+  constexpr int AE = BM * kLpBK / kLpThreads, BE = BN * kLpBK / kLpThreads;
+  __shared__ __attribute__((aligned(16))) _Float16 sAh[2][BM * kLpPitch];
+  __shared__ __attribute__((aligned(16))) _Float16 sAl[2][BM * kLpPitch];
+  __shared__ __attribute__((aligned(16))) _Float16 sBh[2][BN * kLpPitch];
+  __shared__ __attribute__((aligned(16))) _Float16 sBl[2][BN * kLpPitch];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int lr = lane & 15, lk = lane >> 4;
+  int buf = 0;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const GemmTile tl = tiles[t];
+    const GemmTask tk = tasks[tl.task];
+    const int m0 = tl.m0, n0 = tl.n0;
+    f32x4_t tot[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) tot[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    for (int sidx = 0; sidx < tk.nseg; ++sidx) {
+      const int si = tk.seg0 + sidx;
+      const GemmSeg sg = segs[si];
+      const bool tA = sg.flags & 1, tB = sg.flags & 2;
+      const int K = sg.K;
+      if (K <= 0) continue;
+      const int ea = f16_scale_exp(segmax[2 * si]), eb = f16_scale_exp(segmax[2 * si + 1]);
+      const double fa = ldexp(sg.alpha, ea), fb = ldexp(1.0, eb);
+      f32x4_t acc[FM][FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+      double ra[AE], rb[BE];
+      auto load = [&](int k0) {
+#pragma unroll
+        for (int r = 0; r < AE; ++r) {
+          const int idx = tid + r * kLpThreads;
+          int m, k;
+          if (!tA) { m = idx % BM; k = idx / BM; } else { k = idx % kLpBK; m = idx / kLpBK; }
+          const int gm = m0 + m, gk = k0 + k;
+          double v = 0.0;
+          if (gm < tk.M && gk < K)
+            v = tA ? sg.A[(int64_t)gk + (int64_t)gm * sg.lda] : sg.A[(int64_t)gm + (int64_t)gk * sg.lda];
+          ra[r] = v * fa;
+        }
+#pragma unroll
+        for (int r = 0; r < BE; ++r) {
+          const int idx = tid + r * kLpThreads;
+          int n, k;
+          if (!tB) { k = idx % kLpBK; n = idx / kLpBK; } else { n = idx % BN; k = idx / BN; }
+          const int gn = n0 + n, gk = k0 + k;
+          double v = 0.0;
+          if (gn < tk.N && gk < K)
+            v = tB ? sg.B[(int64_t)gn + (int64_t)gk * sg.ldb] : sg.B[(int64_t)gk + (int64_t)gn * sg.ldb];
+          rb[r] = v * fb;
+        }
+      };
+      auto store = [&](int b) {
+#pragma unroll
+        for (int r = 0; r < AE; ++r) {
+          const int idx = tid + r * kLpThreads;
+          int m, k;
+          if (!tA) { m = idx % BM; k = idx / BM; } else { k = idx % kLpBK; m = idx / kLpBK; }
+          _Float16 h, l;
+          split_f16(ra[r], h, l);
+          sAh[b][m * kLpPitch + k] = h;
+          sAl[b][m * kLpPitch + k] = l;
+        }
+#pragma unroll
+        for (int r = 0; r < BE; ++r) {
+          const int idx = tid + r * kLpThreads;
+          int n, k;
+          if (!tB) { k = idx % kLpBK; n = idx / kLpBK; } else { n = idx % BN; k = idx / BN; }
+          _Float16 h, l;
+          split_f16(rb[r], h, l);
+          sBh[b][n * kLpPitch + k] = h;
+          sBl[b][n * kLpPitch + k] = l;
+        }
+      };
+      load(0);
+      for (int k0 = 0; k0 < K; k0 += kLpBK) {
+        store(buf);
+        __syncthreads();
+        if (k0 + kLpBK < K) load(k0 + kLpBK);
+        half8_t ah[FM], al[FM], bh[FN], bl[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+          const int row = wm * WM + i * 16 + lr;
+          ah[i] = *reinterpret_cast<const half8_t*>(&sAh[buf][row * kLpPitch + lk * 8]);
+          al[i] = *reinterpret_cast<const half8_t*>(&sAl[buf][row * kLpPitch + lk * 8]);
+        }
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          const int col = wn * WN + j * 16 + lr;
+          bh[j] = *reinterpret_cast<const half8_t*>(&sBh[buf][col * kLpPitch + lk * 8]);
+          bl[j] = *reinterpret_cast<const half8_t*>(&sBl[buf][col * kLpPitch + lk * 8]);
+        }
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) {
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+          }
+        buf ^= 1;
+      }
+      const float unscale = ldexpf(1.0f, -(ea + eb));  // exact power of two
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) tot[i][j] += acc[i][j] * unscale;
+    }
+    // accumulator item r of lane l: row 4*(l/16) + r, column l%16
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int col = n0 + wn * WN + j * 16 + lr;
+        if (col >= tk.N) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm * WM + i * 16 + lk * 4 + r;
+          if (row < tk.M) {
+            double* p = tk.C + (int64_t)row + (int64_t)col * tk.ldc;
+            double v = (double)tot[i][j][r];
+            if (tk.beta != 0.0) v += tk.beta * *p;
+            *p = v;
+          }
+        }
+      }
+  }
+}
+
+template <int BM, int BN>
+static void gemm_f16_kernel_launch(Context& c, const GemmLaunch& g) {
+  hipLaunchKernelGGL((k_gemm_f16x3<BM, BN>), dim3(g.grid), dim3(kLpThreads), 0, c.stream, g.tiles, g.ntiles,
+                     g.tasks, g.segs, g.segmax);
+}
+
+static const void* gemm_f16_kernel_ptr(int cfg) {
+  switch (cfg) {
+    case 0: return (const void*)k_gemm_f16x3<32, 32>;
+    case 1: return (const void*)k_gemm_f16x3<64, 64>;
+    case 2: return (const void*)k_gemm_f16x3<128, 128>;
+    case 3: return (const void*)k_gemm_f16x3<64, 128>;
+    default: return (const void*)k_gemm_f16x3<128, 64>;
+  }
+}
+
+int gemm_occupancy_f16(int cfg) {
+  int nb = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, gemm_f16_kernel_ptr(cfg), kLpThreads, 0);
+  if (e != hipSuccess) throw Error(-3, std::string("HIP: ") + hipGetErrorString(e));
+  return std::max(nb, 1);
+}
+
+void issue_gemm_f16(Context& c, const GemmLaunch& g) {
+  if (g.ntiles <= 0) return;
+  // operand maxima of every segment, measured right before the product
+  launch_fill(c, g.segmax, std::max(2, 2 * g.nseg), 0.0);
+  if (g.nmaxwork > 0) {
+    hipLaunchKernelGGL(k_seg_absmax, dim3(std::min(g.nmaxwork, c.num_cu * 8)), dim3(256), 0, c.stream, g.tasks,
+                       g.segs, g.segtask, g.maxwork, g.nmaxwork, g.segmax);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw Error(-3, std::string("HIP: ") + hipGetErrorString(e));
+  }
+  switch (g.cfg) {
+    case 0: gemm_f16_kernel_launch<32, 32>(c, g); break;
+    case 1: gemm_f16_kernel_launch<64, 64>(c, g); break;
+    case 2: gemm_f16_kernel_launch<128, 128>(c, g); break;
+    case 3: gemm_f16_kernel_launch<64, 128>(c, g); break;
+    default: gemm_f16_kernel_launch<128, 64>(c, g); break;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) throw Error(-3, std::string("HIP: ") + hipGetErrorString(e));
+}
+
 // Lane-layout probe of v_mfma_f32_16x16x32_bf16 (one wave, in-bounds):
 //  mode 0: a = lane+1, b = 1; mode 1: a = 1, b = lane+1 (row / column sums)
 //  mode 2: A[i][0] = i+1, A[i][1] = 1, B[0][j] = 1, B[1][j] = 16(j+1) under
@@ -231,13 +472,7 @@ void issue_gemm_lp(Context& c, const GemmLaunch& g) {
 }
 
 void launch_gemm_lp(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs, int cfg) {
-  if (tasks.empty()) return;
-  if (cfg < 0) cfg = pick_gemm_cfg(c, tasks, segs, true);
-  GemmLaunch g = prepare_gemm(c, tasks, segs, cfg, true);
-  if (g.ntiles == 0) return;
-  const int ps = c.prof_begin();
-  issue_gemm_lp(c, g);
-  if (ps >= 0) c.prof_end(ps, KIND_GEMM_LP, gemm_flops(tasks, segs), gemm_bytes(tasks, segs));
+  launch_gemm_arith(c, tasks, segs, cfg, 1);
 }
 
 }  // namespace dmrj

@@ -1,0 +1,236 @@
+// Batched block one-sided Jacobi for the larger charge sectors of the
+// two-site SVD.
+//
+// The working matrix X (rows x ncols, column-major, ld rows) is M when the
+// left factor is wanted orthonormal (split (U, D V^T)) and M^T otherwise.
+// One-sided (Hestenes) Jacobi rotates column pairs of X until all columns are
+// mutually orthogonal: X W = U D with W orthogonal, never formed.  Columns are
+// grouped into blocks of bw; one kernel launch per round of a round-robin
+// (circle) schedule over the blocks pairs every block once, nblk/2 disjoint
+// block pairs per sector and all sectors in the same launch.  A work-group
+// loads its block pair (rows x 2bw) into LDS and orthogonalises those columns
+// among themselves with inner circle rounds (one group of lanes per column
+// pair, 3 dot products and a plane rotation), then writes them back.  Rounds
+// repeat as outer sweeps until a sweep rotates nothing beyond
+// |x_p . x_q| <= tol sqrt(|x_p|^2 |x_q|^2), tol = sqrt(rows) eps (the
+// threshold of LAPACK's dgesvj).  Column norms are then the singular values
+// and the normalised columns the singular vectors of that side; the other
+// factor comes from one grouped GEMM with M (U^T M = D V^T or M V = U D), so
+// W is never accumulated.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "dmrj.h"
+#include "internal.h"
+
+namespace dmrj {
+
+#define HIP_OK(x)                                                                    \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess) throw Error(-3, std::string("HIP: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int kBjThreads = 256;
+
+__device__ inline double bj_group_sum(double v, int G) {
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// circle schedule: round s of n players (n even), pair k
+__device__ inline void circle_pair(int n, int s, int k, int& p, int& q) {
+  if (k == 0) {
+    p = s;
+    q = n - 1;
+  } else {
+    p = (s + k) % (n - 1);
+    q = (s - k + (n - 1)) % (n - 1);
+  }
+  if (p > q) {
+    const int t = p;
+    p = q;
+    q = t;
+  }
+}
+
+__global__ __launch_bounds__(kBjThreads) void k_bj_step(const BjJob* __restrict__ jobs, const int2* __restrict__ work,
+                                                        int nwork, int step, int inner_max) {
+  extern __shared__ double panel[];  // [2bw][rows]
+  __shared__ int rot_any;
+  const int tid = threadIdx.x;
+  for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+    const int2 w = work[wi];
+    const BjJob jb = jobs[w.x];
+    if (step >= jb.nblk - 1) continue;
+    int bp, bq;
+    circle_pair(jb.nblk, step, w.y, bp, bq);
+    const int p0 = bp * jb.bw, q0 = bq * jb.bw;
+    const int np_ = max(0, min(jb.bw, jb.ncols - p0)), nq_ = max(0, min(jb.bw, jb.ncols - q0));
+    if (np_ == 0 || nq_ == 0) continue;  // paired with an empty block this round
+    const int nc = np_ + nq_;
+    const int rows = jb.rows;
+    // columns 0..np_-1 from block p, np_..nc-1 from block q
+    for (int e = tid; e < nc * rows; e += kBjThreads) {
+      const int c = e / rows, r = e % rows;
+      const int gc = c < np_ ? p0 + c : q0 + (c - np_);
+      panel[e] = jb.X[(int64_t)r + (int64_t)gc * rows];
+    }
+    __syncthreads();
+    const int ncp = (nc + 1) & ~1;  // even number of players
+    const int P = ncp / 2;
+    int G = 64;
+    while (G > 1 && G * P > kBjThreads) G >>= 1;
+    const int grp = tid / G, gl = tid % G;
+    const double tol = 2.220446049250313e-16 * sqrt((double)rows);
+    bool any_step = false;
+    for (int sweep = 0; sweep < inner_max; ++sweep) {
+      if (tid == 0) rot_any = 0;
+      __syncthreads();
+      for (int rd = 0; rd < ncp - 1; ++rd) {
+        if (grp < P) {
+          int p, q;
+          circle_pair(ncp, rd, grp, p, q);
+          if (q < nc) {
+            double* xp = panel + (size_t)p * rows;
+            double* xq = panel + (size_t)q * rows;
+            double a = 0.0, b = 0.0, g = 0.0;
+            for (int r = gl; r < rows; r += G) {
+              const double u = xp[r], v = xq[r];
+              a += u * u;
+              b += v * v;
+              g += u * v;
+            }
+            a = bj_group_sum(a, G);
+            b = bj_group_sum(b, G);
+            g = bj_group_sum(g, G);
+            if (g != 0.0 && fabs(g) > tol * sqrt(a * b)) {
+              const double zeta = (b - a) / (2.0 * g);
+              const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+              const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+              for (int r = gl; r < rows; r += G) {
+                const double u = xp[r], v = xq[r];
+                xp[r] = c * u - s * v;
+                xq[r] = s * u + c * v;
+              }
+              if (gl == 0) rot_any = 1;
+            }
+          }
+        }
+        __syncthreads();
+      }
+      const int any = rot_any;
+      __syncthreads();
+      if (!any) break;
+      any_step = true;
+    }
+    for (int e = tid; e < nc * rows; e += kBjThreads) {
+      const int c = e / rows, r = e % rows;
+      const int gc = c < np_ ? p0 + c : q0 + (c - np_);
+      jb.X[(int64_t)r + (int64_t)gc * rows] = panel[e];
+    }
+    if (any_step && tid == 0) atomicOr(jb.flag, 1);
+    __syncthreads();
+  }
+}
+
+// column norms of X into S (unsorted)
+__global__ __launch_bounds__(kBjThreads) void k_bj_norms(const BjJob* __restrict__ jobs, const int2* __restrict__ work,
+                                                         int nwork) {
+  __shared__ double red[4];
+  for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+    const int2 w = work[wi];
+    const BjJob jb = jobs[w.x];
+    const double* x = jb.X + (int64_t)w.y * jb.rows;
+    double a = 0.0;
+    for (int r = threadIdx.x; r < jb.rows; r += kBjThreads) a += x[r] * x[r];
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) jb.S[w.y] = sqrt(red[0] + red[1] + red[2] + red[3]);
+    __syncthreads();
+  }
+}
+
+// normalised column perm[k] of X -> column k of D (trans = 0: D[r + k ldd])
+// or row k of D (trans = 1: D[k + r ldd])
+__global__ __launch_bounds__(kBjThreads) void k_bj_gather(const BjGather* __restrict__ g, int ng) {
+  for (int wi = blockIdx.x; wi < ng; wi += gridDim.x) {
+    const BjGather o = g[wi];
+    const double inv = o.sigma > 0.0 ? 1.0 / o.sigma : 0.0;
+    for (int r = threadIdx.x; r < o.rows; r += kBjThreads) {
+      const double v = o.src[r] * inv;
+      if (o.trans)
+        o.dst[(int64_t)r * o.ldd] = v;
+      else
+        o.dst[r] = v;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBjThreads) void k_transpose(const double* __restrict__ M, int R, int C,
+                                                           double* __restrict__ X) {
+  __shared__ double tile[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  const int tilesR = (R + 31) / 32, tilesC = (C + 31) / 32;
+  for (int t = blockIdx.x; t < tilesR * tilesC; t += gridDim.x) {
+    const int r0 = (t % tilesR) * 32, c0 = (t / tilesR) * 32;
+    for (int j = ty; j < 32; j += 8) {
+      const int r = r0 + tx, cc = c0 + j;
+      tile[j][tx] = (r < R && cc < C) ? M[(int64_t)r + (int64_t)cc * R] : 0.0;
+    }
+    __syncthreads();
+    for (int j = ty; j < 32; j += 8) {
+      const int cc = c0 + tx, r = r0 + j;  // X[cc + r*C] = M[r + cc*R]
+      if (cc < C && r < R) X[(int64_t)cc + (int64_t)r * C] = tile[tx][j];
+    }
+    __syncthreads();
+  }
+}
+
+void launch_transpose(Context& c, const double* M, int R, int C, double* X) {
+  if (R <= 0 || C <= 0) return;
+  const int tiles = ((R + 31) / 32) * ((C + 31) / 32);
+  hipLaunchKernelGGL(k_transpose, dim3(std::min(tiles, c.num_cu * 8)), dim3(kBjThreads), 0, c.stream, M, R, C, X);
+  HIP_OK(hipGetLastError());
+}
+
+size_t bj_lds_bytes(int rows, int bw) { return (size_t)2 * bw * rows * sizeof(double); }
+
+void launch_bj_sweep(Context& c, const BjJob* djobs, const std::vector<int2>& work, int nsteps, int inner_max,
+                     size_t lds) {
+  if (work.empty() || nsteps <= 0) return;
+  static bool attr_set = false;
+  if (!attr_set) {  // allow dynamic LDS up to the device limit
+    (void)hipFuncSetAttribute((const void*)k_bj_step, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.lds_max);
+    (void)hipGetLastError();
+    attr_set = true;
+  }
+  auto* dw = (int2*)c.upload(work.data(), work.size() * sizeof(int2));
+  const int grid = (int)std::min<size_t>(work.size(), (size_t)c.num_cu * 2);
+  for (int s = 0; s < nsteps; ++s) {
+    hipLaunchKernelGGL(k_bj_step, dim3(grid), dim3(kBjThreads), lds, c.stream, djobs, dw, (int)work.size(), s,
+                       inner_max);
+    HIP_OK(hipGetLastError());
+  }
+}
+
+void launch_bj_norms(Context& c, const BjJob* djobs, const std::vector<int2>& work) {
+  if (work.empty()) return;
+  auto* dw = (int2*)c.upload(work.data(), work.size() * sizeof(int2));
+  const int grid = (int)std::min<size_t>(work.size(), (size_t)c.num_cu * 8);
+  hipLaunchKernelGGL(k_bj_norms, dim3(grid), dim3(kBjThreads), 0, c.stream, djobs, dw, (int)work.size());
+  HIP_OK(hipGetLastError());
+}
+
+void launch_bj_gather(Context& c, const std::vector<BjGather>& g) {
+  if (g.empty()) return;
+  auto* dg = (BjGather*)c.upload(g.data(), g.size() * sizeof(BjGather));
+  const int grid = (int)std::min<size_t>(g.size(), (size_t)c.num_cu * 8);
+  hipLaunchKernelGGL(k_bj_gather, dim3(grid), dim3(kBjThreads), 0, c.stream, dg, (int)g.size());
+  HIP_OK(hipGetLastError());
+}
+
+}  // namespace dmrj

@@ -202,8 +202,8 @@ def dmrg(psi, mpo, m=0, minm=2, sweeps=1, cutoff=0.0, silent=True, goal=0.0, met
         params = algvars(psi)
     params.maxm, params.minm, params.sweeps, params.cutoff = m, minm, sweeps, cutoff
     params.silent, params.goal, params.precision = silent, goal, precision
-    if precision not in ("f64", "bf16x3"):
-        raise ValueError("precision must be 'f64' or 'bf16x3'")
+    if precision not in ("f64", "bf16x3", "f16x3"):
+        raise ValueError("precision must be 'f64', 'bf16x3' or 'f16x3'")
     params.correction = correction if correction is not None else ("none" if precision == "f64" else "sweep")
     if SvNbond is not None:
         params.SvNbond = SvNbond

@@ -14,11 +14,11 @@
 #  shared library (dmrjtensor.jl_amd/lib/libdmrjhip.so by default).
 module DMRJhip
 
-export gpuMPS, gpuMPO, gpuEnv, dmrg, makeEnv, Lupdate, Rupdate, expect, twostep, free!
+export gpuMPS, gpuMPO, gpuEnv, dmrg, makeEnv, Lupdate, Rupdate, expect, twostep, tdvp, free!
 
 const libdmrj = get(ENV, "DMRJHIP_LIB", joinpath(@__DIR__, "..", "lib", "libdmrjhip.so"))
 
-const PREC = Dict("f64" => Cint(0), "bf16x3" => Cint(1))
+const PREC = Dict("f64" => Cint(0), "bf16x3" => Cint(1), "f16x3" => Cint(2))
 const CORRECT = Cint(16)
 
 mutable struct Context
@@ -79,6 +79,26 @@ function upload(A::Array{Float64,3}; qL=nothing, qP=nothing, qR=nothing)
   return gpuTens(out[])
 end
 
+"ComplexF64 upload: Julia's complex arrays are interleaved (re, im), the layout dmrj_site_from_dense_z takes"
+function upload(A::Array{ComplexF64,3}; qL=nothing, qP=nothing, qR=nothing)
+  ctx = context()
+  nq = qL === nothing ? 0 : size(qL, 1)
+  out = Ref{Ptr{Cvoid}}(C_NULL)
+  pl = nq == 0 ? C_NULL : pointer(charges(qL))
+  pp = nq == 0 ? C_NULL : pointer(charges(qP))
+  pr = nq == 0 ? C_NULL : pointer(charges(qR))
+  GC.@preserve qL qP qR A begin
+    rc = ccall((:dmrj_site_from_dense_z, libdmrj), Cint,
+               (Ptr{Cvoid}, Cint, Int64, Int64, Int64, Ptr{ComplexF64}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32},
+                Ref{Ptr{Cvoid}}),
+               ctx.h, nq, size(A, 1), size(A, 2), size(A, 3), A, pl, pp, pr, out)
+  end
+  check(ctx, rc, "dmrj_site_from_dense_z")
+  return gpuTens(out[])
+end
+
+iscomplex(t::gpuTens) = ccall((:dmrj_tensor_is_complex, libdmrj), Cint, (Ptr{Cvoid},), t.h) == 1
+
 function info(t::gpuTens)
   d = zeros(Int64, 9)
   ccall((:dmrj_tensor_info, libdmrj), Cint, (Ptr{Cvoid}, Ptr{Int64}), t.h, d)
@@ -89,6 +109,13 @@ function download(t::gpuTens)
   ctx = context()
   d = info(t)
   dims = d[1] == 1 ? (d[2], d[3], d[4], d[5]) : (d[2], d[3], d[4])
+  if iscomplex(t)
+    z = zeros(ComplexF64, dims...)
+    rc = ccall((:dmrj_tensor_to_dense_z, libdmrj), Cint,
+               (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{ComplexF64}, Ptr{Int32}, Ptr{Int32}), ctx.h, t.h, z, C_NULL, C_NULL)
+    check(ctx, rc, "dmrj_tensor_to_dense_z")
+    return z
+  end
   out = zeros(Float64, dims...)
   rc = ccall((:dmrj_tensor_to_dense, libdmrj), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Ptr{Int32}, Ptr{Int32}),
              ctx.h, t.h, out, C_NULL, C_NULL)
@@ -280,6 +307,83 @@ function dmrg(psi::gpuMPS, mpo::gpuMPO; m::Integer=0, minm::Integer=2, sweeps::I
   end
   ccall((:dmrj_ctx_set_env_precision, libdmrj), Cint, (Ptr{Cvoid}, Cint), context().h, 0)
   return params.energy
+end
+
+"krylovTimeEvol (algs/time.jl:182-202) of a two-site (two=true) or one-site tensor; complex prefactor"
+function krylovTimeEvol(Lenv::gpuTens, Renv::gpuTens, A::gpuTens, mpo::gpuMPO, i::Integer, r::Integer,
+                        prefactor::Number; two::Bool=true)
+  ctx = context()
+  out = Ref{Ptr{Cvoid}}(C_NULL); e = Ref{Float64}(0.0); k = Ref{Cint}(0)
+  pf = ComplexF64(prefactor)
+  if two
+    rc = ccall((:dmrj_krylov_expm2, libdmrj), Cint,
+               (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Float64, Float64, Cint,
+                Ref{Ptr{Cvoid}}, Ref{Float64}, Ref{Cint}),
+               ctx.h, Lenv.h, Renv.h, A.h, mpo[i], mpo[i+1], r, real(pf), imag(pf), 0, out, e, k)
+  else
+    rc = ccall((:dmrj_krylov_expm1, libdmrj), Cint,
+               (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Float64, Float64, Cint,
+                Ref{Ptr{Cvoid}}, Ref{Float64}, Ref{Cint}),
+               ctx.h, Lenv.h, Renv.h, A.h, mpo[i], r, real(pf), imag(pf), 0, out, e, k)
+  end
+  check(ctx, rc, "dmrj_krylov_expm")
+  return gpuTens(out[]), e[]
+end
+
+normalize!(t::gpuTens) = check(context(), ccall((:dmrj_normalize, libdmrj), Cint, (Ptr{Cvoid}, Ptr{Cvoid}),
+                                                context().h, t.h), "dmrj_normalize")
+
+"tdvp_twosite (algs/time.jl:288-362): one half-sweep, step prefactor/(Ns-1)/2 per bond"
+function tdvp(psi::gpuMPS, mpo::gpuMPO; m::Integer=0, minm::Integer=2, cutoff::Float64=0., r::Integer=2,
+              prefactor::Number=1., Lenv=nothing, Renv=nothing, qtot=nothing)
+  ctx = context()
+  Ns = length(psi)
+  timestep = prefactor / (Ns - 1) / 2
+  if Lenv === nothing || Renv === nothing
+    Lenv, Renv = makeEnv(psi, mpo; qtot=qtot)
+  end
+  j = psi.oc == Ns ? -1 : 1
+  En = 0.
+  for _ = 1:Ns-1
+    i = psi.oc
+    iL, iR = j > 0 ? (i, i + 1) : (i - 1, i)
+    th = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ctx, ccall((:dmrj_theta2, libdmrj), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{Ptr{Cvoid}}),
+                     ctx.h, psi[iL].h, psi[iR].h, th), "dmrj_theta2")
+    AA = gpuTens(th[])
+    newpsi, En = krylovTimeEvol(Lenv[iL], Renv[iR], AA, mpo, iL, r, timestep)
+    free!(AA)
+    normalize!(newpsi)
+    nl = Ref{Ptr{Cvoid}}(C_NULL); nr = Ref{Ptr{Cvoid}}(C_NULL); te = Ref{Float64}(0.); kept = Ref{Int64}(0)
+    sv = zeros(8)
+    check(ctx, ccall((:dmrj_svd_trunc2, libdmrj), Cint,
+                     (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64, Float64, Cint, Ref{Ptr{Cvoid}}, Ref{Ptr{Cvoid}},
+                      Ref{Float64}, Ref{Int64}, Ptr{Float64}, Int64),
+                     ctx.h, newpsi.h, m, minm, cutoff, j > 0 ? 1 : -1, nl, nr, te, kept, sv, 8), "dmrj_svd_trunc2")
+    free!(newpsi); free!(psi[iL]); free!(psi[iR])
+    left = gpuTens(nl[]); right = gpuTens(nr[])
+    if j > 0
+      psi[iL] = left
+      Lenv[iR] = Lupdate(Lenv[iL], psi[iL], mpo[iL])
+      normalize!(right)
+      psi[iR], En = krylovTimeEvol(Lenv[iR], Renv[iR], right, mpo, iR, r, timestep; two=false)
+      free!(right)
+    else
+      psi[iR] = right
+      Renv[iL] = Rupdate(Renv[iR], psi[iR], mpo[iR])
+      normalize!(left)
+      psi[iL], En = krylovTimeEvol(Lenv[iL], Renv[iL], left, mpo, iL, r, timestep; two=false)
+      free!(left)
+    end
+    psi.oc += j
+    normalize!(psi[psi.oc])
+    if psi.oc == Ns && j > 0
+      j = -1
+    elseif psi.oc == 1 && j < 0
+      j = 1
+    end
+  end
+  return En, Lenv, Renv
 end
 
 end # module

@@ -45,12 +45,15 @@ enum {
   DMRJ_ERR_NUMERIC = -5   /* NaN/Inf or non-symmetric input detected */
 };
 
-/* precision of the H_eff / environment contractions: FP64 MFMA, or BF16
- * split precision (hi*hi + hi*lo + lo*hi on BF16 MFMA, FP32 accumulation);
+/* precision of the H_eff / environment contractions: FP64 MFMA, BF16
+ * split precision (hi*hi + hi*lo + lo*hi on BF16 MFMA, FP32 accumulation),
+ * or FP16 split precision with per-block scaling (every K-segment's operands
+ * scaled by powers of two to max ~2^14 before the FP16 hi/lo split, the same
+ * three products on FP16 MFMA, FP32 accumulation, exact unscaling);
  * OR DMRJ_PREC_CORRECT into the precision of lanczos2 / twostep to replace
  * the energy by the FP64 Rayleigh quotient of the Ritz vector (one FP64
  * H_eff application; residual norm via dmrj_ctx_last_residual). */
-enum { DMRJ_PREC_F64 = 0, DMRJ_PREC_BF16X3 = 1, DMRJ_PREC_CORRECT = 16 };
+enum { DMRJ_PREC_F64 = 0, DMRJ_PREC_BF16X3 = 1, DMRJ_PREC_F16X3 = 2, DMRJ_PREC_CORRECT = 16 };
 
 /* ---- context --------------------------------------------------------- */
 /* No counterpart in the reference (Julia holds no device state). */
@@ -91,7 +94,11 @@ int dmrj_ctx_marker(dmrj_ctx* ctx, int id);
  * one-sided Jacobi for sectors with min(R,C) <= 64 and one batched symmetric
  * eigensolve of the augmented matrices [[0,M],[M^T,0]] for the larger ones;
  * 1 = rocSOLVER gesdd per sector, 2 = gesvd per sector, 3 = batched Jacobi +
- * gesvd. */
+ * gesvd, 4 = batched Jacobi for small sectors + batched block one-sided
+ * Jacobi (columns of M or M^T in LDS-resident block pairs) for the larger
+ * ones, the other factor by a grouped GEMM with M; sectors too tall for LDS
+ * fall back to the augmented eigensolve.  Complex tensors always use a
+ * batched Hermitian eigensolve of [[0,M],[M^H,0]]. */
 int dmrj_ctx_set_svd_backend(dmrj_ctx* ctx, int mode);
 /* precision of dmrj_env_left / dmrj_env_right (DMRJ_PREC_F64 by default) */
 int dmrj_ctx_set_env_precision(dmrj_ctx* ctx, int precision);
@@ -239,7 +246,7 @@ int dmrj_normalize(dmrj_ctx* ctx, dmrj_tensor* x);
 /* ---- kernel self-tests (no counterpart in the reference) -------------- */
 /* One grouped-GEMM launch on host arrays: C[MxN] = op(A) op(B) with the tile
  * configuration cfg & 15 (0: 32x32, 1: 64x64, 2: 128x128) and the precision
- * cfg >> 4 (DMRJ_PREC_F64 or DMRJ_PREC_BF16X3).  A is stored KxM when
+ * cfg >> 4 (DMRJ_PREC_F64, DMRJ_PREC_BF16X3 or DMRJ_PREC_F16X3).  A is stored KxM when
  * tA != 0 (else MxK), B is stored NxK when tB != 0 (else KxN), column-major,
  * leading dimension = stored row count. */
 int dmrj_selftest_gemm(dmrj_ctx* ctx, int cfg, int64_t M, int64_t N, int64_t K, int tA, int tB,

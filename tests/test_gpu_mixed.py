@@ -1,11 +1,12 @@
-"""Precision form: BF16 split-precision H_eff / environment contractions with
-FP64 Lanczos scalars, FP64 SVD and FP64 energy correction, against the FP64
-drop-in.
+"""Precision forms: BF16 split precision and FP16 split precision with
+per-block scaling in the H_eff / environment contractions, with FP64 Lanczos
+scalars, FP64 SVD and FP64 energy correction, against the FP64 drop-in.
 
 Tolerances: one split-precision H_eff application agrees with FP64 to
-<= 1e-4 relative (about 2^-16 per product, FP32 accumulation); the FP64
-corrected energy of a split-precision run, after FP64 polish sweeps, matches
-the converged FP64 energy to |dE/E| <= 1e-10.
+<= 1e-4 relative for BF16x3 (about 2^-16 per product) and <= 1e-5 for FP16x3
+(about 2^-21, FP32 accumulation); the FP64 corrected energy of a
+split-precision run, after FP64 polish sweeps, matches the converged FP64
+energy to |dE/E| <= 1e-10.
 """
 import sys
 
@@ -37,16 +38,19 @@ def test_heff2_bf16x3_vs_f64(pkg, ctx):
     ctx.call("dmrj_theta2", dev[1].h, dev[2].h, Ct.byref(th))
     th = pkg.DeviceTensor(ctx, th)
     out = {}
-    for prec in (0, 1):
+    for prec in (0, 1, 2):
         h = Ct.c_void_p()
         ctx.call("dmrj_heff2_apply", L2.h, mpo.device(ctx, 2), mpo.device(ctx, 3), R3.h, th.h, prec, Ct.byref(h))
         out[prec] = pkg.DeviceTensor(ctx, h).to_dense()[0]
     err = np.max(np.abs(out[1] - out[0])) / np.max(np.abs(out[0]))
-    say(" rel err", err)
+    err16 = np.max(np.abs(out[2] - out[0])) / np.max(np.abs(out[0]))
+    say(" rel err bf16x3 %.2e f16x3 %.2e" % (err, err16))
     assert err < 1e-4
+    assert err16 < 1e-5
 
 
-def test_mixed_dmrg_u1_heisenberg_converges_to_f64(pkg, ctx):
+@pytest.mark.parametrize("precision", ["bf16x3", "f16x3"])
+def test_mixed_dmrg_u1_heisenberg_converges_to_f64(pkg, ctx, precision):
     """U(1) Heisenberg L=32, m=64: FP64 reference run vs split-precision
     sweeps (FP64 energy correction each sweep) followed by FP64 polish
     sweeps; the converged energies agree to 1e-10 relative."""
@@ -54,13 +58,13 @@ def test_mixed_dmrg_u1_heisenberg_converges_to_f64(pkg, ctx):
     qphys = np.array([[1], [-1]])
     states = [i % 2 for i in range(L)]
     mpo = pkg.MPO(pkg.makeMPO(pkg.heisenbergMPO(), 2, L).H, qphys=qphys)
-    say("case mixed U(1) L=32 m=64")
+    say("case mixed U(1) L=32 m=64", precision)
     p64 = pkg.product_state(states, 2, qphys=qphys, ctx=ctx)
     a64 = pkg.algvars(p64)
     pkg.dmrg(p64, mpo, m=64, sweeps=10, cutoff=1e-12, params=a64)
     pmx = pkg.product_state(states, 2, qphys=qphys, ctx=ctx)
     amx = pkg.algvars(pmx)
-    pkg.dmrg(pmx, mpo, m=64, sweeps=8, cutoff=1e-12, params=amx, precision="bf16x3", correction="sweep")
+    pkg.dmrg(pmx, mpo, m=64, sweeps=8, cutoff=1e-12, params=amx, precision=precision, correction="sweep")
     pkg.dmrg(pmx, mpo, m=64, sweeps=2, cutoff=1e-12, params=amx, precision="f64")
     for h64, hmx in zip(a64.history, amx.history):
         say(" sweep", h64["sweep"], "%.14f" % h64["energy"], "%.14f" % hmx["energy"], hmx["precision"],
