@@ -54,7 +54,10 @@ def main():
     ap.add_argument("--svd-backend", type=int, default=None, help="dmrj_ctx_set_svd_backend mode")
     ap.add_argument("--cutoff", type=float, default=0.0,
                     help="SVD cutoff of the ramp and timed sweeps (0: bond dimension capped by m only)")
-    ap.add_argument("--precision", default="f64")
+    ap.add_argument("--precision", default="f64", choices=["f64", "bf16x3", "f16x3"],
+                    help="arithmetic of the H_eff*psi GEMMs")
+    ap.add_argument("--env-precision", default=None, choices=["f64", "bf16x3", "f16x3"],
+                    help="arithmetic of the environment-update GEMMs (default: same as --precision)")
     ap.add_argument("--max-grow", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -117,20 +120,25 @@ def main():
         psi = pkg.product_state(states, 2, qphys=qphys, ctx=ctx)
         desc = "Heisenberg L=%d d=2 D=5 U(1) Sz=0" % L
     params = pkg.algvars(psi)
+
+    def sweeps(mm, n):
+        pkg.dmrg(psi, mpo, m=mm, sweeps=n, cutoff=args.cutoff, params=params, precision=args.precision,
+                 env_precision=args.env_precision)
+
     t_ramp = time.perf_counter()
     for mr in [int(x) for x in args.ramp.split(",") if x]:
-        pkg.dmrg(psi, mpo, m=mr, sweeps=1, cutoff=args.cutoff, params=params, precision=args.precision)
+        sweeps(mr, 1)
         log("ramp m=%d E=%.12f maxm=%d t=%.2fs" % (mr, params.energy, params.history[-1]["biggestm"],
                                                    params.history[-1]["time"]))
     # keep sweeping at the target m until the bond dimension has grown to it
     for _ in range(args.max_grow):
         if params.history and params.history[-1]["biggestm"] >= int(0.95 * m):
             break
-        pkg.dmrg(psi, mpo, m=m, sweeps=1, cutoff=args.cutoff, params=params, precision=args.precision)
+        sweeps(m, 1)
         log("grow m=%d E=%.12f maxm=%d t=%.2fs" % (m, params.energy, params.history[-1]["biggestm"],
                                                    params.history[-1]["time"]))
     for _ in range(args.warmup):
-        pkg.dmrg(psi, mpo, m=m, sweeps=1, cutoff=args.cutoff, params=params, precision=args.precision)
+        sweeps(m, 1)
         log("warmup m=%d E=%.12f maxm=%d t=%.2fs" % (m, params.energy, params.history[-1]["biggestm"],
                                                      params.history[-1]["time"]))
     log("ramp+warmup %.1fs" % (time.perf_counter() - t_ramp))
@@ -138,12 +146,12 @@ def main():
     ctx.profile(True)
     ctx.sync()
     ctx.flops(reset=True)
-    for k in ("gemm_f64", "mix", "copy", "svd", "gemm_lp", "graph"):
+    for k in ("gemm_f64", "mix", "copy", "svd", "gemm_lp", "graph", "gemm_f16"):
         ctx.kernel_stats(k, reset=True)
     tune0 = ctx.autotune_stats()
     ctx.marker(1)       # kernel-trace cut: start of the timed region
     t0 = time.perf_counter()
-    pkg.dmrg(psi, mpo, m=m, sweeps=args.steps, cutoff=args.cutoff, params=params, precision=args.precision)
+    sweeps(m, args.steps)
     ctx.sync()
     t1 = time.perf_counter()
     ctx.marker(2)       # end of the timed region
@@ -151,18 +159,24 @@ def main():
     tune1 = ctx.autotune_stats()
     elapsed = t1 - t0
     fl_heff, fl_env = ctx.flops()
-    stats = {k: ctx.kernel_stats(k) for k in ("gemm_f64", "mix", "copy", "svd", "gemm_lp", "graph")}
+    stats = {k: ctx.kernel_stats(k) for k in ("gemm_f64", "mix", "copy", "svd", "gemm_lp", "graph", "gemm_f16")}
     ctx.profile(False)
     for h in params.history[-args.steps:]:
         log("timed sweep E=%.12f maxtrunc=%.3e m=%d t=%.3fs" % (h["energy"], h["maxtrunc"], h["biggestm"],
                                                                h["time"]))
     sweep_s = elapsed / args.steps
     tflops = (fl_heff + fl_env) / elapsed / 1e12
-    g = stats["gemm_f64"]
+    # dominant kernel: the grouped GEMM of the arithmetic the sweep ran in
+    gkind, gname, gpeak = {"f64": ("gemm_f64", "k_gemm_f64 (FP64 grouped GEMM, MFMA 16x16x4)", PEAK_FP64),
+                           "bf16x3": ("gemm_lp", "k_gemm_bf16x3 (BF16x3 split grouped GEMM, MFMA 16x16x32)",
+                                      PEAK_BF16),
+                           "f16x3": ("gemm_f16", "k_gemm_f16x3 (scaled FP16x3 split grouped GEMM, MFMA "
+                                     "16x16x32)", PEAK_BF16)}[args.precision]
+    g = stats[gkind]
     g_tflops = g["flops"] / (g["ms"] * 1e-3) / 1e12 if g["ms"] > 0 else 0.0
     g_avg_ms = g["ms"] / max(1, g["launches"])
-    log("gemm launches=%d avg=%.4f ms total=%.3f s, %.2f TFLOP/s; mix %.3f s %.0f GB/s; copy %.3f s; "
-        "svd %.3f s" % (g["launches"], g_avg_ms, g["ms"] / 1e3, g_tflops, stats["mix"]["ms"] / 1e3,
+    log("%s launches=%d avg=%.4f ms total=%.3f s, %.2f TFLOP/s; mix %.3f s %.0f GB/s; copy %.3f s; "
+        "svd %.3f s" % (gkind, g["launches"], g_avg_ms, g["ms"] / 1e3, g_tflops, stats["mix"]["ms"] / 1e3,
                         stats["mix"]["bytes"] / max(1e-9, stats["mix"]["ms"] * 1e-3) / 1e9,
                         stats["copy"]["ms"] / 1e3, stats["svd"]["ms"] / 1e3))
 
@@ -188,16 +202,21 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "f64" else args.precision,
+        "dtype": {"f64": "f64", "bf16x3": "bf16x3 (FP32 accumulate, FP64 Lanczos scalars/SVD/energy)",
+                  "f16x3": "f16x3 per-block scaled (FP32 accumulate, FP64 Lanczos scalars/SVD/energy)"}[
+                      args.precision],
         "data": "synthetic (seeded random product state, Heisenberg MPO)",
         "config": {"workload": "%s, two-site DMRG sweep at m=%d (ramp %s), Krylov 2, cutoff %g"
                                % (desc, m, args.ramp, args.cutoff), "model": args.model,
-                   "L": L, "m": m, "energy": params.energy,
+                   "L": L, "m": m, "energy": params.energy, "precision": args.precision,
+                   "env_precision": args.env_precision or args.precision,
+                   "svd_backend": args.svd_backend if args.svd_backend is not None else "default",
                    "heff_flops_per_sweep": fl_heff / args.steps, "env_flops_per_sweep": fl_env / args.steps,
                    "sweep_s": sweep_s},
-        "roofline": {"bound": "mfma", "kernel": "k_gemm_f64 (FP64 grouped GEMM, MFMA 16x16x4)",
-                     "achieved": round(g_tflops, 3), "peak": PEAK_FP64, "unit": "TFLOP/s",
-                     "frac": round(g_tflops / PEAK_FP64, 4), "traffic": traffic,
+        "roofline": {"bound": "mfma", "kernel": gname,
+                     "achieved": round(g_tflops, 3), "peak": gpeak, "unit": "TFLOP/s",
+                     "frac": round(g_tflops / gpeak, 4), "traffic": traffic,
+                     "flops_basis": "algorithmic FP64-equivalent flops (split forms execute 3 MFMA products each)",
                      "traffic_note": traffic_note,
                      "avg_launch_ms": round(g_avg_ms, 5), "launches": g["launches"],
                      "flops_per_launch": g["flops"] / max(1, g["launches"]),

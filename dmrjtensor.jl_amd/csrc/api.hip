@@ -1014,6 +1014,7 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
       }
   }
   launch_copy(c, gather);
+  double aa2 = 0.0;  // |AA|^2
   {
     // never hand non-finite data to the solver
     const double* xs[1] = {AA.data};
@@ -1022,6 +1023,7 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
     HIP_OK(hipMemcpyAsync(&c.hscal[S_CHK], c.scal + S_CHK, sizeof(double), hipMemcpyDeviceToHost, c.stream));
     HIP_OK(hipStreamSynchronize(c.stream));
     chk = c.hscal[S_CHK];
+    aa2 = chk;
     if (!std::isfinite(chk)) {
       for (auto& s : secs) {
         c.alloc.release(s.M);
@@ -1079,7 +1081,8 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
       s.rowsX = rows;
       s.ncolsX = ncols;
       is_small[si] = 1;  // handled
-      jobs.push_back({s.X, s.Sn, nullptr, rows, ncols, nblk, bw});
+      const double tiny = 8.0 * 2.220446049250313e-16 * std::sqrt((double)rows);
+      jobs.push_back({s.X, s.Sn, nullptr, tiny * tiny * aa2, rows, ncols, nblk, bw});
       const double mm = (double)std::max(s.R, s.C), nn = (double)std::min(s.R, s.C);
       svd_flops += 4.0 * mm * nn * nn + 8.0 * nn * nn * nn;
     }

@@ -288,6 +288,7 @@ struct BjJob {
   double* X;      // rows x ncols working matrix, ld rows, orthogonalised in place
   double* S;      // ncols column norms (output of launch_bj_norms)
   int* flag;      // set to 1 by a round that rotated
+  double tiny2;   // columns with |x|^2 below this are numerically zero (rank deficiency)
   int rows, ncols, nblk, bw;  // nblk even (an empty block pads an odd count)
 };
 struct BjGather {

@@ -13,7 +13,10 @@
 // pair, 3 dot products and a plane rotation), then writes them back.  Rounds
 // repeat as outer sweeps until a sweep rotates nothing beyond
 // |x_p . x_q| <= tol sqrt(|x_p|^2 |x_q|^2), tol = sqrt(rows) eps (the
-// threshold of LAPACK's dgesvj).  Column norms are then the singular values
+// threshold of LAPACK's dgesvj); columns whose norm fell to the rounding
+// level of the whole tensor (8 eps sqrt(rows) |AA|) are numerically zero and
+// take no further rotations (a sector with more columns than rows has
+// ncols - rank of them).  Column norms are then the singular values
 // and the normalised columns the singular vectors of that side; the other
 // factor comes from one grouped GEMM with M (U^T M = D V^T or M V = U D), so
 // W is never accumulated.
@@ -106,7 +109,9 @@ __global__ __launch_bounds__(kBjThreads) void k_bj_step(const BjJob* __restrict_
             a = bj_group_sum(a, G);
             b = bj_group_sum(b, G);
             g = bj_group_sum(g, G);
-            if (g != 0.0 && fabs(g) > tol * sqrt(a * b)) {
+            // a column at the rounding level of the whole tensor is numerically
+            // zero (more columns than rank): leave it, it carries no weight
+            if (g != 0.0 && fabs(g) > tol * sqrt(a * b) && fmin(a, b) > jb.tiny2) {
               const double zeta = (b - a) / (2.0 * g);
               const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
               const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;

@@ -29,7 +29,7 @@ class algvars:
     reference's algvars)."""
 
     def __init__(self, psi, maxm=0, minm=2, sweeps=1, cutoff=0.0, goal=0.0, silent=True,
-                 SvNbond=None, precision="f64", correction="none"):
+                 SvNbond=None, precision="f64", correction="none", env_precision=None):
         self.maxm = maxm
         self.minm = minm
         self.sweeps = sweeps
@@ -38,6 +38,8 @@ class algvars:
         self.silent = silent
         self.SvNbond = (len(psi) + 1) // 2 if SvNbond is None else SvNbond
         self.precision = precision
+        # arithmetic of the environment updates (None: same as precision)
+        self.env_precision = env_precision
         # FP64 energy correction of split-precision runs: "none", "bond" (every
         # bond step) or "sweep" (the last bond step of each sweep, whose energy
         # is the one a sweep reports)
@@ -159,7 +161,8 @@ def optmps(psi, mpo, params, stepfct=twostep, cvgfct=dmrgcvg, measfct=startval):
         j = -1
     else:
         j = 1
-    psi.ctx.call("dmrj_ctx_set_env_precision", _lib.PREC[params.precision] & 15)
+    envp = params.env_precision or params.precision
+    psi.ctx.call("dmrj_ctx_set_env_precision", _lib.PREC[envp] & 15)
     try:
         _sweeps(psi, mpo, params, stepfct, cvgfct, Lenv, Renv, Nsteps, j)
     finally:
@@ -191,7 +194,7 @@ def _sweeps(psi, mpo, params, stepfct, cvgfct, Lenv, Renv, Nsteps, j):
 
 
 def dmrg(psi, mpo, m=0, minm=2, sweeps=1, cutoff=0.0, silent=True, goal=0.0, method="twosite",
-         SvNbond=None, precision="f64", correction=None, params=None):
+         SvNbond=None, precision="f64", correction=None, params=None, env_precision=None):
     """dmrg(psi, mpo; m, minm, sweeps, cutoff, silent, goal, method)
     (algs/DMRG.jl:549-589).  Optimises ``psi`` in place and returns the
     energy; the per-sweep report is in ``params.history`` when a params
@@ -204,6 +207,9 @@ def dmrg(psi, mpo, m=0, minm=2, sweeps=1, cutoff=0.0, silent=True, goal=0.0, met
     params.silent, params.goal, params.precision = silent, goal, precision
     if precision not in ("f64", "bf16x3", "f16x3"):
         raise ValueError("precision must be 'f64', 'bf16x3' or 'f16x3'")
+    if env_precision not in (None, "f64", "bf16x3", "f16x3"):
+        raise ValueError("env_precision must be None, 'f64', 'bf16x3' or 'f16x3'")
+    params.env_precision = env_precision
     params.correction = correction if correction is not None else ("none" if precision == "f64" else "sweep")
     if SvNbond is not None:
         params.SvNbond = SvNbond

@@ -133,7 +133,8 @@ def test_complex_svd_truncation(pkg, ctx):
         kk = k.value
         say(" m", m, "kept", kk, "truncerr %.3e %.3e" % (te.value, terr))
         assert kk == len(s)
-        assert np.allclose(sv[:kk], s, rtol=1e-12, atol=1e-14)
+        # the C ABI returns the kept values sorted; the oracle keeps sector order
+        assert np.allclose(np.sort(sv[:kk])[::-1], np.sort(s)[::-1], rtol=1e-12, atol=1e-14)
         assert abs(te.value - terr) <= 1e-12 * max(terr, 1e-300) + 1e-15
         gl = pkg.DeviceTensor(ctx, l)
         gr = pkg.DeviceTensor(ctx, r)

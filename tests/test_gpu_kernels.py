@@ -145,7 +145,8 @@ def test_svd_block_jacobi_split(pkg, ctx, direction, nq):
         U, s, V, terr, _ = od.fsvd(AA, m, 2, 0.0, qL, qP, qR)
         say(" bj nq", nq, "dir", direction, "m", m, "kept", len(sv), "te %.3e %.3e" % (te, terr))
         assert len(sv) == len(s)
-        assert np.allclose(sv, s, rtol=1e-11, atol=1e-14)
+        # the C ABI returns the kept values sorted; the oracle keeps sector order
+        assert np.allclose(sv, np.sort(s)[::-1], rtol=1e-11, atol=1e-14)
         assert abs(te - terr) <= 1e-10 * max(terr, 1e-300) + 1e-15
         A = gl.to_dense(nq)[0]
         B = gr.to_dense(nq)[0]

@@ -73,3 +73,29 @@ def test_mixed_dmrg_u1_heisenberg_converges_to_f64(pkg, ctx, precision):
     assert abs(e64 - emx) / abs(e64) <= 1e-10
     # the corrected split-precision energy of each sweep is an FP64 Rayleigh quotient
     assert all(h["residual"] is not None for h in amx.history[:8])
+
+
+@pytest.mark.parametrize("precision,tol", [("f16x3", 1e-10), ("bf16x3", 1e-8)])
+def test_mixed_heff_fp64_env_energy_without_polish(pkg, ctx, precision, tol):
+    """Split-precision H_eff*psi with FP64 environments and the FP64 energy
+    correction at the end of every sweep: the corrected energy is a Rayleigh
+    quotient of a state whose error is first order in the product error, so
+    it meets the FP64 energy at second order without any FP64 polish sweep
+    (FP16x3: ~2^-22 per product, |dE/E| <= 1e-10; BF16x3: ~2^-16, <= 1e-8)."""
+    L = 32
+    qphys = np.array([[1], [-1]])
+    states = [i % 2 for i in range(L)]
+    mpo = pkg.MPO(pkg.makeMPO(pkg.heisenbergMPO(), 2, L).H, qphys=qphys)
+    say("case mixed H_eff, FP64 env", precision)
+    p64 = pkg.product_state(states, 2, qphys=qphys, ctx=ctx)
+    a64 = pkg.algvars(p64)
+    pkg.dmrg(p64, mpo, m=64, sweeps=8, cutoff=1e-12, params=a64)
+    pmx = pkg.product_state(states, 2, qphys=qphys, ctx=ctx)
+    amx = pkg.algvars(pmx)
+    pkg.dmrg(pmx, mpo, m=64, sweeps=8, cutoff=1e-12, params=amx, precision=precision, correction="sweep",
+             env_precision="f64")
+    for h64, hmx in zip(a64.history, amx.history):
+        say(" sweep", h64["sweep"], "%.14f" % h64["energy"], "%.14f" % hmx["energy"],
+            "rel %.2e" % (abs(h64["energy"] - hmx["energy"]) / abs(h64["energy"])))
+    e64, emx = a64.history[-1]["energy"], amx.history[-1]["energy"]
+    assert abs(e64 - emx) / abs(e64) <= tol
