@@ -65,6 +65,7 @@ def main():
                     help="capture each Lanczos step into a hipGraph (kernels inside graphs are not timed "
                          "individually, so the roofline then covers the environment-update GEMMs only)")
     ap.add_argument("--no-autotune", action="store_true", help="fixed tile-shape heuristic instead of the autotuner")
+    ap.add_argument("--no-xcd-order", action="store_true", help="plain descending-work tile order")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_k_gemm_f64.json"),
                     help="HBM bytes per k_gemm_f64 launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE")
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -86,6 +87,8 @@ def main():
         return
     if args.no_autotune:
         ctx.set_autotune(0)
+    if args.no_xcd_order:
+        ctx.set_tile_order(0)
     if args.graph:
         ctx.set_graph(1)
     if args.svd_backend is not None:

@@ -31,6 +31,7 @@ SIGNATURES = {
     "dmrj_ctx_memstats": [_p, _i64p],
     "dmrj_ctx_set_gemm_cfg": [_p, C.c_int],
     "dmrj_ctx_set_autotune": [_p, C.c_int],
+    "dmrj_ctx_set_tile_order": [_p, C.c_int],
     "dmrj_ctx_autotune_stats": [_p, _dp],
     "dmrj_ctx_set_graph": [_p, C.c_int],
     "dmrj_ctx_marker": [_p, C.c_int],
@@ -173,6 +174,10 @@ class Context:
 
     def set_gemm_cfg(self, cfg):
         self.call("dmrj_ctx_set_gemm_cfg", int(cfg))
+
+    def set_tile_order(self, mode):
+        """1 = XCD-aware grouped-GEMM tile order (default), 0 = descending work only."""
+        self.call("dmrj_ctx_set_tile_order", int(mode))
 
     def set_autotune(self, mode):
         """1 = autotune GEMM tile shapes per block-size histogram, 0 = heuristic, 2 = clear cache."""

@@ -1848,6 +1848,13 @@ int dmrj_ctx_set_gemm_cfg(dmrj_ctx* c, int cfg) {
   });
 }
 
+int dmrj_ctx_set_tile_order(dmrj_ctx* c, int mode) {
+  return guard(c, [&] {
+    if (mode < 0 || mode > 1) throw Error(DMRJ_ERR_ARG, "tile order must be 0 or 1");
+    c->xcd_order = mode;
+  });
+}
+
 int dmrj_ctx_set_autotune(dmrj_ctx* c, int mode) {
   return guard(c, [&] {
     if (mode < 0 || mode > 2) throw Error(DMRJ_ERR_ARG, "autotune mode must be 0, 1 or 2");

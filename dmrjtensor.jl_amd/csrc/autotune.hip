@@ -34,6 +34,51 @@ static int occupancy(Context& c, int cfg, int arith) {
   return o;
 }
 
+// XCD-aware order.  Work-group b of a launch runs on XCD b mod 8 (the
+// dispatcher deals work-groups round-robin over the eight XCDs, each with its
+// own L2), and the persistent grid (a multiple of 8, or one work-group per
+// tile) gives tile t to work-group t mod grid, i.e. to XCD t mod 8.  Tiles
+// that share a B panel (same task, same column block) form a group; groups
+// are dealt to eight streams by least accumulated work in descending-work
+// order, and position 8i + x of the launch order takes the i-th tile of
+// stream x, so each XCD's L2 sees whole panel groups.
+static std::vector<GemmTile> xcd_interleave(const std::vector<GemmTile>& tiles, const std::vector<GemmTask>& tasks,
+                                            const std::vector<GemmSeg>& segs) {
+  constexpr int kXcd = 8;
+  std::vector<std::vector<GemmTile>> stream(kXcd);
+  std::vector<double> load(kXcd, 0.0);
+  size_t i = 0;
+  while (i < tiles.size()) {
+    size_t j = i + 1;
+    while (j < tiles.size() && tiles[j].task == tiles[i].task && tiles[j].n0 == tiles[i].n0) ++j;
+    const GemmTask& t = tasks[tiles[i].task];
+    double k = 0.0;
+    for (int q = 0; q < t.nseg; ++q) k += segs[t.seg0 + q].K;
+    const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    for (size_t e = i; e < j; ++e) stream[x].push_back(tiles[e]);
+    load[x] += (double)(j - i) * (k + 1.0);
+    i = j;
+  }
+  std::vector<GemmTile> out;
+  out.reserve(tiles.size());
+  std::vector<size_t> pos(kXcd, 0);
+  while (out.size() < tiles.size()) {
+    for (int x = 0; x < kXcd && out.size() < tiles.size(); ++x) {
+      int src = x;
+      if (pos[src] >= stream[src].size()) {  // exhausted: borrow from the longest remaining stream
+        size_t best = 0;
+        for (int y = 0; y < kXcd; ++y)
+          if (stream[y].size() - pos[y] > best) {
+            best = stream[y].size() - pos[y];
+            src = y;
+          }
+      }
+      out.push_back(stream[src][pos[src]++]);
+    }
+  }
+  return out;
+}
+
 GemmLaunch prepare_gemm(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs,
                         int cfg, int arith) {
   if (cfg < 0 || cfg >= kNumGemmShapes) throw Error(DMRJ_ERR_ARG, "GEMM tile config out of range");
@@ -58,6 +103,7 @@ GemmLaunch prepare_gemm(Context& c, const std::vector<GemmTask>& tasks, const st
       for (int m0 = 0; m0 < t.M; m0 += bm) tiles.push_back({o.second, m0, n0, cfg});
   }
   if (tiles.empty()) return g;
+  if (c.xcd_order) tiles = xcd_interleave(tiles, tasks, segs);
   g.tiles = (const GemmTile*)c.upload(tiles.data(), tiles.size() * sizeof(GemmTile));
   g.tasks = (const GemmTask*)c.upload(tasks.data(), tasks.size() * sizeof(GemmTask));
   g.segs = (const GemmSeg*)c.upload(segs.data(), std::max<size_t>(1, segs.size()) * sizeof(GemmSeg));

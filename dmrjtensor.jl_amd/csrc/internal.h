@@ -201,6 +201,7 @@ struct Context {
   int last_gemm_cfg = -1;
   // grouped-GEMM autotuner: 1 = time the tile shapes per histogram key
   int autotune = 1;
+  int xcd_order = 1;  // 1: grouped-GEMM tiles dealt to XCDs by shared panel
   // 1 = each Lanczos step (both H_eff applications, inner products and the
   // Ritz update) is captured into one hipGraph and launched as a unit
   int graph_mode = 0;

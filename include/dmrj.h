@@ -73,6 +73,11 @@ int dmrj_ctx_set_gemm_cfg(dmrj_ctx* ctx, int cfg);
  * 2 = clear the cache and counters.  The shape never changes result bits.
  * A cfg set by dmrj_ctx_set_gemm_cfg takes precedence. */
 int dmrj_ctx_set_autotune(dmrj_ctx* ctx, int mode);
+/* Grouped-GEMM tile order (no counterpart in the reference): 1 (default) =
+ * tiles sharing a B panel are dealt to the same XCD (work-group b runs on XCD
+ * b mod 8, each XCD has its own L2); 0 = plain descending-work order.  The
+ * order never changes result bits. */
+int dmrj_ctx_set_tile_order(dmrj_ctx* ctx, int mode);
 /* out5 = {cached keys, cache hits, tuned misses, launches not tunable
  *         (beta != 0 or aliasing), total tuning time in ms} */
 int dmrj_ctx_autotune_stats(dmrj_ctx* ctx, double* out5);

@@ -112,32 +112,38 @@ def test_hubbard_L10_u1xu1_reference_constant(pkg, ctx):
 
 
 def test_autotune_is_result_neutral(pkg, ctx):
-    """The grouped-GEMM autotuner picks tile shapes only: every shape sums K
-    in the same order, so per-sweep energies and spectra are bit-identical
-    with the tuner on and off."""
+    """The grouped-GEMM autotuner picks tile shapes and the XCD-aware order
+    deals tiles to work-groups only: every shape sums K in the same order, so
+    per-sweep energies and spectra are bit-identical with the tuner on and
+    off and with either tile order."""
     say("case autotune on/off U(1) L=32 m=96")
     L = 32
     qphys = np.array([[1], [-1]])
     states = [i % 2 for i in range(L)]
     qmpo = pkg.MPO(pkg.makeMPO(pkg.heisenbergMPO(), 2, L).H, qphys=qphys)
     runs = []
-    for mode in (0, 1):
+    for mode, order in ((0, 1), (1, 1), (1, 0)):
         ctx.set_autotune(2)
         ctx.set_autotune(mode)
-        psi = pkg.product_state(states, 2, qphys=qphys, ctx=ctx)
-        params = pkg.algvars(psi)
-        pkg.dmrg(psi, qmpo, m=96, sweeps=3, cutoff=1e-10, params=params)
+        ctx.set_tile_order(order)
+        try:
+            psi = pkg.product_state(states, 2, qphys=qphys, ctx=ctx)
+            params = pkg.algvars(psi)
+            pkg.dmrg(psi, qmpo, m=96, sweeps=3, cutoff=1e-10, params=params)
+        finally:
+            ctx.set_tile_order(1)
         st = ctx.autotune_stats()
-        say(" mode", mode, "E %.15f" % params.energy, st)
+        say(" autotune", mode, "xcd order", order, "E %.15f" % params.energy, st)
         runs.append(params.history)
         if mode == 1:
             assert st["keys"] > 0 and st["tuned"] > 0
             assert st["hits"] > 0
     ctx.set_autotune(1)
-    for a, b in zip(*runs):
-        assert a["energy"] == b["energy"]
-        assert a["maxtrunc"] == b["maxtrunc"]
-        assert list(a["storeD"]) == list(b["storeD"])
+    for other in runs[1:]:
+        for a, b in zip(runs[0], other):
+            assert a["energy"] == b["energy"]
+            assert a["maxtrunc"] == b["maxtrunc"]
+            assert list(a["storeD"]) == list(b["storeD"])
 
 
 def test_graph_captured_lanczos_matches_direct(pkg, ctx):
