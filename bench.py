@@ -152,6 +152,7 @@ def main():
     for k in ("gemm_f64", "mix", "copy", "svd", "gemm_lp", "graph", "gemm_f16"):
         ctx.kernel_stats(k, reset=True)
     tune0 = ctx.autotune_stats()
+    ctx.svd_stats(reset=True)
     ctx.marker(1)       # kernel-trace cut: start of the timed region
     t0 = time.perf_counter()
     sweeps(m, args.steps)
@@ -160,6 +161,7 @@ def main():
     ctx.marker(2)       # end of the timed region
     ctx.sync()
     tune1 = ctx.autotune_stats()
+    svdst = ctx.svd_stats()
     elapsed = t1 - t0
     fl_heff, fl_env = ctx.flops()
     stats = {k: ctx.kernel_stats(k) for k in ("gemm_f64", "mix", "copy", "svd", "gemm_lp", "graph", "gemm_f16")}
@@ -214,6 +216,8 @@ def main():
                    "L": L, "m": m, "energy": params.energy, "precision": args.precision,
                    "env_precision": args.env_precision or args.precision,
                    "svd_backend": args.svd_backend if args.svd_backend is not None else "default",
+                   "svd_s_per_step": round(stats["svd"]["ms"] / 1e3 / args.steps, 4),
+                   "jacobi_sweeps_per_svd": round(svdst["sweeps"] / svdst["calls"], 2) if svdst["calls"] else None,
                    "heff_flops_per_sweep": fl_heff / args.steps, "env_flops_per_sweep": fl_env / args.steps,
                    "sweep_s": sweep_s},
         "roofline": {"bound": "mfma", "kernel": gname,

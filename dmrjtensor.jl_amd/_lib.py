@@ -32,6 +32,7 @@ SIGNATURES = {
     "dmrj_ctx_set_gemm_cfg": [_p, C.c_int],
     "dmrj_ctx_set_autotune": [_p, C.c_int],
     "dmrj_ctx_set_tile_order": [_p, C.c_int],
+    "dmrj_ctx_svd_stats": [_p, _dp, C.c_int],
     "dmrj_ctx_autotune_stats": [_p, _dp],
     "dmrj_ctx_set_graph": [_p, C.c_int],
     "dmrj_ctx_marker": [_p, C.c_int],
@@ -174,6 +175,12 @@ class Context:
 
     def set_gemm_cfg(self, cfg):
         self.call("dmrj_ctx_set_gemm_cfg", int(cfg))
+
+    def svd_stats(self, reset=False):
+        """Block-Jacobi SVD: calls and total sweeps since the last reset."""
+        out = np.zeros(2)
+        self.call("dmrj_ctx_svd_stats", dptr(out), 1 if reset else 0)
+        return dict(calls=int(out[0]), sweeps=int(out[1]))
 
     def set_tile_order(self, mode):
         """1 = XCD-aware grouped-GEMM tile order (default), 0 = descending work only."""

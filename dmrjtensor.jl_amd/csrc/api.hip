@@ -954,7 +954,7 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
     double *M = nullptr, *U = nullptr, *Vt = nullptr, *s = nullptr, *E = nullptr;
     int kmin = 0, keep = 0;
     // block-Jacobi sectors: working matrix, its column norms, sort order
-    double *X = nullptr, *Sn = nullptr;
+    double* X = nullptr;
     int bj = 0, rowsX = 0, ncolsX = 0;
     std::vector<int> perm;
     std::vector<double> hn;
@@ -1060,8 +1060,12 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
   // orthonormal U wanted) or on M^T (dir < 0, orthonormal V wanted); the
   // other factor is one grouped GEMM with M after the truncation
   int* bjflag = nullptr;
+  double* bjnorm = nullptr;
   if (!z && c.svd_backend == 4) {
     std::vector<BjJob> jobs;
+    std::vector<int> noff;  // offset of every job's columns in the shared norm / permutation arrays
+    int64_t ncol_all = 0;
+    const double atol = 2.220446049250313e-16 * std::sqrt(aa2);
     for (int si : order) {
       Sec& s = secs[si];
       if (s.kmin == 0 || is_small[si]) continue;
@@ -1070,7 +1074,6 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
       while (bw > 4 && bj_lds_bytes(rows, bw) > c.lds_max) bw >>= 1;
       if (bj_lds_bytes(rows, bw) > c.lds_max) continue;  // too tall for LDS: augmented solver below
       s.X = (double*)c.alloc.alloc((int64_t)rows * ncols * sizeof(double));
-      s.Sn = (double*)c.alloc.alloc((int64_t)ncols * sizeof(double));
       if (dir > 0)
         HIP_OK(hipMemcpyAsync(s.X, s.M, (int64_t)rows * ncols * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
       else
@@ -1081,35 +1084,65 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
       s.rowsX = rows;
       s.ncolsX = ncols;
       is_small[si] = 1;  // handled
-      const double tiny = 8.0 * 2.220446049250313e-16 * std::sqrt((double)rows);
-      jobs.push_back({s.X, s.Sn, nullptr, tiny * tiny * aa2, rows, ncols, nblk, bw});
+      jobs.push_back({s.X, nullptr, nullptr, nullptr, atol * std::sqrt((double)rows), rows, ncols, nblk, bw});
+      noff.push_back((int)ncol_all);
+      ncol_all += ncols;
       const double mm = (double)std::max(s.R, s.C), nn = (double)std::min(s.R, s.C);
       svd_flops += 4.0 * mm * nn * nn + 8.0 * nn * nn * nn;
     }
     if (!jobs.empty()) {
       const int nj = (int)jobs.size();
       bjflag = (int*)c.alloc.alloc(nj * sizeof(int));
+      bjnorm = (double*)c.alloc.alloc(std::max<int64_t>(ncol_all, 1) * sizeof(double));
       size_t lds = 0;
       for (int j = 0; j < nj; ++j) {
         jobs[j].flag = bjflag + j;
+        jobs[j].S = bjnorm + noff[j];
         lds = std::max(lds, bj_lds_bytes(jobs[j].rows, jobs[j].bw));
       }
-      auto* djobs = (BjJob*)c.upload(jobs.data(), nj * sizeof(BjJob));
+      std::vector<int> hperm(ncol_all);
+      std::vector<double> hnorm(ncol_all);
       std::vector<char> active(nj, 1);
       std::vector<int> hflag(nj, 0);
-      constexpr int kBjMaxSweeps = 40;
+      auto norms_to_host = [&](bool all) {
+        std::vector<int2> nw;
+        for (int j = 0; j < nj; ++j)
+          if (all || active[j])
+            for (int col = 0; col < jobs[j].ncols; ++col) nw.push_back(make_int2(j, col));
+        auto* dj = (BjJob*)c.upload(jobs.data(), nj * sizeof(BjJob));
+        launch_bj_norms(c, dj, nw);
+        HIP_OK(hipMemcpyAsync(hnorm.data(), bjnorm, ncol_all * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+      };
+      // de Rijk ordering: columns by decreasing norm before every sweep
+      auto sort_active = [&]() {
+        for (int j = 0; j < nj; ++j) {
+          if (!active[j]) continue;
+          int* p = hperm.data() + noff[j];
+          const double* nrm = hnorm.data() + noff[j];
+          std::iota(p, p + jobs[j].ncols, 0);
+          std::stable_sort(p, p + jobs[j].ncols, [&](int a, int b) { return nrm[a] > nrm[b]; });
+        }
+      };
+      norms_to_host(true);
+      HIP_OK(hipStreamSynchronize(c.stream));
+      sort_active();
+      constexpr int kBjMaxSweeps = 60;
       int sweep = 0;
       for (; sweep < kBjMaxSweeps; ++sweep) {
         std::vector<int2> work;
         int nsteps = 0;
+        const int* dperm = (const int*)c.upload(hperm.data(), ncol_all * sizeof(int));
         for (int j = 0; j < nj; ++j) {
+          jobs[j].perm = dperm + noff[j];
           if (!active[j]) continue;
           for (int k = 0; k < jobs[j].nblk / 2; ++k) work.push_back(make_int2(j, k));
           nsteps = std::max(nsteps, jobs[j].nblk - 1);
         }
         if (work.empty()) break;
+        auto* djobs = (BjJob*)c.upload(jobs.data(), nj * sizeof(BjJob));
         HIP_OK(hipMemsetAsync(bjflag, 0, nj * sizeof(int), c.stream));
         launch_bj_sweep(c, djobs, work, nsteps, 8, lds);
+        norms_to_host(false);
         HIP_OK(hipMemcpyAsync(hflag.data(), bjflag, nj * sizeof(int), hipMemcpyDeviceToHost, c.stream));
         HIP_OK(hipStreamSynchronize(c.stream));
         bool any = false;
@@ -1118,12 +1151,20 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
           any = any || active[j];
         }
         if (!any) break;
+        sort_active();
       }
       if (sweep == kBjMaxSweeps) throw Error(DMRJ_ERR_SOLVER, "block Jacobi SVD did not converge");
-      std::vector<int2> nw;
-      for (int j = 0; j < nj; ++j)
-        for (int col = 0; col < jobs[j].ncols; ++col) nw.push_back(make_int2(j, col));
-      launch_bj_norms(c, djobs, nw);
+      c.bj_sweeps += sweep + 1;
+      c.bj_calls += 1;
+      // final column norms of every sector are in hnorm (the last sweep of
+      // each sector rotated nothing, so its norms are current)
+      int jj = 0;
+      for (int si : order) {
+        Sec& s = secs[si];
+        if (!s.bj) continue;
+        s.hn.assign(hnorm.begin() + noff[jj], hnorm.begin() + noff[jj] + s.ncolsX);
+        ++jj;
+      }
     }
   }
   // large sectors, default backend: one batched symmetric eigensolve of the
@@ -1232,8 +1273,7 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
       Sec& s = secs[si];
       if (s.kmin == 0) continue;
       if (s.bj) {
-        s.hn.resize(s.ncolsX);
-        HIP_OK(hipMemcpyAsync(s.hn.data(), s.Sn, s.ncolsX * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        // column norms already on the host (block-Jacobi loop)
       } else {
         HIP_OK(hipMemcpyAsync(hs.data() + o, s.s, s.kmin * sizeof(double), hipMemcpyDeviceToHost, c.stream));
       }
@@ -1424,9 +1464,9 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
     c.alloc.release(s.s);
     c.alloc.release(s.E);
     c.alloc.release(s.X);
-    c.alloc.release(s.Sn);
   }
   c.alloc.release(bjflag);
+  c.alloc.release(bjnorm);
   c.alloc.release(info);
   c.alloc.release(dscal);
   res.L = tl;
@@ -1845,6 +1885,15 @@ int dmrj_ctx_set_gemm_cfg(dmrj_ctx* c, int cfg) {
   return guard(c, [&] {
     if (cfg < -1 || cfg >= kNumGemmShapes) throw Error(DMRJ_ERR_ARG, "gemm cfg must be -1..4");
     c->gemm_cfg_override = cfg;
+  });
+}
+
+int dmrj_ctx_svd_stats(dmrj_ctx* c, double* out2, int reset) {
+  return guard(c, [&] {
+    if (!out2) throw Error(DMRJ_ERR_ARG, "null output");
+    out2[0] = c->bj_calls;
+    out2[1] = c->bj_sweeps;
+    if (reset) c->bj_calls = c->bj_sweeps = 0;
   });
 }
 

@@ -197,6 +197,7 @@ struct Context {
   // eigensolver of the augmented matrix for large ones, 1 = gesdd,
   // 2 = gesvd, 3 = batched Jacobi for small sectors + gesvd
   int svd_backend = 0;
+  double bj_calls = 0, bj_sweeps = 0;  // block-Jacobi SVD calls and sweeps
   int env_prec = 0;  // precision of environment updates
   int last_gemm_cfg = -1;
   // grouped-GEMM autotuner: 1 = time the tile shapes per histogram key
@@ -286,10 +287,11 @@ void launch_aug_place_z(Context& c, const std::vector<AugJobZ>& jobs);
 
 // ---- batched block one-sided Jacobi for large SVD sectors (svd_jacobi.hip) ----
 struct BjJob {
-  double* X;      // rows x ncols working matrix, ld rows, orthogonalised in place
-  double* S;      // ncols column norms (output of launch_bj_norms)
-  int* flag;      // set to 1 by a round that rotated
-  double tiny2;   // columns with |x|^2 below this are numerically zero (rank deficiency)
+  double* X;         // rows x ncols working matrix, ld rows, orthogonalised in place
+  double* S;         // ncols column norms (output of launch_bj_norms)
+  int* flag;         // set to 1 by a round that rotated
+  const int* perm;   // sweep order of the columns: block b holds perm[b bw .. b bw + bw)
+  double atol;       // absolute rotation threshold: eps sqrt(rows) |AA|
   int rows, ncols, nblk, bw;  // nblk even (an empty block pads an odd count)
 };
 struct BjGather {

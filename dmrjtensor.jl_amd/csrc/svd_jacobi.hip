@@ -11,12 +11,16 @@
 // loads its block pair (rows x 2bw) into LDS and orthogonalises those columns
 // among themselves with inner circle rounds (one group of lanes per column
 // pair, 3 dot products and a plane rotation), then writes them back.  Rounds
-// repeat as outer sweeps until a sweep rotates nothing beyond
-// |x_p . x_q| <= tol sqrt(|x_p|^2 |x_q|^2), tol = sqrt(rows) eps (the
-// threshold of LAPACK's dgesvj); columns whose norm fell to the rounding
-// level of the whole tensor (8 eps sqrt(rows) |AA|) are numerically zero and
-// take no further rotations (a sector with more columns than rows has
-// ncols - rank of them).  Column norms are then the singular values
+// repeat as outer sweeps until a sweep rotates nothing.  A pair is left
+// alone when |x_p . x_q| <= tol sqrt(|x_p|^2 |x_q|^2), tol = sqrt(rows) eps
+// (the relative threshold of LAPACK's dgesvj), or when |x_p . x_q| <=
+// atol max(|x_p|, |x_q|) with atol = sqrt(rows) eps |AA|: the coupling is then
+// below the rounding level of the whole two-site tensor, which is the
+// absolute accuracy a bidiagonal SVD delivers; without it, columns at the
+// rounding level (a sector with more columns than rank) would rotate
+// forever.  Before every sweep the host orders the columns by decreasing norm
+// (de Rijk's pivoting: about half the sweeps on DMRG sectors); the order
+// enters as a permutation, the data does not move.  Column norms are then the singular values
 // and the normalised columns the singular vectors of that side; the other
 // factor comes from one grouped GEMM with M (U^T M = D V^T or M V = U D), so
 // W is never accumulated.
@@ -78,7 +82,7 @@ __global__ __launch_bounds__(kBjThreads) void k_bj_step(const BjJob* __restrict_
     // columns 0..np_-1 from block p, np_..nc-1 from block q
     for (int e = tid; e < nc * rows; e += kBjThreads) {
       const int c = e / rows, r = e % rows;
-      const int gc = c < np_ ? p0 + c : q0 + (c - np_);
+      const int gc = jb.perm[c < np_ ? p0 + c : q0 + (c - np_)];
       panel[e] = jb.X[(int64_t)r + (int64_t)gc * rows];
     }
     __syncthreads();
@@ -109,11 +113,12 @@ __global__ __launch_bounds__(kBjThreads) void k_bj_step(const BjJob* __restrict_
             a = bj_group_sum(a, G);
             b = bj_group_sum(b, G);
             g = bj_group_sum(g, G);
-            // a column at the rounding level of the whole tensor is numerically
-            // zero (more columns than rank): leave it, it carries no weight
-            if (g != 0.0 && fabs(g) > tol * sqrt(a * b) && fmin(a, b) > jb.tiny2) {
+            if (g != 0.0 && fabs(g) > tol * sqrt(a * b) && fabs(g) > jb.atol * sqrt(fmax(a, b))) {
               const double zeta = (b - a) / (2.0 * g);
-              const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+              // t = tan of the rotation angle; 1/(2 zeta) where zeta^2 would overflow
+              const double t = fabs(zeta) > 1e150
+                                   ? 0.5 / zeta
+                                   : (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
               const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
               for (int r = gl; r < rows; r += G) {
                 const double u = xp[r], v = xq[r];
@@ -133,7 +138,7 @@ __global__ __launch_bounds__(kBjThreads) void k_bj_step(const BjJob* __restrict_
     }
     for (int e = tid; e < nc * rows; e += kBjThreads) {
       const int c = e / rows, r = e % rows;
-      const int gc = c < np_ ? p0 + c : q0 + (c - np_);
+      const int gc = jb.perm[c < np_ ? p0 + c : q0 + (c - np_)];
       jb.X[(int64_t)r + (int64_t)gc * rows] = panel[e];
     }
     if (any_step && tid == 0) atomicOr(jb.flag, 1);

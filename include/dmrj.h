@@ -105,6 +105,9 @@ int dmrj_ctx_marker(dmrj_ctx* ctx, int id);
  * fall back to the augmented eigensolve.  Complex tensors always use a
  * batched Hermitian eigensolve of [[0,M],[M^H,0]]. */
 int dmrj_ctx_set_svd_backend(dmrj_ctx* ctx, int mode);
+/* out2 = {SVD calls that used the block one-sided Jacobi, total sweeps of
+ * those calls}; reset != 0 clears the counters.  No reference counterpart. */
+int dmrj_ctx_svd_stats(dmrj_ctx* ctx, double* out2, int reset);
 /* precision of dmrj_env_left / dmrj_env_right (DMRJ_PREC_F64 by default) */
 int dmrj_ctx_set_env_precision(dmrj_ctx* ctx, int precision);
 /* |H AA - E AA| of the last lanczos2 / twostep run with DMRJ_PREC_CORRECT */
