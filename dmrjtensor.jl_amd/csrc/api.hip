@@ -1065,7 +1065,7 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
     std::vector<BjJob> jobs;
     std::vector<int> noff;  // offset of every job's columns in the shared norm / permutation arrays
     int64_t ncol_all = 0;
-    const double atol = 2.220446049250313e-16 * std::sqrt(aa2);
+    const double tiny = 8.0 * 2.220446049250313e-16 * std::sqrt(aa2);
     for (int si : order) {
       Sec& s = secs[si];
       if (s.kmin == 0 || is_small[si]) continue;
@@ -1084,7 +1084,7 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
       s.rowsX = rows;
       s.ncolsX = ncols;
       is_small[si] = 1;  // handled
-      jobs.push_back({s.X, nullptr, nullptr, nullptr, atol * std::sqrt((double)rows), rows, ncols, nblk, bw});
+      jobs.push_back({s.X, nullptr, nullptr, nullptr, tiny * tiny * rows, rows, ncols, nblk, bw});
       noff.push_back((int)ncol_all);
       ncol_all += ncols;
       const double mm = (double)std::max(s.R, s.C), nn = (double)std::min(s.R, s.C);

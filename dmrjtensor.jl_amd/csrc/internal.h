@@ -291,7 +291,7 @@ struct BjJob {
   double* S;         // ncols column norms (output of launch_bj_norms)
   int* flag;         // set to 1 by a round that rotated
   const int* perm;   // sweep order of the columns: block b holds perm[b bw .. b bw + bw)
-  double atol;       // absolute rotation threshold: eps sqrt(rows) |AA|
+  double tiny2;      // columns with |x|^2 below this are numerically zero (rank deficiency)
   int rows, ncols, nblk, bw;  // nblk even (an empty block pads an odd count)
 };
 struct BjGather {

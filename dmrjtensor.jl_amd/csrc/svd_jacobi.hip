@@ -11,16 +11,15 @@
 // loads its block pair (rows x 2bw) into LDS and orthogonalises those columns
 // among themselves with inner circle rounds (one group of lanes per column
 // pair, 3 dot products and a plane rotation), then writes them back.  Rounds
-// repeat as outer sweeps until a sweep rotates nothing.  A pair is left
-// alone when |x_p . x_q| <= tol sqrt(|x_p|^2 |x_q|^2), tol = sqrt(rows) eps
-// (the relative threshold of LAPACK's dgesvj), or when |x_p . x_q| <=
-// atol max(|x_p|, |x_q|) with atol = sqrt(rows) eps |AA|: the coupling is then
-// below the rounding level of the whole two-site tensor, which is the
-// absolute accuracy a bidiagonal SVD delivers; without it, columns at the
-// rounding level (a sector with more columns than rank) would rotate
-// forever.  Before every sweep the host orders the columns by decreasing norm
-// (de Rijk's pivoting: about half the sweeps on DMRG sectors); the order
-// enters as a permutation, the data does not move.  Column norms are then the singular values
+// repeat as outer sweeps until a sweep rotates nothing beyond
+// |x_p . x_q| <= tol sqrt(|x_p|^2 |x_q|^2), tol = sqrt(rows) eps (the
+// relative threshold of LAPACK's dgesvj), which keeps every column pair
+// orthogonal to working precision.  Columns whose norm fell to the rounding
+// level of the whole tensor (8 eps sqrt(rows) |AA|) are numerically zero and
+// take no further rotations.  Before every sweep the host orders the columns
+// by decreasing norm (de Rijk's pivoting: about half the sweeps on DMRG
+// sectors); the order enters as a permutation, the data does not move.  A
+// sector that does not converge in 60 sweeps is reported as a solver error.  Column norms are then the singular values
 // and the normalised columns the singular vectors of that side; the other
 // factor comes from one grouped GEMM with M (U^T M = D V^T or M V = U D), so
 // W is never accumulated.
@@ -113,7 +112,7 @@ __global__ __launch_bounds__(kBjThreads) void k_bj_step(const BjJob* __restrict_
             a = bj_group_sum(a, G);
             b = bj_group_sum(b, G);
             g = bj_group_sum(g, G);
-            if (g != 0.0 && fabs(g) > tol * sqrt(a * b) && fabs(g) > jb.atol * sqrt(fmax(a, b))) {
+            if (g != 0.0 && fabs(g) > tol * sqrt(a * b) && fmin(a, b) > jb.tiny2) {
               const double zeta = (b - a) / (2.0 * g);
               // t = tan of the rotation angle; 1/(2 zeta) where zeta^2 would overflow
               const double t = fabs(zeta) > 1e150

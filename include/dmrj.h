@@ -102,7 +102,9 @@ int dmrj_ctx_marker(dmrj_ctx* ctx, int id);
  * gesvd, 4 = batched Jacobi for small sectors + batched block one-sided
  * Jacobi (columns of M or M^T in LDS-resident block pairs) for the larger
  * ones, the other factor by a grouped GEMM with M; sectors too tall for LDS
- * fall back to the augmented eigensolve.  Complex tensors always use a
+ * fall back to the augmented eigensolve (experimental: slower than 0 on
+ * DMRG sectors, and a strongly rank-deficient sector can exhaust its sweep
+ * limit, reported as DMRJ_ERR_SOLVER).  Complex tensors always use a
  * batched Hermitian eigensolve of [[0,M],[M^H,0]]. */
 int dmrj_ctx_set_svd_backend(dmrj_ctx* ctx, int mode);
 /* out2 = {SVD calls that used the block one-sided Jacobi, total sweeps of
