@@ -68,6 +68,11 @@ def main():
     ap.add_argument("--no-xcd-order", action="store_true", help="plain descending-work tile order")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_k_gemm_f64.json"),
                     help="HBM bytes per k_gemm_f64 launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE")
+    ap.add_argument("--mode", default="dmrg", choices=["dmrg", "tdvp"],
+                    help="dmrg: two-site DMRG sweeps (default); tdvp: real-time two-site TDVP half-sweeps of the "
+                         "Heisenberg chain from the Neel state (ComplexF64)")
+    ap.add_argument("--dt", type=float, default=0.05, help="TDVP time step per half-sweep")
+    ap.add_argument("--krylov", type=int, default=12, help="TDVP Krylov dimension")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run one block-copy kernel over this many doubles (counter calibration)")
     args = ap.parse_args()
@@ -85,6 +90,8 @@ def main():
         log("copy kernel: %d doubles read and written (%.1f MiB each way)"
             % (args.calibrate_copy, args.calibrate_copy * 8 / 2 ** 20))
         return
+    if args.mode == "tdvp":
+        return bench_tdvp(pkg, ctx, args)
     if args.no_autotune:
         ctx.set_autotune(0)
     if args.no_xcd_order:
@@ -232,6 +239,56 @@ def main():
                      "hits_in_timed_region": tune1["hits"] - tune0["hits"]},
         "graph": ctx.graph_stats() if args.graph else None,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_tdvp(pkg, ctx, args):
+    """Real-time two-site TDVP (algs/time.jl:288-362): Heisenberg chain from
+    the Neel state, U(1), prefactor -i dt per half-sweep, Krylov dimension
+    --krylov, cutoff 1e-12, bond dimension capped at m.  --warmup half-sweeps
+    untimed, then --steps timed half-sweeps."""
+    L = args.L or 100
+    m = args.m or 1024
+    qphys = np.array([[1], [-1]])
+    mpo = pkg.MPO(pkg.makeMPO(pkg.heisenbergMPO(), 2, L).H, qphys=qphys)
+    psi = pkg.product_state([i % 2 for i in range(L)], 2, qphys=qphys, ctx=ctx)
+    Le = Re = None
+    for _ in range(args.warmup):
+        _, Le, Re = pkg.tdvp_twosite(psi, mpo, m=m, cutoff=1e-12, r=args.krylov, prefactor=-1j * args.dt,
+                                     Lenv=Le, Renv=Re)
+    ctx.profile(True)
+    ctx.sync()
+    ctx.flops(reset=True)
+    for k in ("gemm_f64", "mix", "copy", "svd"):
+        ctx.kernel_stats(k, reset=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _, Le, Re = pkg.tdvp_twosite(psi, mpo, m=m, cutoff=1e-12, r=args.krylov, prefactor=-1j * args.dt,
+                                     Lenv=Le, Renv=Re)
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    fl_heff, fl_env = ctx.flops()
+    stats = {k: ctx.kernel_stats(k) for k in ("gemm_f64", "mix", "copy", "svd")}
+    ctx.profile(False)
+    chi = max(psi.bond_dims())
+    sz = pkg.local_expect(psi, np.diag([0.5, -0.5]), L // 2)
+    g = stats["gemm_f64"]
+    g_tflops = g["flops"] / (g["ms"] * 1e-3) / 1e12 if g["ms"] > 0 else 0.0
+    log("tdvp: %d half-sweeps after %d, t=%.3f, max bond %d, <Sz_%d>=%.12f, %.3f s per half-sweep, "
+        "gemm %.2f TFLOP/s, svd %.3f s" % (args.steps, args.warmup, args.dt * (args.warmup + args.steps), chi,
+                                          L // 2, sz, elapsed / args.steps, g_tflops, stats["svd"]["ms"] / 1e3))
+    out = {
+        "metric": "real-time two-site TDVP half-sweep (ComplexF64, U(1))",
+        "value": round((fl_heff + fl_env) / elapsed / 1e12, 4), "unit": "TFLOP/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c128 (split planes, FP64 arithmetic)", "data": "synthetic (Neel state)",
+        "config": {"workload": "Heisenberg L=%d Neel start, real-time TDVP dt=%g Krylov %d, m<=%d, cutoff 1e-12"
+                               % (L, args.dt, args.krylov, m), "max_bond": chi, "sz_centre": sz},
+        "roofline": {"bound": "mfma", "kernel": "k_gemm_f64 (complexified plans)", "achieved": round(g_tflops, 3),
+                     "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": round(g_tflops / PEAK_FP64, 4), "traffic": None},
+        "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
 

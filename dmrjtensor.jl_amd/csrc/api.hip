@@ -953,11 +953,16 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
     int64_t R = 0, C = 0;
     double *M = nullptr, *U = nullptr, *Vt = nullptr, *s = nullptr, *E = nullptr;
     int kmin = 0, keep = 0;
-    // block-Jacobi sectors: working matrix, its column norms, sort order
+    // sectors solved for one orthonormal factor (block Jacobi, backend 4, or
+    // Gram eigensolve, backend 5); the other factor comes from a GEMM with M.
+    // colsrc / coldiv: source column (rowsX elements) and divisor of the k-th
+    // largest singular vector of that side
     double* X = nullptr;
     int bj = 0, rowsX = 0, ncolsX = 0;
     std::vector<int> perm;
     std::vector<double> hn;
+    std::vector<const double*> colsrc;
+    std::vector<double> coldiv;
   };
   std::map<Charge, int> qidx;
   std::vector<Sec> secs;
@@ -1167,6 +1172,60 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
       }
     }
   }
+  // large sectors, backend 5: batched symmetric eigensolve of the Gram matrix
+  // of the side that must come out orthonormal (M M^T for dir > 0, M^T M for
+  // dir < 0: size n = R or C instead of R + C); eigenvalues sigma^2 carry an
+  // absolute error ~eps |M|^2; the other factor by the grouped GEMM below
+  double* gramZ = nullptr;
+  double* gramW = nullptr;
+  double* gramE = nullptr;
+  rocblas_int* graminfo = nullptr;
+  int ngram = 0, gramN = 0;
+  std::vector<int> gramsec;
+  if (!z && c.svd_backend == 5) {
+    for (int si : order) {
+      Sec& s = secs[si];
+      if (s.kmin == 0 || is_small[si]) continue;
+      gramsec.push_back(si);
+      gramN = std::max<int>(gramN, (int)(dir > 0 ? s.R : s.C));
+    }
+    if (!gramsec.empty()) {
+      ngram = (int)gramsec.size();
+      const int64_t NN = (int64_t)gramN * gramN;
+      gramZ = (double*)c.alloc.alloc(NN * ngram * sizeof(double));
+      gramW = (double*)c.alloc.alloc((int64_t)gramN * ngram * sizeof(double));
+      gramE = (double*)c.alloc.alloc((int64_t)gramN * ngram * sizeof(double));
+      graminfo = (rocblas_int*)c.alloc.alloc(sizeof(rocblas_int) * ngram);
+      launch_fill(c, gramZ, NN * ngram, 0.0);
+      std::vector<GemmTask> tasks;
+      std::vector<GemmSeg> segs;
+      std::vector<int2> padjobs;
+      for (int b = 0; b < ngram; ++b) {
+        Sec& s = secs[gramsec[b]];
+        const int n = (int)(dir > 0 ? s.R : s.C);
+        double* Zb = gramZ + b * NN;
+        tasks.push_back({Zb, gramN, n, n, (int)segs.size(), 1, 0.0});
+        if (dir > 0)
+          segs.push_back({s.M, s.M, s.R, s.R, (int)s.C, 2, 1.0});  // M M^T
+        else
+          segs.push_back({s.M, s.M, s.R, s.R, (int)s.R, 1, 1.0});  // M^T M
+        padjobs.push_back(make_int2(b, n));
+        s.bj = 1;
+        s.rowsX = n;
+        is_small[gramsec[b]] = 1;  // handled
+        const double nn = (double)n, kk = (double)(dir > 0 ? s.C : s.R);
+        svd_flops += 2.0 * nn * nn * kk + 9.0 * nn * nn * nn;
+      }
+      launch_gemm(c, tasks, segs, -1);
+      // padding rows / columns: -1 on the diagonal keeps their eigenvalues
+      // below every sigma^2 >= 0 and their eigenvectors outside the sector
+      launch_pad_diag(c, gramZ, gramN, padjobs, -1.0);
+      rocblas_status st = rocsolver_dsyevd_strided_batched(hb, rocblas_evect_original, rocblas_fill_upper, gramN,
+                                                           gramZ, gramN, NN, gramW, gramN, gramE, gramN, graminfo,
+                                                           ngram);
+      if (st != rocblas_status_success) throw Error(DMRJ_ERR_SOLVER, "rocsolver_dsyevd_strided_batched failed");
+    }
+  }
   // large sectors, default backend: one batched symmetric eigensolve of the
   // augmented matrices [[0, M], [M^T, 0]] (eigenvalues +-sigma, absolute
   // accuracy eps |M| like a bidiagonal SVD, no squaring of M)
@@ -1280,6 +1339,13 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
       o += s.kmin;
     }
   }
+  std::vector<double> hgram((size_t)std::max(1, ngram * gramN));
+  std::vector<rocblas_int> hginfo(std::max(1, ngram));
+  if (ngram) {
+    HIP_OK(hipMemcpyAsync(hgram.data(), gramW, (size_t)ngram * gramN * sizeof(double), hipMemcpyDeviceToHost,
+                          c.stream));
+    HIP_OK(hipMemcpyAsync(hginfo.data(), graminfo, ngram * sizeof(rocblas_int), hipMemcpyDeviceToHost, c.stream));
+  }
   std::vector<rocblas_int> hinfo(std::max(1, nsvd));
   if (nsvd) HIP_OK(hipMemcpyAsync(hinfo.data(), info, nsvd * sizeof(rocblas_int), hipMemcpyDeviceToHost, c.stream));
   std::vector<rocblas_int> hainfo(std::max(1, naug));
@@ -1296,18 +1362,39 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
   c.stage_reset();
   for (int i = 0; i < nsvd; ++i)
     if (hinfo[i] != 0) throw Error(DMRJ_ERR_SOLVER, "rocsolver SVD did not converge");
+  for (int i = 0; i < ngram; ++i)
+    if (hginfo[i] != 0) throw Error(DMRJ_ERR_SOLVER, "rocsolver_dsyevd (Gram) did not converge");
   {
-    // block-Jacobi sectors: column norms sorted descending (stable), the
-    // largest kmin are the singular values
+    // one-factor sectors: singular values descending and the column of the
+    // orthonormal factor belonging to each
+    std::map<int, int> gram_of;
+    for (int b = 0; b < ngram; ++b) gram_of[gramsec[b]] = b;
     int64_t o = 0;
     for (int si : order) {
       Sec& s = secs[si];
       if (s.kmin == 0) continue;
-      if (s.bj) {
+      auto g = gram_of.find(si);
+      if (g != gram_of.end()) {
+        // Gram sector: eigenvalues ascending, sigma_k = sqrt(lambda_{N-1-k}), unit eigenvectors
+        const int b = g->second;
+        const int64_t NN = (int64_t)gramN * gramN;
+        for (int l = 0; l < s.kmin; ++l) {
+          const double lam = hgram[(size_t)b * gramN + gramN - 1 - l];
+          hs[o + l] = std::sqrt(std::max(lam, 0.0));
+          s.colsrc.push_back(gramZ + b * NN + (int64_t)(gramN - 1 - l) * gramN);
+          s.coldiv.push_back(1.0);
+        }
+      } else if (s.bj) {
+        // block-Jacobi sector: column norms sorted descending (stable), the
+        // largest kmin are the singular values
         s.perm.resize(s.ncolsX);
         std::iota(s.perm.begin(), s.perm.end(), 0);
         std::stable_sort(s.perm.begin(), s.perm.end(), [&](int a, int b) { return s.hn[a] > s.hn[b]; });
-        for (int l = 0; l < s.kmin; ++l) hs[o + l] = s.hn[s.perm[l]];
+        for (int l = 0; l < s.kmin; ++l) {
+          hs[o + l] = s.hn[s.perm[l]];
+          s.colsrc.push_back(s.X + (int64_t)s.perm[l] * s.rowsX);
+          s.coldiv.push_back(hs[o + l]);
+        }
       }
       o += s.kmin;
     }
@@ -1385,11 +1472,11 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
       Sec& s = secs[si];
       if (s.bj && s.keep > 0) {
         for (int k = 0; k < s.keep; ++k) {
-          const double* src = s.X + (int64_t)s.perm[k] * s.rowsX;
+          const double* src = s.colsrc[k];
           if (dir > 0)
-            gat.push_back({src, s.U + (int64_t)k * s.R, 1, hs[hoff + k], s.rowsX, 0});
+            gat.push_back({src, s.U + (int64_t)k * s.R, 1, s.coldiv[k], s.rowsX, 0});
           else
-            gat.push_back({src, s.Vt + k, s.kmin, hs[hoff + k], s.rowsX, 1});
+            gat.push_back({src, s.Vt + k, s.kmin, s.coldiv[k], s.rowsX, 1});
         }
         if (dir > 0) {
           tasks.push_back({s.Vt, s.kmin, s.keep, (int)s.C, (int)segs.size(), 1, 0.0});
@@ -1467,6 +1554,10 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
   }
   c.alloc.release(bjflag);
   c.alloc.release(bjnorm);
+  c.alloc.release(gramZ);
+  c.alloc.release(gramW);
+  c.alloc.release(gramE);
+  c.alloc.release(graminfo);
   c.alloc.release(info);
   c.alloc.release(dscal);
   res.L = tl;
@@ -1876,7 +1967,7 @@ int dmrj_selftest_mfma_bf16(dmrj_ctx* c, float* out768) {
 
 int dmrj_ctx_set_svd_backend(dmrj_ctx* c, int mode) {
   return guard(c, [&] {
-    if (mode < 0 || mode > 4) throw Error(DMRJ_ERR_ARG, "svd backend must be 0..4");
+    if (mode < 0 || mode > 5) throw Error(DMRJ_ERR_ARG, "svd backend must be 0..5");
     c->svd_backend = mode;
   });
 }

@@ -307,6 +307,7 @@ void launch_bj_sweep(Context& c, const BjJob* djobs, const std::vector<int2>& wo
 void launch_bj_norms(Context& c, const BjJob* djobs, const std::vector<int2>& work);
 void launch_bj_gather(Context& c, const std::vector<BjGather>& g);
 void launch_transpose(Context& c, const double* M, int R, int C, double* X);
+void launch_pad_diag(Context& c, double* Z, int N, const std::vector<int2>& jobs, double v);
 void launch_aug_extract_z(Context& c, const std::vector<AugJobZ>& jobs);
 
 // ---- kernel launchers (kernels.hip) ----

@@ -206,6 +206,23 @@ void launch_transpose(Context& c, const double* M, int R, int C, double* X) {
   HIP_OK(hipGetLastError());
 }
 
+// diagonal entries d in [n, N) of the N x N matrix b of a strided batch
+__global__ void k_pad_diag(double* __restrict__ Z, int N, const int2* __restrict__ jobs, int njobs, double v) {
+  const int64_t NN = (int64_t)N * N;
+  for (int j = blockIdx.x; j < njobs; j += gridDim.x) {
+    const int2 jb = jobs[j];
+    for (int d = jb.y + threadIdx.x; d < N; d += blockDim.x) Z[jb.x * NN + d + (int64_t)d * N] = v;
+  }
+}
+
+void launch_pad_diag(Context& c, double* Z, int N, const std::vector<int2>& jobs, double v) {
+  if (jobs.empty()) return;
+  auto* dj = (int2*)c.upload(jobs.data(), jobs.size() * sizeof(int2));
+  hipLaunchKernelGGL(k_pad_diag, dim3((unsigned)std::min<size_t>(jobs.size(), 1024)), dim3(kBjThreads), 0, c.stream, Z,
+                     N, dj, (int)jobs.size(), v);
+  HIP_OK(hipGetLastError());
+}
+
 size_t bj_lds_bytes(int rows, int bw) { return (size_t)2 * bw * rows * sizeof(double); }
 
 void launch_bj_sweep(Context& c, const BjJob* djobs, const std::vector<int2>& work, int nsteps, int inner_max,

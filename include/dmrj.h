@@ -104,7 +104,13 @@ int dmrj_ctx_marker(dmrj_ctx* ctx, int id);
  * ones, the other factor by a grouped GEMM with M; sectors too tall for LDS
  * fall back to the augmented eigensolve (experimental: slower than 0 on
  * DMRG sectors, and a strongly rank-deficient sector can exhaust its sweep
- * limit, reported as DMRJ_ERR_SOLVER).  Complex tensors always use a
+ * limit, reported as DMRJ_ERR_SOLVER), 5 = batched Jacobi for small
+ * sectors + one batched symmetric eigensolve of the Gram matrix of the side
+ * that must come out orthonormal (M M^T or M^T M, size min-side instead of
+ * R + C), the other factor by a grouped GEMM with M: faster, exactly
+ * orthonormal factor, singular values with absolute error ~eps |M|^2 (not
+ * the FP64 drop-in's accuracy: small sigma and truncation errors near a
+ * cutoff differ from LAPACK's).  Complex tensors always use a
  * batched Hermitian eigensolve of [[0,M],[M^H,0]]. */
 int dmrj_ctx_set_svd_backend(dmrj_ctx* ctx, int mode);
 /* out2 = {SVD calls that used the block one-sided Jacobi, total sweeps of

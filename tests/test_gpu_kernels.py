@@ -57,7 +57,7 @@ def test_grouped_gemm_f64(pkg, ctx, cfg, tA, tB, M, N, K):
 
 @pytest.mark.parametrize("backend", [0, pytest.param(1, marks=pytest.mark.xfail(
     reason="rocSOLVER gesdd loses the small singular values of graded spectra; kept only as a "
-           "selectable backend, never the default", strict=False)), 2, 3, 4])
+           "selectable backend, never the default", strict=False)), 2, 3, 4, 5])
 @pytest.mark.parametrize("shape", [(24, 40), (64, 50), (128, 128), (90, 200), (400, 360)])
 def test_svd_backends_graded(pkg, ctx, backend, shape):
     """Two-site split of a graded tensor (singular values 1 .. 1e-9): every SVD
@@ -93,6 +93,12 @@ def test_svd_backends_graded(pkg, ctx, backend, shape):
     big = want[:n] > 1e-6
     rel_err = np.max(np.abs(sv[:n][big] - want[:n][big]) / want[:n][big])
     say(" backend", backend, "shape", shape, "abs/sigma_max %.2e" % abs_err, "rel(sigma>1e-6) %.2e" % rel_err)
+    if backend == 5:
+        # Gram eigensolve: sigma^2 to eps |M|^2 absolute, sigma to eps |M|^2 / sigma
+        big = want[:n] > 1e-3
+        assert abs_err < 5e-7
+        assert np.max(np.abs(sv[:n][big] - want[:n][big]) / want[:n][big]) < 1e-9
+        return
     # one-sided Jacobi accumulates O(n eps) rounding over its rotations
     assert abs_err < (1e-13 if backend == 4 else 1e-14)
 
@@ -184,6 +190,51 @@ def test_grouped_gemm_f16x3_scaled(pkg, ctx, cfg, tA, tB, M, N, K, scaleA, scale
     err = np.max(np.abs(C - want) / scale)
     say(" f16x3 cfg", cfg, tA, tB, M, N, K, scaleA, scaleB, "err/|A||B| %.2e" % err)
     assert err < 5e-6
+
+
+@pytest.mark.parametrize("direction", [1, -1])
+def test_svd_gram_split(pkg, ctx, direction):
+    """Backend 5 (Gram eigensolve + GEMM for the other factor), U(1) tensor:
+    the canonical factor is orthonormal to working precision, the product is
+    the best rank-k approximation (renormalised) to 1e-10, the large
+    singular values match LAPACK to 1e-10."""
+    from oracle import dmrg as od
+    rng = np.random.default_rng(29 + direction)
+    qL = np.array([[-2]] * 30 + [[0]] * 130 + [[2]] * 40)
+    qP = np.array([[1], [-1]])
+    qR = np.array([[-2]] * 50 + [[0]] * 120 + [[2]] * 35)
+    mL, mR = len(qL), len(qR)
+    AA = rng.standard_normal((mL, 2, 2, mR))
+    for a in range(mL):
+        for s1 in range(2):
+            for s2 in range(2):
+                AA[a, s1, s2, qL[a, 0] + qP[s1, 0] + qP[s2, 0] != qR[:, 0]] = 0.0
+    AA *= np.logspace(0, -4, mR)[None, None, None, :]
+    AA /= np.linalg.norm(AA)
+    t = pkg.site2_from_dense(ctx, AA, qL, qP, qP, qR, nq=1)
+    ctx.set_svd_backend(5)
+    try:
+        gl, gr, te, sv = _split(pkg, ctx, t, 120, direction)
+    finally:
+        ctx.set_svd_backend(0)
+    U, s, V, terr, _ = od.fsvd(AA, 120, 2, 0.0, qL, qP, qR)
+    ss = np.sort(s)[::-1]
+    big = ss > 1e-3 * ss[0]
+    say(" gram dir", direction, "kept", len(sv), "te %.3e %.3e" % (te, terr))
+    assert len(sv) == len(s)
+    assert np.allclose(sv[big], ss[big], rtol=1e-10)
+    A = gl.to_dense(1)[0]
+    B = gr.to_dense(1)[0]
+    k = A.shape[2]
+    if direction > 0:
+        X = A.reshape(-1, k, order="F")
+        assert np.allclose(X.T @ X, np.eye(k), atol=1e-13)
+    else:
+        X = B.reshape(k, -1, order="F")
+        assert np.allclose(X @ X.T, np.eye(k), atol=1e-13)
+    prod = np.tensordot(A, B, axes=(2, 0))
+    want = np.tensordot(U, s[:, None, None] * V, axes=(2, 0))
+    assert np.max(np.abs(prod - want)) / np.max(np.abs(want)) <= 1e-10
 
 
 def test_mfma_bf16_lane_layout(pkg, ctx):
