@@ -1489,6 +1489,33 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
       hoff += s.kmin;
     }
     launch_bj_gather(c, gat);
+    // canonical factor: orthonormal completion for numerically zero singular
+    // values (sigma <= 1e-10 sigma_max); they carry no weight
+    double smax = 0.0;
+    for (int64_t i = 0; i < n; ++i) smax = std::max(smax, ents[i].v);
+    const double szero = 1e-10 * smax;
+    std::vector<CompleteJob> cj;
+    std::vector<double*> scratch;
+    hoff = 0;
+    for (int si : order) {
+      Sec& s = secs[si];
+      if (s.keep > 0) {
+        int c0 = s.keep;
+        while (c0 > 0 && hs[hoff + c0 - 1] <= szero) --c0;
+        if (c0 < s.keep) {
+          const int nv = (int)(dir > 0 ? s.R : s.C);
+          double* sc = (double*)c.alloc.alloc((int64_t)(2 * nv + s.keep) * sizeof(double));
+          scratch.push_back(sc);
+          if (dir > 0)
+            cj.push_back({s.U, s.R, 1, sc, sc + nv, sc + 2 * nv, nv, c0, s.keep});
+          else
+            cj.push_back({s.Vt, 1, s.kmin, sc, sc + nv, sc + 2 * nv, nv, c0, s.keep});
+        }
+      }
+      hoff += s.kmin;
+    }
+    if (!z) launch_complete_basis(c, cj);
+    for (double* p : scratch) c.alloc.release(p);
     launch_gemm(c, tasks, segs, -1);
   }
   double* dscal = (double*)c.alloc.alloc(std::max<size_t>(1, scal.size()) * sizeof(double));

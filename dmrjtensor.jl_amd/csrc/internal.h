@@ -308,6 +308,15 @@ void launch_bj_norms(Context& c, const BjJob* djobs, const std::vector<int2>& wo
 void launch_bj_gather(Context& c, const std::vector<BjGather>& g);
 void launch_transpose(Context& c, const double* M, int R, int C, double* X);
 void launch_pad_diag(Context& c, double* Z, int N, const std::vector<int2>& jobs, double v);
+struct CompleteJob {
+  double* Q;          // canonical factor: element r of vector c at Q[c vs + r es]
+  int64_t vs, es;
+  double* v;          // scratch, n
+  double* d;          // scratch, n
+  double* coef;       // scratch, keep
+  int n, c0, keep;    // vectors c0 .. keep-1 are replaced
+};
+void launch_complete_basis(Context& c, const std::vector<CompleteJob>& jobs);
 void launch_aug_extract_z(Context& c, const std::vector<AugJobZ>& jobs);
 
 // ---- kernel launchers (kernels.hip) ----

@@ -223,6 +223,114 @@ void launch_pad_diag(Context& c, double* Z, int N, const std::vector<int2>& jobs
   HIP_OK(hipGetLastError());
 }
 
+// Orthonormal completion of a kept canonical factor.  Singular vectors of
+// numerically zero singular values are not determined to working precision
+// by any of the solvers here (the two halves of an augmented eigenvector of
+// the null space, or a Jacobi column at the rounding level, are not
+// orthogonal to the rest), while the reference's LAPACK SVD returns an
+// orthonormal completion.  Vectors c0 .. keep-1 (element r of vector c at
+// Q[c vs + r es]) are replaced, in order, by the unit vector e_j after two
+// classical Gram-Schmidt passes
+// against vectors 0 .. c-1, normalised, where e_j is the unit vector with the
+// largest residual (its squared norm, 1 - sum_p Q_pj^2, is tracked for all j
+// and is at least (n - c) / n).  They carry zero weight, so the state is
+// unchanged; the factor becomes orthonormal.  One work-group per sector;
+// v (n), d (n) and coef (keep) are global scratch.
+__device__ void complete_candidate(const CompleteJob& jb, int c, int j, double* red, double* nrm_sh) {
+  const int n = jb.n;
+  for (int r = threadIdx.x; r < n; r += kBjThreads) jb.v[r] = (r == j) ? 1.0 : 0.0;
+  __syncthreads();
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int p = threadIdx.x; p < c; p += kBjThreads) {
+      double a = 0.0;
+      for (int r = 0; r < n; ++r) a += jb.Q[(int64_t)p * jb.vs + (int64_t)r * jb.es] * jb.v[r];
+      jb.coef[p] = a;
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < n; r += kBjThreads) {
+      double a = jb.v[r];
+      for (int p = 0; p < c; ++p) a -= jb.coef[p] * jb.Q[(int64_t)p * jb.vs + (int64_t)r * jb.es];
+      jb.v[r] = a;
+    }
+    __syncthreads();
+  }
+  double a = 0.0;
+  for (int r = threadIdx.x; r < n; r += kBjThreads) a += jb.v[r] * jb.v[r];
+  for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) *nrm_sh = sqrt(red[0] + red[1] + red[2] + red[3]);
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kBjThreads) void k_complete_basis(const CompleteJob* __restrict__ jobs, int njobs) {
+  __shared__ double red[4];
+  __shared__ double nrm_sh;
+  __shared__ double bv[4];
+  __shared__ int bi[4];
+  __shared__ int best_sh;
+  for (int ji = blockIdx.x; ji < njobs; ji += gridDim.x) {
+    const CompleteJob jb = jobs[ji];
+    const int n = jb.n;
+    // d_j = 1 - sum_p Q_pj^2: squared residual of e_j against the vectors so far
+    for (int j = threadIdx.x; j < n; j += kBjThreads) {
+      double a = 1.0;
+      for (int p = 0; p < jb.c0; ++p) {
+        const double q = jb.Q[(int64_t)p * jb.vs + (int64_t)j * jb.es];
+        a -= q * q;
+      }
+      jb.d[j] = a;
+    }
+    __syncthreads();
+    for (int c = jb.c0; c < jb.keep; ++c) {
+      // candidate: the e_j with the largest residual (>= (n - c) / n squared)
+      double mv = -1.0;
+      int mi = 0;
+      for (int j = threadIdx.x; j < n; j += kBjThreads)
+        if (jb.d[j] > mv) {
+          mv = jb.d[j];
+          mi = j;
+        }
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_down(mv, o, 64);
+        const int oi = __shfl_down(mi, o, 64);
+        if (ov > mv) {
+          mv = ov;
+          mi = oi;
+        }
+      }
+      if ((threadIdx.x & 63) == 0) {
+        bv[threadIdx.x >> 6] = mv;
+        bi[threadIdx.x >> 6] = mi;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        int b = 0;
+        for (int w = 1; w < 4; ++w)
+          if (bv[w] > bv[b]) b = w;
+        best_sh = bi[b];
+      }
+      __syncthreads();
+      complete_candidate(jb, c, best_sh, red, &nrm_sh);
+      const double nrm = nrm_sh;
+      for (int r = threadIdx.x; r < n; r += kBjThreads) {
+        const double q = nrm > 0.0 ? jb.v[r] / nrm : 0.0;
+        jb.Q[(int64_t)c * jb.vs + (int64_t)r * jb.es] = q;
+        jb.d[r] -= q * q;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+void launch_complete_basis(Context& c, const std::vector<CompleteJob>& jobs) {
+  if (jobs.empty()) return;
+  auto* dj = (CompleteJob*)c.upload(jobs.data(), jobs.size() * sizeof(CompleteJob));
+  hipLaunchKernelGGL(k_complete_basis, dim3((unsigned)std::min<size_t>(jobs.size(), 1024)), dim3(kBjThreads), 0,
+                     c.stream, dj, (int)jobs.size());
+  HIP_OK(hipGetLastError());
+}
+
 size_t bj_lds_bytes(int rows, int bw) { return (size_t)2 * bw * rows * sizeof(double); }
 
 void launch_bj_sweep(Context& c, const BjJob* djobs, const std::vector<int2>& work, int nsteps, int inner_max,

@@ -237,6 +237,42 @@ def test_svd_gram_split(pkg, ctx, direction):
     assert np.max(np.abs(prod - want)) / np.max(np.abs(want)) <= 1e-10
 
 
+@pytest.mark.parametrize("backend", [0, 5])
+@pytest.mark.parametrize("direction", [1, -1])
+def test_svd_rank_deficient_orthonormal_completion(pkg, ctx, backend, direction):
+    """A rank-3 two-site tensor split with m = 0 and cutoff 0 keeps all
+    min(R, C) values, most of them zero.  Their singular vectors are not
+    determined by the data; the canonical factor must still be orthonormal
+    (the reference's LAPACK SVD returns an orthonormal completion) and the
+    product must equal the tensor."""
+    rng = np.random.default_rng(41 + backend + direction)
+    mL, mR = 60, 70
+    M = rng.standard_normal((2 * mL, 3)) @ rng.standard_normal((3, 2 * mR))
+    AA = (M / np.linalg.norm(M)).reshape(mL, 2, 2, mR, order="F")
+    t = pkg.site2_from_dense(ctx, AA)
+    ctx.set_svd_backend(backend)
+    try:
+        gl, gr, te, sv = _split(pkg, ctx, t, 0, direction)
+    finally:
+        ctx.set_svd_backend(0)
+    A = gl.to_dense()[0]
+    B = gr.to_dense()[0]
+    k = A.shape[2]
+    say(" rank-3 backend", backend, "dir", direction, "kept", k, "sv[:4]", sv[:4])
+    assert k == 2 * mL
+    if direction > 0:
+        X = A.reshape(-1, k, order="F")
+        err = np.max(np.abs(X.T @ X - np.eye(k)))
+    else:
+        X = B.reshape(k, -1, order="F")
+        err = np.max(np.abs(X @ X.T - np.eye(k)))
+    prod = np.tensordot(A, B, axes=(2, 0))
+    perr = np.max(np.abs(prod - AA)) / np.max(np.abs(AA))
+    say("  orthonormality err %.2e product err %.2e" % (err, perr))
+    assert err < 1e-12
+    assert perr < 1e-12
+
+
 def test_mfma_bf16_lane_layout(pkg, ctx):
     import ctypes as Ct
     out = np.zeros(768, dtype=np.float32)

@@ -62,6 +62,23 @@ def test_heisenberg_L10_reaches_ed(pkg, ctx):
     assert abs(E - (-4.258035207282883)) < 1e-9
 
 
+def test_cutoff_zero_from_product_state_is_variational(pkg, ctx):
+    """Cutoff 0 from the Neel product state keeps numerically zero singular
+    values (their vectors completed orthonormally): energies decrease sweep
+    by sweep and reach the L=10 exact ground state (test/MPtask_test.jl:323)."""
+    say("case cutoff 0 product start L=10")
+    L = 10
+    qphys = np.array([[1], [-1]])
+    qmpo = pkg.MPO(pkg.makeMPO(pkg.heisenbergMPO(), 2, L).H, qphys=qphys)
+    psi = pkg.product_state([i % 2 for i in range(L)], 2, qphys=qphys, ctx=ctx)
+    params = pkg.algvars(psi)
+    pkg.dmrg(psi, qmpo, m=32, sweeps=6, cutoff=0.0, params=params)
+    es = [h["energy"] for h in params.history]
+    say(" energies", ["%.12f" % e for e in es], "m", [h["biggestm"] for h in params.history])
+    assert all(b <= a + 1e-12 for a, b in zip(es, es[1:]))
+    assert abs(es[-1] - (-4.258035207282883)) < 1e-9
+
+
 def test_u1_heisenberg_L24_vs_oracle(pkg, ctx):
     """U(1) Sz blocks, Neel start; per-sweep energies vs the QN oracle and
     dense-vs-QN agreement (test/dmrg_test.jl:82-85)."""
@@ -175,3 +192,27 @@ def test_graph_captured_lanczos_matches_direct(pkg, ctx):
             assert a["energy"] == b["energy"]
             assert a["maxtrunc"] == b["maxtrunc"]
             assert list(a["storeD"]) == list(b["storeD"])
+
+
+def test_gram_svd_backend_energy(pkg, ctx):
+    """The fast SVD backend (Gram eigensolve, singular values to eps |M|^2
+    absolute) reaches the same converged ground-state energy as the FP64
+    drop-in backend: U(1) L=32, m=64, cutoff 1e-10, 8 sweeps, |dE/E| <= 1e-10."""
+    L = 32
+    qphys = np.array([[1], [-1]])
+    states = [i % 2 for i in range(L)]
+    qmpo = pkg.MPO(pkg.makeMPO(pkg.heisenbergMPO(), 2, L).H, qphys=qphys)
+    energies = []
+    for backend in (0, 5):
+        say("case svd backend", backend, "U(1) L=32 m=64")
+        ctx.set_svd_backend(backend)
+        try:
+            psi = pkg.product_state(states, 2, qphys=qphys, ctx=ctx)
+            params = pkg.algvars(psi)
+            pkg.dmrg(psi, qmpo, m=64, sweeps=8, cutoff=1e-10, params=params)
+        finally:
+            ctx.set_svd_backend(0)
+        for h in params.history:
+            say(" sweep", h["sweep"], "%.15f" % h["energy"], "m", h["biggestm"], "maxtrunc", h["maxtrunc"])
+        energies.append(params.energy)
+    assert abs(energies[0] - energies[1]) <= 1e-10 * abs(energies[0])
