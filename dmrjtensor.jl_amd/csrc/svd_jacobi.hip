@@ -236,28 +236,55 @@ void launch_pad_diag(Context& c, double* Z, int N, const std::vector<int2>& jobs
 // and is at least (n - c) / n).  They carry zero weight, so the state is
 // unchanged; the factor becomes orthonormal.  One work-group per sector;
 // v (n), d (n) and coef (keep) are global scratch.
+// One candidate: v = e_j, two CGS passes against vectors 0 .. c-1, |v| in
+// *nrm_sh.  Loops are ordered so that consecutive lanes read consecutive
+// addresses for both layouts (es == 1: vectors are columns; vs == 1: rows).
 __device__ void complete_candidate(const CompleteJob& jb, int c, int j, double* red, double* nrm_sh) {
   const int n = jb.n;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int kWaves = kBjThreads / 64;
   for (int r = threadIdx.x; r < n; r += kBjThreads) jb.v[r] = (r == j) ? 1.0 : 0.0;
   __syncthreads();
   for (int pass = 0; pass < 2; ++pass) {
-    for (int p = threadIdx.x; p < c; p += kBjThreads) {
-      double a = 0.0;
-      for (int r = 0; r < n; ++r) a += jb.Q[(int64_t)p * jb.vs + (int64_t)r * jb.es] * jb.v[r];
-      jb.coef[p] = a;
+    // coef[p] = <Q_p, v>
+    if (jb.es == 1) {
+      for (int p = wave; p < c; p += kWaves) {
+        const double* q = jb.Q + (int64_t)p * jb.vs;
+        double a = 0.0;
+        for (int r = lane; r < n; r += 64) a += q[r] * jb.v[r];
+        for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+        if (lane == 0) jb.coef[p] = a;
+      }
+    } else {
+      for (int p = threadIdx.x; p < c; p += kBjThreads) {
+        double a = 0.0;
+        for (int r = 0; r < n; ++r) a += jb.Q[(int64_t)p * jb.vs + (int64_t)r * jb.es] * jb.v[r];
+        jb.coef[p] = a;
+      }
     }
     __syncthreads();
-    for (int r = threadIdx.x; r < n; r += kBjThreads) {
-      double a = jb.v[r];
-      for (int p = 0; p < c; ++p) a -= jb.coef[p] * jb.Q[(int64_t)p * jb.vs + (int64_t)r * jb.es];
-      jb.v[r] = a;
+    // v -= sum_p coef[p] Q_p
+    if (jb.es == 1) {
+      for (int r = threadIdx.x; r < n; r += kBjThreads) {
+        double a = jb.v[r];
+        for (int p = 0; p < c; ++p) a -= jb.coef[p] * jb.Q[(int64_t)p * jb.vs + r];
+        jb.v[r] = a;
+      }
+    } else {
+      for (int r = wave; r < n; r += kWaves) {
+        const double* q = jb.Q + (int64_t)r * jb.es;  // element r of every vector, contiguous over p
+        double a = 0.0;
+        for (int p = lane; p < c; p += 64) a += jb.coef[p] * q[p];
+        for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+        if (lane == 0) jb.v[r] -= a;
+      }
     }
     __syncthreads();
   }
   double a = 0.0;
   for (int r = threadIdx.x; r < n; r += kBjThreads) a += jb.v[r] * jb.v[r];
   for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+  if (lane == 0) red[wave] = a;
   __syncthreads();
   if (threadIdx.x == 0) *nrm_sh = sqrt(red[0] + red[1] + red[2] + red[3]);
   __syncthreads();
