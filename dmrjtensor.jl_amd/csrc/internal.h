@@ -15,6 +15,7 @@
 //     an n_a' x sum_w n_a matrix, ld = n_a').
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstdint>
 #include <map>
@@ -226,6 +227,13 @@ struct Context {
   double prof_ms[8] = {0}, prof_flops[8] = {0}, prof_bytes[8] = {0}, prof_count[8] = {0};
   int prof_begin();                                // returns slot or -1
   void prof_end(int slot, int kind, double flops, double bytes);
+  // Timing of one kernel by its own dispatch: prof_begin_ext() sets ext_a /
+  // ext_b, which the next launch helper passes to hipExtLaunchKernelGGL
+  // (start and stop stamped by that dispatch, as a kernel trace records it);
+  // prof_end_ext() files the record and clears them.
+  hipEvent_t ext_a = nullptr, ext_b = nullptr;
+  int prof_begin_ext();
+  void prof_end_ext(int slot, int kind, double flops, double bytes);
   void prof_resolve();                             // after a stream sync
   void* upload(const void* host, size_t bytes);   // staged async copy, returns device ptr
   void stage_reset();
@@ -311,12 +319,16 @@ void launch_pad_diag(Context& c, double* Z, int N, const std::vector<int2>& jobs
 struct CompleteJob {
   double* Q;          // canonical factor: element r of vector c at Q[c vs + r es]
   int64_t vs, es;
-  double* v;          // scratch, n
-  double* d;          // scratch, n
-  double* coef;       // scratch, keep
   int n, c0, keep;    // vectors c0 .. keep-1 are replaced
+  // scratch, set by complete_basis
+  double* W = nullptr;     // n x 32 candidates
+  double* coef = nullptr;  // keep x 32
+  double* d = nullptr;     // n residuals
+  double* seld = nullptr;  // 32
+  int* cand = nullptr;     // 32
 };
-void launch_complete_basis(Context& c, const std::vector<CompleteJob>& jobs);
+// Orthonormal completion (svd_jacobi.hip); allocates and releases its scratch.
+void complete_basis(Context& c, std::vector<CompleteJob>& jobs);
 void launch_aug_extract_z(Context& c, const std::vector<AugJobZ>& jobs);
 
 // ---- kernel launchers (kernels.hip) ----

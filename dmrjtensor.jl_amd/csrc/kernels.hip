@@ -407,8 +407,12 @@ double gemm_bytes(const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>
 
 template <int BM, int BN>
 static void gemm_f64_kernel_launch(Context& c, const GemmLaunch& g) {
-  hipLaunchKernelGGL((k_gemm_f64<BM, BN>), dim3(g.grid), dim3(kThreads), 0, c.stream, g.tiles, g.ntiles,
-                     g.tasks, g.segs);
+  if (c.ext_a)
+    hipExtLaunchKernelGGL((k_gemm_f64<BM, BN>), dim3(g.grid), dim3(kThreads), 0, c.stream, c.ext_a, c.ext_b, 0,
+                          g.tiles, g.ntiles, g.tasks, g.segs);
+  else
+    hipLaunchKernelGGL((k_gemm_f64<BM, BN>), dim3(g.grid), dim3(kThreads), 0, c.stream, g.tiles, g.ntiles,
+                       g.tasks, g.segs);
 }
 
 static const void* gemm_f64_kernel_ptr(int cfg) {
@@ -475,8 +479,10 @@ __global__ void k_region_marker(int id, double* sink) {
   if (id < 0 && threadIdx.x == 0) sink[0] = (double)id;
 }
 
+// The marker's id is also its work-group count, so kernel traces (which
+// record grid sizes, not arguments) tell the markers apart.
 void launch_marker(Context& c, int id) {
-  hipLaunchKernelGGL(k_region_marker, dim3(1), dim3(64), 0, c.stream, id, c.partial);
+  hipLaunchKernelGGL(k_region_marker, dim3(std::max(1, id)), dim3(64), 0, c.stream, id, c.partial);
   HIP_OK(hipGetLastError());
 }
 

@@ -437,8 +437,12 @@ void launch_mfma_bf16_probe(Context& c, float* out) {
 
 template <int BM, int BN>
 static void gemm_lp_kernel_launch(Context& c, const GemmLaunch& g) {
-  hipLaunchKernelGGL((k_gemm_bf16x3<BM, BN>), dim3(g.grid), dim3(kLpThreads), 0, c.stream, g.tiles, g.ntiles,
-                     g.tasks, g.segs);
+  if (c.ext_a)
+    hipExtLaunchKernelGGL((k_gemm_bf16x3<BM, BN>), dim3(g.grid), dim3(kLpThreads), 0, c.stream, c.ext_a, c.ext_b,
+                          0, g.tiles, g.ntiles, g.tasks, g.segs);
+  else
+    hipLaunchKernelGGL((k_gemm_bf16x3<BM, BN>), dim3(g.grid), dim3(kLpThreads), 0, c.stream, g.tiles, g.ntiles,
+                       g.tasks, g.segs);
 }
 
 static const void* gemm_lp_kernel_ptr(int cfg) {

@@ -229,133 +229,283 @@ void launch_pad_diag(Context& c, double* Z, int N, const std::vector<int2>& jobs
 // the null space, or a Jacobi column at the rounding level, are not
 // orthogonal to the rest), while the reference's LAPACK SVD returns an
 // orthonormal completion.  Vectors c0 .. keep-1 (element r of vector c at
-// Q[c vs + r es]) are replaced, in order, by the unit vector e_j after two
-// classical Gram-Schmidt passes
-// against vectors 0 .. c-1, normalised, where e_j is the unit vector with the
-// largest residual (its squared norm, 1 - sum_p Q_pj^2, is tracked for all j
-// and is at least (n - c) / n).  They carry zero weight, so the state is
-// unchanged; the factor becomes orthonormal.  One work-group per sector;
-// v (n), d (n) and coef (keep) are global scratch.
-// One candidate: v = e_j, two CGS passes against vectors 0 .. c-1, |v| in
-// *nrm_sh.  Loops are ordered so that consecutive lanes read consecutive
-// addresses for both layouts (es == 1: vectors are columns; vs == 1: rows).
-__device__ void complete_candidate(const CompleteJob& jb, int c, int j, double* red, double* nrm_sh) {
+// Q[c vs + r es]) are replaced by an orthonormal basis of part of the
+// orthogonal complement of vectors 0 .. c0-1.  They carry zero weight, so the
+// state is unchanged; the factor becomes orthonormal.
+//
+// Blocked greedy construction, kCB vectors at a time:
+//   * candidates are the unit vectors e_j with the largest residuals
+//     d_j = 1 - sum_p Q_pj^2 against the vectors so far (tracked for all j;
+//     the largest is at least (n - c) / n);
+//   * block classical Gram-Schmidt, twice, against the vectors so far:
+//     W -= Q (Q^T W), four grouped-GEMM launches over all sectors of a bond;
+//   * within the block, modified Gram-Schmidt (twice) over the candidates in
+//     order of decreasing d; a candidate whose residual fell below 1e-4, or
+//     below a tenth of its d, is dropped;
+//   * dropped places are filled one vector at a time by the exact greedy step
+//     (e_j of largest d, two classical Gram-Schmidt passes against all vectors
+//     so far), so every block completes exactly its vectors.
+// One work-group per sector in k_complete_block; per sector scratch W (n x
+// kCB), coef (keep x kCB), d (n), seld (kCB) and cand (kCB ints).
+constexpr int kCB = 32;
+
+__device__ double cb_block_sum(double a, double* red) {
+  for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+  __syncthreads();
+  const double t = red[0] + red[1] + red[2] + red[3];
+  __syncthreads();
+  return t;
+}
+
+// index of the largest d_j over j not in excl[0 .. nex); ties go to the
+// lowest index.  Requires n > nex.
+__device__ int cb_argmax(const double* d, int n, const int* excl, int nex, double* bv, int* bi) {
+  double mv = -2.0;
+  int mi = n;
+  for (int j = threadIdx.x; j < n; j += kBjThreads) {
+    bool skip = false;
+    for (int t = 0; t < nex; ++t) skip |= excl[t] == j;
+    if (!skip && d[j] > mv) {
+      mv = d[j];
+      mi = j;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ov = __shfl_down(mv, o, 64);
+    const int oi = __shfl_down(mi, o, 64);
+    if (ov > mv || (ov == mv && oi < mi)) {
+      mv = ov;
+      mi = oi;
+    }
+  }
+  if ((threadIdx.x & 63) == 0) {
+    bv[threadIdx.x >> 6] = mv;
+    bi[threadIdx.x >> 6] = mi;
+  }
+  __syncthreads();
+  double best_v = bv[0];
+  int best = bi[0];
+  for (int w = 1; w < 4; ++w)
+    if (bv[w] > best_v || (bv[w] == best_v && bi[w] < best)) {
+      best_v = bv[w];
+      best = bi[w];
+    }
+  __syncthreads();
+  return best;
+}
+
+// Exact greedy step: v = e_j after two classical Gram-Schmidt passes against
+// vectors 0 .. c-1 of Q; returns |v|.  Loops are ordered so that consecutive
+// lanes read consecutive addresses for both layouts (es == 1: vectors are
+// columns; vs == 1: rows).
+__device__ double complete_candidate(const CompleteJob& jb, double* v, double* coef, int c, int j, double* red) {
   const int n = jb.n;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   constexpr int kWaves = kBjThreads / 64;
-  for (int r = threadIdx.x; r < n; r += kBjThreads) jb.v[r] = (r == j) ? 1.0 : 0.0;
+  for (int r = threadIdx.x; r < n; r += kBjThreads) v[r] = (r == j) ? 1.0 : 0.0;
   __syncthreads();
   for (int pass = 0; pass < 2; ++pass) {
-    // coef[p] = <Q_p, v>
     if (jb.es == 1) {
       for (int p = wave; p < c; p += kWaves) {
         const double* q = jb.Q + (int64_t)p * jb.vs;
         double a = 0.0;
-        for (int r = lane; r < n; r += 64) a += q[r] * jb.v[r];
+        for (int r = lane; r < n; r += 64) a += q[r] * v[r];
         for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
-        if (lane == 0) jb.coef[p] = a;
+        if (lane == 0) coef[p] = a;
       }
     } else {
       for (int p = threadIdx.x; p < c; p += kBjThreads) {
         double a = 0.0;
-        for (int r = 0; r < n; ++r) a += jb.Q[(int64_t)p * jb.vs + (int64_t)r * jb.es] * jb.v[r];
-        jb.coef[p] = a;
+        for (int r = 0; r < n; ++r) a += jb.Q[(int64_t)p * jb.vs + (int64_t)r * jb.es] * v[r];
+        coef[p] = a;
       }
     }
     __syncthreads();
-    // v -= sum_p coef[p] Q_p
     if (jb.es == 1) {
       for (int r = threadIdx.x; r < n; r += kBjThreads) {
-        double a = jb.v[r];
-        for (int p = 0; p < c; ++p) a -= jb.coef[p] * jb.Q[(int64_t)p * jb.vs + r];
-        jb.v[r] = a;
+        double a = v[r];
+        for (int p = 0; p < c; ++p) a -= coef[p] * jb.Q[(int64_t)p * jb.vs + r];
+        v[r] = a;
       }
     } else {
       for (int r = wave; r < n; r += kWaves) {
         const double* q = jb.Q + (int64_t)r * jb.es;  // element r of every vector, contiguous over p
         double a = 0.0;
-        for (int p = lane; p < c; p += 64) a += jb.coef[p] * q[p];
+        for (int p = lane; p < c; p += 64) a += coef[p] * q[p];
         for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
-        if (lane == 0) jb.v[r] -= a;
+        if (lane == 0) v[r] -= a;
       }
     }
     __syncthreads();
   }
   double a = 0.0;
-  for (int r = threadIdx.x; r < n; r += kBjThreads) a += jb.v[r] * jb.v[r];
-  for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
-  if (lane == 0) red[wave] = a;
-  __syncthreads();
-  if (threadIdx.x == 0) *nrm_sh = sqrt(red[0] + red[1] + red[2] + red[3]);
+  for (int r = threadIdx.x; r < n; r += kBjThreads) a += v[r] * v[r];
+  return sqrt(cb_block_sum(a, red));
+}
+
+// Q vectors c .. c+s-1 = W columns 0 .. s-1; d_r -= sum of their squares.
+__device__ void cb_store(const CompleteJob& jb, int c, int s) {
+  const int n = jb.n;
+  if (s <= 0) return;
+  if (jb.es == 1) {
+    for (int u = 0; u < s; ++u)
+      for (int r = threadIdx.x; r < n; r += kBjThreads)
+        jb.Q[(int64_t)(c + u) * jb.vs + r] = jb.W[(int64_t)u * n + r];
+  } else {
+    for (int64_t e = threadIdx.x; e < (int64_t)s * n; e += kBjThreads) {
+      const int u = (int)(e % s), r = (int)(e / s);
+      jb.Q[(int64_t)(c + u) * jb.vs + (int64_t)r * jb.es] = jb.W[(int64_t)u * n + r];
+    }
+  }
+  for (int r = threadIdx.x; r < n; r += kBjThreads) {
+    double a = jb.d[r];
+    for (int u = 0; u < s; ++u) {
+      const double q = jb.W[(int64_t)u * n + r];
+      a -= q * q;
+    }
+    jb.d[r] = a;
+  }
   __syncthreads();
 }
 
-__global__ __launch_bounds__(kBjThreads) void k_complete_basis(const CompleteJob* __restrict__ jobs, int njobs) {
+// b < 0: residuals d of all unit vectors and the first candidate block.
+// b >= 0: finish block b (its candidates in W have been projected against
+// vectors 0 .. c0 + kCB b - 1), then set up the candidates of block b + 1.
+__global__ __launch_bounds__(kBjThreads) void k_complete_block(const CompleteJob* __restrict__ jobs, int njobs,
+                                                               int b) {
   __shared__ double red[4];
-  __shared__ double nrm_sh;
   __shared__ double bv[4];
   __shared__ int bi[4];
-  __shared__ int best_sh;
+  __shared__ int cand_sh[kCB];
   for (int ji = blockIdx.x; ji < njobs; ji += gridDim.x) {
     const CompleteJob jb = jobs[ji];
     const int n = jb.n;
-    // d_j = 1 - sum_p Q_pj^2: squared residual of e_j against the vectors so far
-    for (int j = threadIdx.x; j < n; j += kBjThreads) {
-      double a = 1.0;
-      for (int p = 0; p < jb.c0; ++p) {
-        const double q = jb.Q[(int64_t)p * jb.vs + (int64_t)j * jb.es];
-        a -= q * q;
-      }
-      jb.d[j] = a;
-    }
-    __syncthreads();
-    for (int c = jb.c0; c < jb.keep; ++c) {
-      // candidate: the e_j with the largest residual (>= (n - c) / n squared)
-      double mv = -1.0;
-      int mi = 0;
-      for (int j = threadIdx.x; j < n; j += kBjThreads)
-        if (jb.d[j] > mv) {
-          mv = jb.d[j];
-          mi = j;
+    int cnext;
+    if (b < 0) {
+      if (jb.es == 1) {
+        for (int j = threadIdx.x; j < n; j += kBjThreads) {
+          double a = 1.0;
+          for (int p = 0; p < jb.c0; ++p) {
+            const double q = jb.Q[(int64_t)p * jb.vs + j];
+            a -= q * q;
+          }
+          jb.d[j] = a;
         }
-      for (int o = 32; o > 0; o >>= 1) {
-        const double ov = __shfl_down(mv, o, 64);
-        const int oi = __shfl_down(mi, o, 64);
-        if (ov > mv) {
-          mv = ov;
-          mi = oi;
+      } else {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        for (int j = wave; j < n; j += kBjThreads / 64) {
+          const double* q = jb.Q + (int64_t)j * jb.es;  // contiguous over p
+          double a = 0.0;
+          for (int p = lane; p < jb.c0; p += 64) a += q[p] * q[p];
+          for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+          if (lane == 0) jb.d[j] = 1.0 - a;
         }
-      }
-      if ((threadIdx.x & 63) == 0) {
-        bv[threadIdx.x >> 6] = mv;
-        bi[threadIdx.x >> 6] = mi;
       }
       __syncthreads();
+      cnext = jb.c0;
+    } else {
+      const int cb = jb.c0 + kCB * b;
+      if (cb >= jb.keep) continue;
+      const int nb = min(kCB, jb.keep - cb);
+      int s = 0;
+      for (int t = 0; t < nb; ++t) {
+        double* wt = jb.W + (int64_t)t * n;
+        for (int pass = 0; pass < 2; ++pass)
+          for (int u = 0; u < s; ++u) {
+            const double* wu = jb.W + (int64_t)u * n;
+            double a = 0.0;
+            for (int r = threadIdx.x; r < n; r += kBjThreads) a += wu[r] * wt[r];
+            a = cb_block_sum(a, red);
+            for (int r = threadIdx.x; r < n; r += kBjThreads) wt[r] -= a * wu[r];
+            __syncthreads();
+          }
+        double a = 0.0;
+        for (int r = threadIdx.x; r < n; r += kBjThreads) a += wt[r] * wt[r];
+        const double nrm2 = cb_block_sum(a, red);
+        if (nrm2 >= 1e-4 && nrm2 >= 0.1 * jb.seld[t]) {
+          const double inv = 1.0 / sqrt(nrm2);
+          double* ws = jb.W + (int64_t)s * n;
+          for (int r = threadIdx.x; r < n; r += kBjThreads) ws[r] = wt[r] * inv;
+          __syncthreads();
+          ++s;
+        }
+      }
+      cb_store(jb, cb, s);
+      // places left by dropped candidates: exact greedy steps against all
+      // vectors so far (W is free now: column 0 serves as the work vector)
+      for (int col = cb + s; col < cb + nb; ++col) {
+        const int j = cb_argmax(jb.d, n, cand_sh, 0, bv, bi);
+        const double nrm = complete_candidate(jb, jb.W, jb.coef, col, j, red);
+        const double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+        for (int r = threadIdx.x; r < n; r += kBjThreads) jb.W[r] *= inv;
+        __syncthreads();
+        cb_store(jb, col, 1);
+      }
+      cnext = cb + nb;
+    }
+    const int nbn = min(kCB, jb.keep - cnext);
+    if (nbn <= 0) continue;
+    for (int t = 0; t < nbn; ++t) {
+      const int j = cb_argmax(jb.d, n, cand_sh, t, bv, bi);
       if (threadIdx.x == 0) {
-        int b = 0;
-        for (int w = 1; w < 4; ++w)
-          if (bv[w] > bv[b]) b = w;
-        best_sh = bi[b];
-      }
-      __syncthreads();
-      complete_candidate(jb, c, best_sh, red, &nrm_sh);
-      const double nrm = nrm_sh;
-      for (int r = threadIdx.x; r < n; r += kBjThreads) {
-        const double q = nrm > 0.0 ? jb.v[r] / nrm : 0.0;
-        jb.Q[(int64_t)c * jb.vs + (int64_t)r * jb.es] = q;
-        jb.d[r] -= q * q;
+        cand_sh[t] = j;
+        jb.cand[t] = j;
+        jb.seld[t] = jb.d[j];
       }
       __syncthreads();
     }
+    for (int64_t e = threadIdx.x; e < (int64_t)nbn * n; e += kBjThreads) jb.W[e] = 0.0;
+    __syncthreads();
+    if (threadIdx.x < nbn) jb.W[(int64_t)threadIdx.x * n + cand_sh[threadIdx.x]] = 1.0;
+    __syncthreads();
   }
 }
 
-void launch_complete_basis(Context& c, const std::vector<CompleteJob>& jobs) {
+void complete_basis(Context& c, std::vector<CompleteJob>& jobs) {
   if (jobs.empty()) return;
+  std::vector<double*> scratch;
+  int nblk = 0;
+  for (CompleteJob& j : jobs) {
+    if (j.c0 < 0 || j.keep > j.n || j.c0 >= j.keep) throw Error(DMRJ_ERR_STATE, "completion job out of range");
+    const int64_t sz = (int64_t)kCB * j.n + (int64_t)kCB * j.keep + j.n + 2 * kCB;
+    double* p = (double*)c.alloc.alloc(sz * (int64_t)sizeof(double));
+    scratch.push_back(p);
+    j.W = p;
+    j.coef = j.W + (int64_t)kCB * j.n;
+    j.d = j.coef + (int64_t)kCB * j.keep;
+    j.seld = j.d + j.n;
+    j.cand = reinterpret_cast<int*>(j.seld + kCB);
+    nblk = std::max(nblk, (j.keep - j.c0 + kCB - 1) / kCB);
+  }
   auto* dj = (CompleteJob*)c.upload(jobs.data(), jobs.size() * sizeof(CompleteJob));
-  hipLaunchKernelGGL(k_complete_basis, dim3((unsigned)std::min<size_t>(jobs.size(), 1024)), dim3(kBjThreads), 0,
-                     c.stream, dj, (int)jobs.size());
+  const unsigned grid = (unsigned)std::min<size_t>(jobs.size(), 1024);
+  hipLaunchKernelGGL(k_complete_block, dim3(grid), dim3(kBjThreads), 0, c.stream, dj, (int)jobs.size(), -1);
   HIP_OK(hipGetLastError());
+  for (int b = 0; b < nblk; ++b) {
+    // W -= Q (Q^T W) twice; op(Q^T) is cb x n, op(Q) n x cb
+    std::vector<GemmTask> t1, t2;
+    std::vector<GemmSeg> s1, s2;
+    for (const CompleteJob& j : jobs) {
+      const int cb = j.c0 + kCB * b;
+      if (cb >= j.keep || cb == 0) continue;
+      const int nb = std::min(kCB, j.keep - cb);
+      const bool cols = j.es == 1;
+      const int64_t ldq = cols ? j.vs : j.es;
+      t1.push_back({j.coef, j.keep, cb, nb, (int)s1.size(), 1, 0.0});
+      s1.push_back({j.Q, j.W, ldq, j.n, j.n, cols ? 1 : 0, 1.0});
+      t2.push_back({j.W, j.n, j.n, nb, (int)s2.size(), 1, 1.0});
+      s2.push_back({j.Q, j.coef, ldq, j.keep, cb, cols ? 0 : 1, -1.0});
+    }
+    if (!t1.empty())
+      for (int pass = 0; pass < 2; ++pass) {
+        launch_gemm(c, t1, s1, -1);
+        launch_gemm(c, t2, s2, -1);
+      }
+    hipLaunchKernelGGL(k_complete_block, dim3(grid), dim3(kBjThreads), 0, c.stream, dj, (int)jobs.size(), b);
+    HIP_OK(hipGetLastError());
+  }
+  for (double* p : scratch) c.alloc.release(p);
 }
 
 size_t bj_lds_bytes(int rows, int bw) { return (size_t)2 * bw * rows * sizeof(double); }

@@ -1,9 +1,11 @@
 """Summaries of rocprofv3 CSV output for the DMRG benchmark.
 
-The benchmark brackets its timed sweeps with two dispatches of the empty
-``k_region_marker`` kernel; this script cuts kernel traces and counter
-collections to the dispatches between those markers, so that the figures
-describe the same launches as the benchmark's own HIP-event timing.
+The library dispatches the empty ``k_region_marker`` kernel with as many
+work-groups as its id: 1 and 2 bracket the benchmark's timed sweeps, 3 and 4
+bracket each set of autotuner trial launches.  This script cuts kernel traces
+and counter collections to the dispatches between markers 1 and 2, less those
+between a 3 and the next 4, so that the figures describe the same launches as
+the benchmark's own HIP-event timing.
 
     python rocprof_summary.py trace KERNEL_TRACE.csv [--kernel k_gemm_f64]
     python rocprof_summary.py traffic --fetch F.csv --write W.csv \
@@ -37,12 +39,36 @@ def _rows(path):
         return r.fieldnames, list(r)
 
 
-def _region(ids_names):
-    """(first, last) dispatch ids strictly between the first two markers."""
-    marks = sorted(i for i, n in ids_names if MARKER in n)
-    if len(marks) < 2:
+def _marker_id(r, hdr):
+    """Marker id = work-groups of the dispatch (grid size / work-group size)."""
+    g = int(float(r[_col(hdr, "Grid_Size_X", "Grid_Size")]))
+    w = int(float(r[_col(hdr, "Workgroup_Size_X", "Workgroup_Size")]))
+    return g // max(1, w)
+
+
+def _selector(markers):
+    """markers: sorted (dispatch id, marker id).  Returns a predicate on
+    dispatch ids that is true inside the timed region and outside tuning
+    windows, or None when the region markers are missing."""
+    start = next((i for i, m in markers if m == 1), None)
+    if start is None:
         return None
-    return marks[0], marks[1]
+    end = next((i for i, m in markers if m == 2 and i > start), None)
+    if end is None:
+        return None
+    windows, open_at = [], None
+    for i, m in markers:
+        if m == 3 and open_at is None:
+            open_at = i
+        elif m == 4 and open_at is not None:
+            windows.append((open_at, i))
+            open_at = None
+    if open_at is not None:
+        windows.append((open_at, float("inf")))
+
+    def inside(i):
+        return start < i < end and not any(a < i < b for a, b in windows)
+    return inside
 
 
 def trace_summary(path, kernel):
@@ -50,12 +76,13 @@ def trace_summary(path, kernel):
     cn, cs, ce = _col(hdr, "Kernel_Name"), _col(hdr, "Start_Timestamp"), _col(hdr, "End_Timestamp")
     cd = _col(hdr, "Dispatch_Id", "Correlation_Id")
     recs = [(int(r[cd]), r[cn], int(r[ce]) - int(r[cs])) for r in rows]
-    reg = _region([(i, n) for i, n, _ in recs])
+    markers = sorted((int(r[cd]), _marker_id(r, hdr)) for r in rows if MARKER in r[cn])
+    inside = _selector(markers)
     out = {}
     for scope in ("run", "region"):
         agg = {}
         for i, n, d in recs:
-            if scope == "region" and (reg is None or not (reg[0] < i < reg[1])):
+            if scope == "region" and (inside is None or not inside(i)):
                 continue
             short = n.split("(")[0].replace("void ", "").strip()
             a = agg.setdefault(short, [0, 0])
@@ -69,32 +96,36 @@ def trace_summary(path, kernel):
         r, g = out["run"].get(k, [0, 0]), out["region"].get(k, [0, 0])
         print("| %s | %d | %.2f | %d | %.2f | %.3f |" % (k, r[0], r[1] / max(r[0], 1) / 1e3, g[0],
                                                          g[1] / max(g[0], 1) / 1e3, g[1] / 1e6))
+    total = sum(v[1] for v in out["region"].values())
     sel = [v for k, v in out["region"].items() if kernel in k]
     if sel:
         n = sum(v[0] for v in sel)
         t = sum(v[1] for v in sel)
-        print("\n%s in the marked region: %d launches, average %.4f ms" % (kernel, n, t / n / 1e6))
+        print("\n%s in the marked region (autotuner trial launches excluded): %d launches, average %.4f ms, "
+              "total %.3f ms of %.3f ms kernel time" % (kernel, n, t / n / 1e6, t / 1e6, total / 1e6))
 
 
 def _per_dispatch(path, counter):
     hdr, rows = _rows(path)
     cn, cd = _col(hdr, "Kernel_Name"), _col(hdr, "Dispatch_Id", "Correlation_Id")
     cc, cv = _col(hdr, "Counter_Name"), _col(hdr, "Counter_Value")
-    vals, names = {}, {}
+    vals, names, markers = {}, {}, {}
     for r in rows:
         i = int(r[cd])
         names[i] = r[cn]
+        if MARKER in r[cn]:
+            markers[i] = _marker_id(r, hdr)
         if r[cc].startswith(counter):
             vals[i] = vals.get(i, 0.0) + float(r[cv])
-    return vals, names
+    return vals, names, sorted(markers.items())
 
 
 def _kernel_mean(path, counter, kernel, need_region=True):
-    vals, names = _per_dispatch(path, counter)
-    reg = _region(names.items())
-    if need_region and reg is None:
-        raise SystemExit("%s: no pair of %s dispatches" % (path, MARKER))
-    sel = [v for i, v in vals.items() if kernel in names[i] and (not need_region or reg[0] < i < reg[1])]
+    vals, names, markers = _per_dispatch(path, counter)
+    inside = _selector(markers)
+    if need_region and inside is None:
+        raise SystemExit("%s: no %s dispatches with ids 1 and 2" % (path, MARKER))
+    sel = [v for i, v in vals.items() if kernel in names[i] and (not need_region or inside(i))]
     if not sel:
         raise SystemExit("%s: no %s dispatches with %s" % (path, kernel, counter))
     return sum(sel) / len(sel), len(sel)

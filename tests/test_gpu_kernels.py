@@ -239,15 +239,16 @@ def test_svd_gram_split(pkg, ctx, direction):
 
 @pytest.mark.parametrize("backend", [0, 5])
 @pytest.mark.parametrize("direction", [1, -1])
-def test_svd_rank_deficient_orthonormal_completion(pkg, ctx, backend, direction):
-    """A rank-3 two-site tensor split with m = 0 and cutoff 0 keeps all
+@pytest.mark.parametrize("mL,mR,rank", [(60, 70, 3), (50, 50, 45), (300, 280, 5)])
+def test_svd_rank_deficient_orthonormal_completion(pkg, ctx, backend, direction, mL, mR, rank):
+    """A low-rank two-site tensor split with m = 0 and cutoff 0 keeps all
     min(R, C) values, most of them zero.  Their singular vectors are not
     determined by the data; the canonical factor must still be orthonormal
     (the reference's LAPACK SVD returns an orthonormal completion) and the
-    product must equal the tensor."""
-    rng = np.random.default_rng(41 + backend + direction)
-    mL, mR = 60, 70
-    M = rng.standard_normal((2 * mL, 3)) @ rng.standard_normal((3, 2 * mR))
+    product must equal the tensor.  The shapes need one to nineteen blocks of
+    32 completed vectors, with the kept vectors fewer or more than a block."""
+    rng = np.random.default_rng(41 + backend + direction + rank)
+    M = rng.standard_normal((2 * mL, rank)) @ rng.standard_normal((rank, 2 * mR))
     AA = (M / np.linalg.norm(M)).reshape(mL, 2, 2, mR, order="F")
     t = pkg.site2_from_dense(ctx, AA)
     ctx.set_svd_backend(backend)
@@ -258,8 +259,8 @@ def test_svd_rank_deficient_orthonormal_completion(pkg, ctx, backend, direction)
     A = gl.to_dense()[0]
     B = gr.to_dense()[0]
     k = A.shape[2]
-    say(" rank-3 backend", backend, "dir", direction, "kept", k, "sv[:4]", sv[:4])
-    assert k == 2 * mL
+    say(" rank", rank, "shape", (mL, mR), "backend", backend, "dir", direction, "kept", k, "sv[:4]", sv[:4])
+    assert k == 2 * min(mL, mR)
     if direction > 0:
         X = A.reshape(-1, k, order="F")
         err = np.max(np.abs(X.T @ X - np.eye(k)))
