@@ -186,7 +186,7 @@ int Context::prof_begin() {
   if (!prof) return -1;
   while (prof_pool.size() < 2) {
     hipEvent_t e;
-    HIP_OK(hipEventCreate(&e));
+    HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));  // timing only: no cache writeback
     prof_pool.push_back(e);
   }
   ProfRec r;
@@ -205,7 +205,7 @@ int Context::prof_begin_ext() {
   if (!prof) return -1;
   while (prof_pool.size() < 2) {
     hipEvent_t e;
-    HIP_OK(hipEventCreate(&e));
+    HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));  // timing only: no cache writeback
     prof_pool.push_back(e);
   }
   ProfRec r;

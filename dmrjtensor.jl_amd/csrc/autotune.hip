@@ -217,7 +217,7 @@ int autotune_gemm(Context& c, const std::vector<GemmTask>& tasks, const std::vec
   c.tune_misses += 1;
   constexpr int kReps = 2;
   hipEvent_t ev[kNumGemmShapes + 1];
-  for (auto& e : ev) HIP_OK(hipEventCreate(&e));
+  for (auto& e : ev) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));  // timing only
   GemmLaunch g[kNumGemmShapes];
   for (int cfg = 0; cfg < kNumGemmShapes; ++cfg) g[cfg] = prepare_gemm(c, tasks, segs, cfg, arith);
   auto issue = [&](const GemmLaunch& x) { issue_gemm(c, x); };
