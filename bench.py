@@ -73,6 +73,10 @@ def main():
                          "Heisenberg chain from the Neel state (ComplexF64)")
     ap.add_argument("--dt", type=float, default=0.05, help="TDVP time step per half-sweep")
     ap.add_argument("--krylov", type=int, default=12, help="TDVP Krylov dimension")
+    ap.add_argument("--compare", action="store_true",
+                    help="after the ramp, run --steps sweeps from copies of the same state in each variant "
+                         "(FP64 SVD backend 0; FP64 backend 5; BF16x3 and FP16x3 H_eff with FP64 environments, "
+                         "backend 5) and report sweep times, speed-ups and |dE/E| against the FP64 drop-in")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run one block-copy kernel over this many doubles (counter calibration)")
     args = ap.parse_args()
@@ -152,6 +156,8 @@ def main():
         log("warmup m=%d E=%.12f maxm=%d t=%.2fs" % (m, params.energy, params.history[-1]["biggestm"],
                                                      params.history[-1]["time"]))
     log("ramp+warmup %.1fs" % (time.perf_counter() - t_ramp))
+    if args.compare:
+        return bench_compare(pkg, ctx, args, psi, mpo, desc, L, m)
 
     ctx.profile(True)
     ctx.sync()
@@ -239,6 +245,56 @@ def main():
                      "hits_in_timed_region": tune1["hits"] - tune0["hits"]},
         "graph": ctx.graph_stats() if args.graph else None,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_compare(pkg, ctx, args, psi, mpo, desc, L, m):
+    """Variants of the same sweeps from copies of one state: the plain FP64
+    drop-in (SVD backend 0) is the reference for time and energy."""
+    variants = [("f64", None, 0), ("f64", None, 5), ("bf16x3", "f64", 5), ("f16x3", "f64", 5)]
+    rows = []
+    for prec, envp, backend in variants:
+        st = psi.copy()
+        ctx.set_svd_backend(backend)
+        params = pkg.algvars(st)
+        ctx.sync()
+        t0 = time.perf_counter()
+        pkg.dmrg(st, mpo, m=m, sweeps=args.steps, cutoff=args.cutoff, params=params, precision=prec,
+                 env_precision=envp)
+        ctx.sync()
+        elapsed = time.perf_counter() - t0
+        hist = params.history[-args.steps:]
+        rows.append({"precision": prec, "env_precision": envp or prec, "svd_backend": backend,
+                     "sweep_s": [round(h["time"], 4) for h in hist], "energies": [h["energy"] for h in hist],
+                     "storeD": hist[-1]["storeD"], "maxtrunc": hist[-1]["maxtrunc"],
+                     "biggestm": hist[-1]["biggestm"], "wall_s": elapsed})
+        log("variant %-6s env %-6s svd %d: sweeps %s s, E %s" % (prec, envp or prec, backend, rows[-1]["sweep_s"],
+                                                                 ["%.12f" % e for e in rows[-1]["energies"]]))
+        st.free()
+        for env in (params.Lenv, params.Renv):
+            for e in (env.V if env is not None else []):
+                if e is not None:
+                    e.free()
+    ctx.set_svd_backend(0)
+    ref = rows[0]
+    t_ref = sum(ref["sweep_s"]) / len(ref["sweep_s"])
+    for r in rows:
+        t = sum(r["sweep_s"]) / len(r["sweep_s"])
+        r["speedup_vs_fp64_dropin"] = round(t_ref / t, 3)
+        r["max_rel_dE"] = max(abs(a - b) / abs(b) for a, b in zip(r["energies"], ref["energies"]))
+        n = min(len(r["storeD"]), len(ref["storeD"]))
+        r["max_abs_d_spectrum"] = max([abs(a - b) for a, b in zip(r["storeD"][:n], ref["storeD"][:n])] or [0.0])
+    best = rows[2]
+    out = {
+        "metric": "DMRG sweep time, mixed vs FP64 GPU from the same state, and |dE/E| against the FP64 drop-in",
+        "value": best["speedup_vs_fp64_dropin"], "unit": "x (sweep-time ratio, BF16x3 + FP64 env + Gram SVD)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_ref * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64 / bf16x3 / f16x3",
+        "data": "synthetic (seeded random product state)",
+        "config": {"workload": "%s, two-site DMRG sweeps at m=%d from one state, Krylov 2, cutoff %g"
+                               % (desc, m, args.cutoff), "L": L, "m": m},
+        "variants": rows, "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
 

@@ -34,6 +34,12 @@ class DeviceTensor:
         self.ctx.check(self.ctx.lib.dmrj_tensor_info(self.h, d), "dmrj_tensor_info")
         return list(d)
 
+    def copy(self):
+        """Device copy (dmrj_tensor_copy)."""
+        h = C.c_void_p()
+        self.ctx.call("dmrj_tensor_copy", self.h, C.byref(h))
+        return DeviceTensor(self.ctx, h)
+
     @property
     def kind(self):
         return self.info()[0]
@@ -306,6 +312,18 @@ class MPS:
 
     def bond_dims(self):
         return [t.shape[2] for t in self.A]
+
+    def copy(self):
+        """Independent device copy of the state (same centre and charges)."""
+        c = MPS.__new__(MPS)
+        c.ctx, c.qphys, c.nq, c.oc, c.d = self.ctx, self.qphys, self.nq, self.oc, list(self.d)
+        c.A = [t.copy() for t in self.A]
+        return c
+
+    def free(self):
+        for t in self.A:
+            t.free()
+        self.A = []
 
 
 def _sort_bonds(hosts, qbond):

@@ -216,3 +216,27 @@ def test_gram_svd_backend_energy(pkg, ctx):
             say(" sweep", h["sweep"], "%.15f" % h["energy"], "m", h["biggestm"], "maxtrunc", h["maxtrunc"])
         energies.append(params.energy)
     assert abs(energies[0] - energies[1]) <= 1e-10 * abs(energies[0])
+
+
+def test_mps_copy_is_independent(pkg, ctx):
+    """MPS.copy gives device copies: optimising the copy leaves the original
+    state (its tensors and energy) unchanged."""
+    say("case MPS copy")
+    L = 10
+    qphys = np.array([[1], [-1]])
+    mpo = pkg.MPO(pkg.makeMPO(pkg.heisenbergMPO(), 2, L).H, qphys=qphys)
+    psi = pkg.product_state([i % 2 for i in range(L)], 2, qphys=qphys, ctx=ctx)
+    pkg.dmrg(psi, mpo, m=16, sweeps=1, cutoff=0.0)
+    before, qb = psi.to_dense()
+    e0 = pkg.expect(psi, mpo)
+    cp = psi.copy()
+    got, qc = cp.to_dense()
+    for a, b in zip(before, got):
+        assert np.array_equal(a, b)
+    e1 = pkg.dmrg(cp, mpo, m=32, sweeps=2, cutoff=0.0)
+    after, _ = psi.to_dense()
+    for a, b in zip(before, after):
+        assert np.array_equal(a, b)
+    assert pkg.expect(psi, mpo) == e0
+    assert e1 < e0 + 1e-12
+    cp.free()
