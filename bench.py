@@ -77,6 +77,9 @@ def main():
                     help="after the ramp, run --steps sweeps from copies of the same state in each variant "
                          "(FP64 SVD backend 0; FP64 backend 5; BF16x3 and FP16x3 H_eff with FP64 environments, "
                          "backend 5) and report sweep times, speed-ups and |dE/E| against the FP64 drop-in")
+    ap.add_argument("--converge", type=int, default=0,
+                    help="with --compare: first sweep at m (up to this many sweeps) until the energy changes by "
+                         "less than 1e-11 relative, so that the variants start from a converged state")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run one block-copy kernel over this many doubles (counter calibration)")
     args = ap.parse_args()
@@ -157,6 +160,13 @@ def main():
                                                      params.history[-1]["time"]))
     log("ramp+warmup %.1fs" % (time.perf_counter() - t_ramp))
     if args.compare:
+        for k in range(args.converge):
+            e_prev = params.energy
+            sweeps(m, 1)
+            rel = abs(params.energy - e_prev) / abs(params.energy)
+            log("converge m=%d E=%.12f dE/E=%.2e t=%.2fs" % (m, params.energy, rel, params.history[-1]["time"]))
+            if rel < 1e-11:
+                break
         return bench_compare(pkg, ctx, args, psi, mpo, desc, L, m)
 
     ctx.profile(True)

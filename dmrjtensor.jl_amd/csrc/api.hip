@@ -1768,7 +1768,9 @@ dmrj_tensor* krylov_expm(Context& c, const Tensor& L, const Tensor& R, const Ten
   const int64_t ne = zv ? 2 * m.size : m.size;  // stored elements of one vector
   double* V = (double*)c.alloc.alloc((size_t)nv * r * sizeof(double));
   double* w = (double*)c.alloc.alloc((size_t)nv * sizeof(double));
+  double* xin = (double*)c.alloc.alloc((size_t)nv * sizeof(double));  // fixed input of the prepared H_eff
   double* sc = (double*)c.alloc.alloc(256 * sizeof(double));  // alpha[0..63], beta[64..127], work[128..]
+  double* gs = (double*)c.alloc.alloc((size_t)(2 * 64 * 256 + 128) * sizeof(double));  // CGS partials, coefs
   launch_fill(c, sc, 256, 0.0);
   // v0 = x / |x|
   {
@@ -1781,13 +1783,13 @@ dmrj_tensor* krylov_expm(Context& c, const Tensor& L, const Tensor& R, const Ten
   }
   double *T1 = nullptr, *T3 = nullptr;
   const int zmask = zv ? ((L.cplx ? 1 : 0) | 2 | (R.cplx ? 4 : 0)) : 0;
-  HeffPlan p = plan_heff(c, m, V, w, L.env, L.data, R.env, R.data, f, &T1, &T3, zmask);
-  const double* px = V;
+  HIP_OK(hipMemcpyAsync(xin, V, ne * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  HeffPlan p = plan_heff(c, m, xin, w, L.env, L.data, R.env, R.data, f, &T1, &T3, zmask);
+  const HeffLaunch hl = prepare_heff(c, p, prec);  // descriptor tables stay valid until the final sync
   for (int j = 0; j < r; ++j) {
     double* vj = V + (size_t)j * nv;
-    rebase_plan(p, px, vj, nv, w, w, nv);
-    px = vj;
-    run_heff(c, p, prec);
+    if (j > 0) HIP_OK(hipMemcpyAsync(xin, vj, ne * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    issue_heff(c, hl, true);
     // alpha_j = Re <v_j, w> (real for a Hermitian H_eff)
     {
       const double* xs[1] = {vj};
@@ -1795,28 +1797,10 @@ dmrj_tensor* krylov_expm(Context& c, const Tensor& L, const Tensor& R, const Ten
       launch_dots(c, 1, xs, ys, ne, sc + j);
     }
     if (j == r - 1) break;
-    // w -= <v_i, w> v_i for all i <= j: two passes of classical Gram-Schmidt
-    // (this removes alpha_j v_j and beta_{j-1} v_{j-1} as well)
-    for (int i = 0; i <= j; ++i) {
-      const double* vi = V + (size_t)i * nv;
-      for (int pass = 0; pass < 2; ++pass) {
-        if (zv) {
-          const double* xr[1] = {vi};
-          const double* yr[1] = {w};
-          launch_dots(c, 1, xr, yr, ne, sc + 140);              // Re <vi, w>
-          const double* xc[2] = {vi, vi + n};
-          const double* yc[2] = {w + n, w};
-          launch_dots(c, 2, xc, yc, m.size, sc + 141);          // vi_r.w_i, vi_i.w_r
-          launch_caxpy_dev(c, w, vi, m.size, sc + 140);
-        } else {
-          const double* xs[1] = {vi};
-          const double* ys[1] = {w};
-          launch_dots(c, 1, xs, ys, m.size, sc + 128);
-          launch_scalar_op(c, sc, 0, 128, 129);
-          launch_axpy_dev(c, w, vi, m.size, sc + 129);
-        }
-      }
-    }
+    // w -= <v_i, w> v_i for all i <= j: two block passes of classical
+    // Gram-Schmidt (this removes alpha_j v_j and beta_{j-1} v_{j-1} as well)
+    for (int pass = 0; pass < 2; ++pass)
+      launch_cgs_pass(c, w, V, nv, m.size, j + 1, zv ? 1 : 0, gs, gs + 2 * 64 * 256);
     {
       const double* xs[1] = {w};
       launch_dots(c, 1, xs, xs, ne, sc + 130);
@@ -1888,6 +1872,8 @@ dmrj_tensor* krylov_expm(Context& c, const Tensor& L, const Tensor& R, const Ten
   c.alloc.release(T3);
   c.alloc.release(V);
   c.alloc.release(w);
+  c.alloc.release(xin);
+  c.alloc.release(gs);
   c.alloc.release(sc);
   if (energy) *energy = emin;
   if (kdim) *kdim = k;

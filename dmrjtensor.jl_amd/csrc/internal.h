@@ -338,6 +338,11 @@ void launch_mix(Context& c, const std::vector<MixDst>& dsts, const std::vector<M
 void launch_copy(Context& c, const std::vector<CopyOp>& ops);
 void launch_fill(Context& c, double* p, int64_t n, double v);
 void launch_marker(Context& c, int id);
+// One block classical Gram-Schmidt pass: w -= sum_{i<k} <v_i, w> v_i (v_i at
+// V + i nv; complex: split planes of n elements).  partial: 2 k 256 doubles,
+// coef: 2 k doubles.
+void launch_cgs_pass(Context& c, double* w, const double* V, int64_t nv, int64_t n, int k, int cplx,
+                     double* partial, double* coef);
 void launch_dots(Context& c, int npairs, const double* const* xs, const double* const* ys,
                  int64_t n, double* out);
 void launch_axpy_dev(Context& c, double* y, const double* x, int64_t n, const double* coef);

@@ -247,6 +247,92 @@ __global__ __launch_bounds__(kThreads) void k_dot_final(const double* partial, d
   }
 }
 
+// Block classical Gram-Schmidt pass of the Krylov reorthogonalisation:
+// coef_i = <v_i, w> for all i < k in one pair of launches (deterministic
+// two-level reduction), then w -= sum_i coef_i v_i in one launch.  Real:
+// vectors of n elements.  Complex (split planes, stride nv >= 2n): <v, w> =
+// (vr.wr + vi.wi) + i (vr.wi - vi.wr) and w -= c v on both planes.
+__global__ __launch_bounds__(kThreads) void k_vdots_partial(const double* __restrict__ V, int64_t nv,
+                                                            const double* __restrict__ w, int64_t n, int cplx,
+                                                            double* __restrict__ partial) {
+  const double* __restrict__ v = V + (int64_t)blockIdx.y * nv;
+  double re = 0.0, im = 0.0;
+  for (int64_t e = blockIdx.x * (int64_t)kThreads + threadIdx.x; e < n; e += (int64_t)gridDim.x * kThreads) {
+    if (cplx) {
+      const double vr = v[e], vi = v[n + e], wr = w[e], wi = w[n + e];
+      re += vr * wr + vi * wi;
+      im += vr * wi - vi * wr;
+    } else {
+      re += v[e] * w[e];
+    }
+  }
+  __shared__ double red[2][kThreads / 64];
+  re = wave_sum(re);
+  im = wave_sum(im);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = re;
+    red[1][threadIdx.x >> 6] = im;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    for (int i = 0; i < kThreads / 64; ++i) {
+      a += red[0][i];
+      b += red[1][i];
+    }
+    const int64_t o = ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * 2;
+    partial[o] = a;
+    partial[o + 1] = b;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_vdots_final(const double* __restrict__ partial, int nblk,
+                                                          double* __restrict__ coef) {
+  const int i = blockIdx.x;
+  double re = 0.0, im = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += kThreads) {
+    re += partial[((int64_t)i * nblk + b) * 2];
+    im += partial[((int64_t)i * nblk + b) * 2 + 1];
+  }
+  __shared__ double red[2][kThreads / 64];
+  re = wave_sum(re);
+  im = wave_sum(im);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = re;
+    red[1][threadIdx.x >> 6] = im;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    for (int t = 0; t < kThreads / 64; ++t) {
+      a += red[0][t];
+      b += red[1][t];
+    }
+    coef[2 * i] = a;
+    coef[2 * i + 1] = b;
+  }
+}
+
+__global__ void k_cgs_sub(double* __restrict__ w, const double* __restrict__ V, int64_t nv, int64_t n, int k,
+                          int cplx, const double* __restrict__ coef) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    double ar = 0.0, ai = 0.0;
+    for (int i = 0; i < k; ++i) {
+      const double* v = V + (int64_t)i * nv;
+      const double cr = coef[2 * i];
+      if (cplx) {
+        const double ci = coef[2 * i + 1], vr = v[e], vi = v[n + e];
+        ar += cr * vr - ci * vi;
+        ai += cr * vi + ci * vr;
+      } else {
+        ar += cr * v[e];
+      }
+    }
+    w[e] -= ar;
+    if (cplx) w[n + e] -= ai;
+  }
+}
+
 __global__ void k_axpy_dev(double* y, const double* x, int64_t n, const double* coef) {
   const double c = *coef;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -558,6 +644,18 @@ void launch_comb_dev(Context& c, double* out, const double* a, const double* b, 
 
 void launch_scalar_op(Context& c, double* slots, int op, int src, int dst) {
   hipLaunchKernelGGL(k_scalar_op, dim3(1), dim3(64), 0, c.stream, slots, op, src, dst);
+  HIP_OK(hipGetLastError());
+}
+
+void launch_cgs_pass(Context& c, double* w, const double* V, int64_t nv, int64_t n, int k, int cplx,
+                     double* partial, double* coef) {
+  if (n <= 0 || k <= 0) return;
+  const int nblk = (int)std::min<int64_t>(kDotBlocks, std::max<int64_t>(1, (n + kThreads - 1) / kThreads));
+  hipLaunchKernelGGL(k_vdots_partial, dim3(nblk, k), dim3(kThreads), 0, c.stream, V, nv, w, n, cplx, partial);
+  HIP_OK(hipGetLastError());
+  hipLaunchKernelGGL(k_vdots_final, dim3(k), dim3(kThreads), 0, c.stream, partial, nblk, coef);
+  HIP_OK(hipGetLastError());
+  hipLaunchKernelGGL(k_cgs_sub, dim3(grid_for(n)), dim3(kThreads), 0, c.stream, w, V, nv, n, k, cplx, coef);
   HIP_OK(hipGetLastError());
 }
 

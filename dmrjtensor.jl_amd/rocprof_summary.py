@@ -10,9 +10,13 @@ the benchmark's own HIP-event timing.
     python rocprof_summary.py trace KERNEL_TRACE.csv [--kernel k_gemm_f64]
     python rocprof_summary.py traffic --fetch F.csv --write W.csv \
         --cal-fetch CF.csv --cal-write CW.csv --cal-doubles N [--kernel k_gemm_f64] [--json OUT]
+    python rocprof_summary.py mfma COUNTERS.csv [--kernel k_gemm_f64] [--json OUT]
 
 ``trace`` prints per-kernel launch counts and durations (whole run and the
-marked region).  ``traffic`` prints the HBM bytes per launch of one kernel in
+marked region).  ``mfma`` reads a run with ``--pmc MfmaUtil MfmaFlopsF64``
+(rocprofv3's derived metrics: MFMA-busy cycles over GUI-active cycles of all
+SIMDs, and executed FP64 MFMA flops) and prints the duration-weighted MFMA
+utilisation and the executed MFMA flops per launch of one kernel.  ``traffic`` prints the HBM bytes per launch of one kernel in
 the marked region from FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each
 scaled by what the same counter reads for the copy kernel of the calibration
 run, which moves exactly 8*N bytes each way.
@@ -154,6 +158,40 @@ def traffic_summary(a):
             json.dump(d, fo, indent=1)
 
 
+def mfma_summary(path, kernel, json_out=None):
+    hdr, rows = _rows(path)
+    cn, cd = _col(hdr, "Kernel_Name"), _col(hdr, "Dispatch_Id", "Correlation_Id")
+    cc, cv = _col(hdr, "Counter_Name"), _col(hdr, "Counter_Value")
+    cs, ce = _col(hdr, "Start_Timestamp"), _col(hdr, "End_Timestamp")
+    util, flops, dur, names, markers = {}, {}, {}, {}, {}
+    for r in rows:
+        i = int(r[cd])
+        names[i] = r[cn]
+        dur[i] = int(r[ce]) - int(r[cs])
+        if MARKER in r[cn]:
+            markers[i] = _marker_id(r, hdr)
+        if r[cc] == "MfmaUtil":
+            util[i] = util.get(i, 0.0) + float(r[cv])
+        elif r[cc] == "MfmaFlopsF64":
+            flops[i] = flops.get(i, 0.0) + float(r[cv])
+    inside = _selector(sorted(markers.items()))
+    if inside is None:
+        raise SystemExit("%s: no %s dispatches with ids 1 and 2" % (path, MARKER))
+    sel = [i for i in util if kernel in names[i] and inside(i)]
+    if not sel:
+        raise SystemExit("%s: no %s dispatches with MfmaUtil" % (path, kernel))
+    tdur = sum(dur[i] for i in sel)
+    wutil = sum(util[i] * dur[i] for i in sel) / max(1, tdur)
+    fl = sum(flops.get(i, 0.0) for i in sel)
+    print("%s in the marked region: %d launches, MFMA busy %.1f %% of all SIMD cycles (duration-weighted), "
+          "executed FP64 MFMA flops %.4g per launch, %.2f TFLOP/s over the counted kernel time"
+          % (kernel, len(sel), wutil, fl / len(sel), fl / max(1, tdur) / 1e3))
+    if json_out:
+        with open(json_out, "w") as fo:
+            json.dump({"kernel": kernel, "launches": len(sel), "mfma_util_pct": wutil,
+                       "mfma_flops_f64_per_launch": fl / len(sel), "kernel_ns_total": tdur}, fo, indent=1)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -169,9 +207,15 @@ def main(argv=None):
     f.add_argument("--kernel", default="k_gemm_f64")
     f.add_argument("--json")
     f.add_argument("--meta", help="extra JSON fields for the output file")
+    mm = sub.add_parser("mfma")
+    mm.add_argument("csv")
+    mm.add_argument("--kernel", default="k_gemm_f64")
+    mm.add_argument("--json")
     a = ap.parse_args(argv)
     if a.cmd == "trace":
         trace_summary(a.csv, a.kernel)
+    elif a.cmd == "mfma":
+        mfma_summary(a.csv, a.kernel, a.json)
     else:
         traffic_summary(a)
 

@@ -94,3 +94,29 @@ def test_traffic_calibrated(tmp_path):
     assert abs(d["fetch_bytes_per_launch"] - 200.0 * 1024 * 2) < 1e-6
     assert abs(d["write_bytes_per_launch"] - 20.0 * 1024) < 1e-6
     assert d["launches_fetch"] == 2 and d["L"] == 8
+
+
+def test_mfma_duration_weighted(tmp_path):
+    m = _mod()
+    p = str(tmp_path / "mf.csv")
+    with open(p, "w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(["Dispatch_Id", "Grid_Size", "Kernel_Name", "Workgroup_Size", "Counter_Name", "Counter_Value",
+                    "Start_Timestamp", "End_Timestamp"])
+        t = 0
+        util = {3: 40.0, 8: 10.0}
+        for i, (n, wg, d) in enumerate(TRACE, start=1):
+            if n == MARK:
+                w.writerow([i, 64 * wg, n, 64, "MfmaUtil", 0, t, t + d])
+            elif "k_gemm_f64" in n:
+                w.writerow([i, 64 * wg, n, 64, "MfmaUtil", util.get(i, 99.0), t, t + d])
+                w.writerow([i, 64 * wg, n, 64, "MfmaFlopsF64", 1e9, t, t + d])
+            t += d + 10
+    buf = io.StringIO()
+    with redirect_stdout(buf):
+        m.mfma_summary(p, "k_gemm_f64", str(tmp_path / "m.json"))
+    d = json.load(open(str(tmp_path / "m.json")))
+    # region launches 3 (50000 ns, 40 %) and 8 (70000 ns, 10 %)
+    assert d["launches"] == 2
+    assert abs(d["mfma_util_pct"] - (40.0 * 50000 + 10.0 * 70000) / 120000) < 1e-12
+    assert d["mfma_flops_f64_per_launch"] == 1e9
