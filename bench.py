@@ -167,7 +167,7 @@ def main():
             log("converge m=%d E=%.12f dE/E=%.2e t=%.2fs" % (m, params.energy, rel, params.history[-1]["time"]))
             if rel < 1e-11:
                 break
-        return bench_compare(pkg, ctx, args, psi, mpo, desc, L, m)
+        return bench_compare(pkg, ctx, args, psi, mpo, desc, L, m, params)
 
     ctx.profile(True)
     ctx.sync()
@@ -259,9 +259,39 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def bench_compare(pkg, ctx, args, psi, mpo, desc, L, m):
+def centre_step_energies(pkg, ctx, psi, mpo, params):
+    """One Lanczos step (dmrj_lanczos2) at the centre bond of the same state
+    in every arithmetic: the energy deviation of each split form (plain, and
+    with the FP64 Rayleigh-quotient correction) from the FP64 step."""
+    import ctypes as C
+    L = len(psi)
+    c = L // 2
+    j = 1
+    while psi.oc < c:
+        i = psi.oc
+        pkg.twostep(0, j, i, i, i + 1, psi, mpo, params.Lenv, params.Renv, params)
+        params.Lenv[i + 1] = pkg.Lupdate(params.Lenv[i], psi[i], mpo, i)
+        psi.oc += 1
+    out = {}
+    for name in ("f64", "bf16x3", "bf16x3c", "f16x3", "f16x3c"):
+        h, e = C.c_void_p(), C.c_double()
+        ctx.call("dmrj_lanczos2", params.Lenv[c].h, params.Renv[c + 1].h, psi[c].h, psi[c + 1].h,
+                 mpo.device(ctx, c), mpo.device(ctx, c + 1), pkg._lib.PREC[name], C.byref(h), C.byref(e))
+        pkg.DeviceTensor(ctx, h).free()
+        out[name] = e.value
+    ref = out["f64"]
+    res = {k: {"energy": v, "rel_dE": abs(v - ref) / abs(ref)} for k, v in out.items()}
+    log("centre bond %d|%d Lanczos step: " % (c, c + 1) +
+        ", ".join("%s %.3e" % (k, v["rel_dE"]) for k, v in res.items() if k != "f64"))
+    return {"bond": [c, c + 1], "variants": res}
+
+
+def bench_compare(pkg, ctx, args, psi, mpo, desc, L, m, base_params):
     """Variants of the same sweeps from copies of one state: the plain FP64
-    drop-in (SVD backend 0) is the reference for time and energy."""
+    drop-in (SVD backend 0) is the reference for time and energy.  Energies
+    after a few sweeps differ by the variants' different truncation paths
+    unless the state is converged; the centre-bond Lanczos step from one
+    state isolates the arithmetic."""
     variants = [("f64", None, 0), ("f64", None, 5), ("bf16x3", "f64", 5), ("f16x3", "f64", 5)]
     rows = []
     for prec, envp, backend in variants:
@@ -304,7 +334,8 @@ def bench_compare(pkg, ctx, args, psi, mpo, desc, L, m):
         "data": "synthetic (seeded random product state)",
         "config": {"workload": "%s, two-site DMRG sweeps at m=%d from one state, Krylov 2, cutoff %g"
                                % (desc, m, args.cutoff), "L": L, "m": m},
-        "variants": rows, "roofline": None, "cpu_baseline": None,
+        "variants": rows, "centre_step": centre_step_energies(pkg, ctx, psi, mpo, base_params),
+        "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
 
