@@ -64,7 +64,8 @@ int dmrj_ctx_sync(dmrj_ctx* ctx);
 int dmrj_version(void);
 /* memory statistics: bytes live, bytes cached, peak bytes */
 int dmrj_ctx_memstats(dmrj_ctx* ctx, int64_t* out3);
-/* force a GEMM tile shape (0: 32x32, 1: 64x64, 2: 128x128, -1: automatic) */
+/* force a GEMM tile shape (0: 32x32, 1: 64x64, 2: 128x128, 3: 64x128,
+ * 4: 128x64, -1: automatic) */
 int dmrj_ctx_set_gemm_cfg(dmrj_ctx* ctx, int cfg);
 /* Grouped-GEMM tile-shape autotuner (no counterpart in the reference: it
  * replaces the BLAS call's own blocking).  mode 1 (default) = time the five
