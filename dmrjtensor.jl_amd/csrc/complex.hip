@@ -11,9 +11,9 @@
 // from a real state, the MPO coefficients) have no imaginary plane and their
 // imaginary products are dropped.
 //
-// Kernels here: the complex axpy of the Krylov reorthogonalisation and the
-// placement / extraction around the batched Hermitian eigensolve of the
-// augmented matrices [[0, M], [M^H, 0]] used by the complex SVD.
+// Kernels here: the placement / extraction around the batched Hermitian
+// eigensolve of the augmented matrices [[0, M], [M^H, 0]] used by the
+// complex SVD.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -123,23 +123,6 @@ void complexify_mix(std::vector<MixDst>& dsts, std::vector<MixTerm>& terms, cons
 }
 
 // ------------------------------------------------------------------ kernels
-__global__ void k_caxpy_dev(double* __restrict__ w, const double* __restrict__ v, int64_t n,
-                            const double* __restrict__ s) {
-  const double cr = s[0], ci = s[1] - s[2];
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-    const double vr = v[e], vi = v[n + e];
-    w[e] -= cr * vr - ci * vi;
-    w[n + e] -= cr * vi + ci * vr;
-  }
-}
-
-void launch_caxpy_dev(Context& c, double* w, const double* v, int64_t n, const double* s) {
-  if (n <= 0) return;
-  const int grid = (int)std::min<int64_t>(4096, std::max<int64_t>(1, (n + 1023) / 1024));
-  hipLaunchKernelGGL(k_caxpy_dev, dim3(grid), dim3(256), 0, c.stream, w, v, n, s);
-  HIP_OK(hipGetLastError());
-}
-
 // Z (N x N, complex interleaved) receives M in rows [0, R), columns [R, R+C):
 // the upper triangle of [[0, M], [M^H, 0]].  Z is zero-filled beforehand.
 __global__ void k_aug_place_z(const AugJobZ* __restrict__ jobs, const int2* __restrict__ work, int nwork) {

@@ -280,8 +280,6 @@ struct BufMap {
 };
 void complexify_gemm(std::vector<GemmTask>& tasks, std::vector<GemmSeg>& segs, const BufMap& m);
 void complexify_mix(std::vector<MixDst>& dsts, std::vector<MixTerm>& terms, const BufMap& m);
-// w -= c v on complex vectors of n elements per plane, c = s[0] + i (s[1] - s[2])
-void launch_caxpy_dev(Context& c, double* w, const double* v, int64_t n, const double* s);
 struct AugJobZ {
   const double* M;   // R x C, two planes, ld R
   void* Z;           // N x N complex (interleaved), ld N
@@ -350,7 +348,6 @@ void launch_scale_dev(Context& c, double* x, int64_t n, const double* coef);
 void launch_comb_dev(Context& c, double* out, const double* a, const double* b, int64_t n,
                      const double* ca, const double* cb);
 void launch_lz_scalar(Context& c, int stage);
-void launch_scalar_op(Context& c, double* slots, int op, int src, int dst);
 void launch_combine(Context& c, double* out, const double* V, int64_t ld, int64_t n, const double* coef, int k);
 void launch_mfma_probe(Context& c, double* out);
 void launch_mfma_bf16_probe(Context& c, float* out);

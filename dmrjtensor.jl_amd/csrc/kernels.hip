@@ -433,13 +433,6 @@ void launch_mfma_probe(Context& c, double* out) {
   HIP_OK(hipGetLastError());
 }
 
-// Small scalar operations on device slots: dst = -src (op 0), dst = src (op 1)
-__global__ void k_scalar_op(double* s, int op, int src, int dst) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  if (op == 0) s[dst] = -s[src];
-  else if (op == 1) s[dst] = s[src];
-}
-
 // out[i] = sum_j c[j] * V[j*ld + i] (Krylov vector combination)
 __global__ void k_combine(double* out, const double* V, int64_t ld, int64_t n, const double* c, int k) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -639,11 +632,6 @@ void launch_comb_dev(Context& c, double* out, const double* a, const double* b, 
   if (n <= 0) return;
   hipLaunchKernelGGL(k_comb_dev, dim3(grid_for(n)), dim3(kThreads), 0, c.stream, out, a, b, n, ca,
                      cb);
-  HIP_OK(hipGetLastError());
-}
-
-void launch_scalar_op(Context& c, double* slots, int op, int src, int dst) {
-  hipLaunchKernelGGL(k_scalar_op, dim3(1), dim3(64), 0, c.stream, slots, op, src, dst);
   HIP_OK(hipGetLastError());
 }
 
