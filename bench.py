@@ -62,8 +62,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--graph", action="store_true",
-                    help="capture each Lanczos step into a hipGraph (kernels inside graphs are not timed "
-                         "individually, so the roofline then covers the environment-update GEMMs only)")
+                    help="capture each Lanczos step (TDVP: each Krylov exponential) into a hipGraph (kernels "
+                         "inside graphs are not timed individually, so the roofline then covers the "
+                         "environment-update GEMMs only)")
     ap.add_argument("--no-autotune", action="store_true", help="fixed tile-shape heuristic instead of the autotuner")
     ap.add_argument("--no-xcd-order", action="store_true", help="plain descending-work tile order")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_k_gemm_f64.json"),
@@ -97,14 +98,14 @@ def main():
         log("copy kernel: %d doubles read and written (%.1f MiB each way)"
             % (args.calibrate_copy, args.calibrate_copy * 8 / 2 ** 20))
         return
-    if args.mode == "tdvp":
-        return bench_tdvp(pkg, ctx, args)
     if args.no_autotune:
         ctx.set_autotune(0)
     if args.no_xcd_order:
         ctx.set_tile_order(0)
     if args.graph:
         ctx.set_graph(1)
+    if args.mode == "tdvp":
+        return bench_tdvp(pkg, ctx, args)
     if args.svd_backend is not None:
         ctx.set_svd_backend(args.svd_backend)
     defaults = {"heisenberg": (100, 1024, "128,256,512"), "hubbard": (64, 2048, "256,512,1024"),
@@ -383,6 +384,7 @@ def bench_tdvp(pkg, ctx, args):
         "vs_baseline": None, "dtype": "c128 (split planes, FP64 arithmetic)", "data": "synthetic (Neel state)",
         "config": {"workload": "Heisenberg L=%d Neel start, real-time TDVP dt=%g Krylov %d, m<=%d, cutoff 1e-12"
                                % (L, args.dt, args.krylov, m), "max_bond": chi, "sz_centre": sz},
+        "graph": ctx.graph_stats() if args.graph else None,
         "roofline": {"bound": "mfma", "kernel": "k_gemm_f64 (complexified plans)", "achieved": round(g_tflops, 3),
                      "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": round(g_tflops / PEAK_FP64, 4), "traffic": None},
         "cpu_baseline": None,

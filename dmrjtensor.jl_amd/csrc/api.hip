@@ -618,7 +618,7 @@ void run_heff(Context& c, const HeffPlan& p, int prec = DMRJ_PREC_F64) {
 // executable graph of the previous call is updated in place when the
 // topology matches (same kernels, new arguments), else re-instantiated.
 template <class F>
-void run_captured(Context& c, F&& body, double flops) {
+void run_captured(Context& c, F&& body, double flops, int slot = 0) {
   HIP_OK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
   try {
     body();
@@ -630,35 +630,35 @@ void run_captured(Context& c, F&& body, double flops) {
   hipGraph_t g = nullptr;
   HIP_OK(hipStreamEndCapture(c.stream, &g));
   bool ok = false;
-  if (c.graph_exec) {
+  if (c.graph_exec[slot]) {
     // the previous launch of this executable graph must have finished before
     // its kernel arguments are rewritten
-    HIP_OK(hipEventSynchronize(c.graph_done));
+    HIP_OK(hipEventSynchronize(c.graph_done[slot]));
     hipGraphNode_t err_node = nullptr;
     hipGraphExecUpdateResult res;
-    ok = hipGraphExecUpdate(c.graph_exec, g, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+    ok = hipGraphExecUpdate(c.graph_exec[slot], g, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
     if (!ok) {
       (void)hipGetLastError();
-      hipGraphExecDestroy(c.graph_exec);
-      c.graph_exec = nullptr;
+      hipGraphExecDestroy(c.graph_exec[slot]);
+      c.graph_exec[slot] = nullptr;
     } else {
       c.graph_updates += 1;
     }
   }
   if (!ok) {
-    hipError_t e = hipGraphInstantiate(&c.graph_exec, g, nullptr, nullptr, 0);
+    hipError_t e = hipGraphInstantiate(&c.graph_exec[slot], g, nullptr, nullptr, 0);
     if (e != hipSuccess) {
       hipGraphDestroy(g);
-      c.graph_exec = nullptr;
+      c.graph_exec[slot] = nullptr;
       throw Error(DMRJ_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     }
     c.graph_instantiations += 1;
   }
   const int ps = c.prof_begin();
-  hipError_t e = hipGraphLaunch(c.graph_exec, c.stream);
+  hipError_t e = hipGraphLaunch(c.graph_exec[slot], c.stream);
   hipGraphDestroy(g);
   if (e != hipSuccess) throw Error(DMRJ_ERR_DEVICE, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
-  HIP_OK(hipEventRecord(c.graph_done, c.stream));
+  HIP_OK(hipEventRecord(c.graph_done[slot], c.stream));
   if (ps >= 0) c.prof_end(ps, KIND_GRAPH, flops, 0.0);
 }
 
@@ -1755,8 +1755,9 @@ void tridiag_eig(int k, const double* a, const double* b, std::vector<double>& l
 // then out = sum_j c_j v_j with c = exp(pf T) e1 (TensorPACK lanczos +
 // exp(M, prefactor) as used by krylovTimeEvol, algs/time.jl:182-202).
 // Returns the lowest Ritz value in *energy and the Krylov dimension used.
+// slot: graph slot of the iteration's kernel sequence (1: two-site, 2: one-site)
 dmrj_tensor* krylov_expm(Context& c, const Tensor& L, const Tensor& R, const Tensor& x, const FusedMpo& f, int r,
-                         double pf_re, double pf_im, int prec, double* energy, int* kdim) {
+                         double pf_re, double pf_im, int prec, double* energy, int* kdim, int slot) {
   if (r < 1 || r > 64) throw Error(DMRJ_ERR_ARG, "Krylov dimension must be 1..64");
   const SiteMeta& m = x.site;
   const int64_t n = std::max<int64_t>(m.size, 1);
@@ -1786,10 +1787,14 @@ dmrj_tensor* krylov_expm(Context& c, const Tensor& L, const Tensor& R, const Ten
   HIP_OK(hipMemcpyAsync(xin, V, ne * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
   HeffPlan p = plan_heff(c, m, xin, w, L.env, L.data, R.env, R.data, f, &T1, &T3, zmask);
   const HeffLaunch hl = prepare_heff(c, p, prec);  // descriptor tables stay valid until the final sync
+  // the whole iteration is a fixed kernel sequence on device scalars: with
+  // graphs on it is captured and launched as one graph
+  const bool graph = c.graph_mode != 0;
+  auto iterate = [&]() {
   for (int j = 0; j < r; ++j) {
     double* vj = V + (size_t)j * nv;
     if (j > 0) HIP_OK(hipMemcpyAsync(xin, vj, ne * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-    issue_heff(c, hl, true);
+    issue_heff(c, hl, !graph);
     // alpha_j = Re <v_j, w> (real for a Hermitian H_eff)
     {
       const double* xs[1] = {vj};
@@ -1813,6 +1818,11 @@ dmrj_tensor* krylov_expm(Context& c, const Tensor& L, const Tensor& R, const Ten
     HIP_OK(hipMemcpyAsync(vn, w, ne * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
     launch_scale_dev(c, vn, ne, c.scal + S_INVB);
   }
+  };
+  if (graph)
+    run_captured(c, iterate, r * hl.flops_alg, slot);
+  else
+    iterate();
   std::vector<double> hs(128);
   HIP_OK(hipMemcpyAsync(hs.data(), sc, 128 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   HIP_OK(hipStreamSynchronize(c.stream));
@@ -1942,7 +1952,7 @@ int dmrj_ctx_create(int device, dmrj_ctx** out) {
     HIP_OK(hipMalloc((void**)&c->partial, 4 * 256 * sizeof(double)));
     c->stage.cap = 8 << 20;
     HIP_OK(hipHostMalloc((void**)&c->stage.base, c->stage.cap, hipHostMallocDefault));
-    HIP_OK(hipEventCreateWithFlags(&c->graph_done, hipEventDisableTiming));
+    for (auto& e : c->graph_done) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   } catch (const Error& e) {
     delete c;
     return e.code;
@@ -1956,8 +1966,10 @@ int dmrj_ctx_destroy(dmrj_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->release_transients();
-  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
-  if (c->graph_done) (void)hipEventDestroy(c->graph_done);
+  for (auto& g : c->graph_exec)
+    if (g) (void)hipGraphExecDestroy(g);
+  for (auto& e : c->graph_done)
+    if (e) (void)hipEventDestroy(e);
   if (c->rocblas) rocblas_destroy_handle((rocblas_handle)c->rocblas);
   if (c->scal) (void)hipFree(c->scal);
   if (c->partial) (void)hipFree(c->partial);
@@ -2476,7 +2488,7 @@ int dmrj_krylov_expm2(dmrj_ctx* c, const dmrj_tensor* L, const dmrj_tensor* R, c
     if (!L || !R || !AA || !W1 || !W2 || !out || L->kind != 1 || R->kind != 2 || AA->kind != 0 || !AA->site.d1)
       throw Error(DMRJ_ERR_ARG, "krylov_expm2: bad handles");
     *out = krylov_expm(*c, *L, *R, *AA, fuse_two(*W1, *W2), krylov, prefactor_re, prefactor_im, base_prec(precision),
-                       energy, kdim);
+                       energy, kdim, 1);
   });
 }
 
@@ -2487,7 +2499,7 @@ int dmrj_krylov_expm1(dmrj_ctx* c, const dmrj_tensor* L, const dmrj_tensor* R, c
     if (!L || !R || !A || !W || !out || L->kind != 1 || R->kind != 2 || A->kind != 0 || A->site.d1)
       throw Error(DMRJ_ERR_ARG, "krylov_expm1: bad handles");
     *out = krylov_expm(*c, *L, *R, *A, fuse_one(*W), krylov, prefactor_re, prefactor_im, base_prec(precision), energy,
-                       kdim);
+                       kdim, 2);
   });
 }
 

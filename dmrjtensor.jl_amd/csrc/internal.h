@@ -207,8 +207,11 @@ struct Context {
   // 1 = each Lanczos step (both H_eff applications, inner products and the
   // Ritz update) is captured into one hipGraph and launched as a unit
   int graph_mode = 0;
-  hipGraphExec_t graph_exec = nullptr;
-  hipEvent_t graph_done = nullptr;
+  // executable graphs by kernel sequence: 0 = Lanczos step, 1 = two-site
+  // Krylov exponential, 2 = one-site Krylov exponential
+  static constexpr int kGraphSlots = 3;
+  hipGraphExec_t graph_exec[kGraphSlots] = {};
+  hipEvent_t graph_done[kGraphSlots] = {};
   double graph_updates = 0, graph_instantiations = 0;
   std::unordered_map<uint64_t, int> tune_cache;
   int occ_cache[3][8] = {{0}};

@@ -85,10 +85,11 @@ int dmrj_ctx_autotune_stats(dmrj_ctx* ctx, double* out5);
 /* Lanczos-step graphs (no counterpart in the reference).  mode 1: every
  * dmrj_lanczos2 / dmrj_twostep Lanczos step (both H_eff applications, the
  * inner products, the 2x2 Ritz solve on the device and the Ritz update, plus
- * the FP64 correction when requested) is captured into one hipGraph and
- * launched as a unit; consecutive steps update the executable graph in
- * place when the kernel sequence matches.  0 (default) = direct launches.
- * Results are identical in both modes. */
+ * the FP64 correction when requested) and every Krylov iteration sequence of
+ * dmrj_krylov_expm2 / dmrj_krylov_expm1 is captured into one hipGraph and
+ * launched as a unit; consecutive calls of the same kind update their
+ * executable graph in place when the kernel sequence matches.  0 (default) =
+ * direct launches.  Results are identical in both modes. */
 int dmrj_ctx_set_graph(dmrj_ctx* ctx, int mode);
 /* out2 = {graph instantiations, in-place updates} */
 int dmrj_ctx_graph_stats(dmrj_ctx* ctx, double* out2);
