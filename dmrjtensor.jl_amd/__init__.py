@@ -15,6 +15,7 @@ from .network import (MPO, MPS, DeviceTensor, Lupdate, Rupdate, environment, exp
                       site_from_dense)
 from .dmrg import algvars, dmrg, optmps, singlesite_update, startval, twostep
 from .tdvp import krylovTimeEvol1, krylovTimeEvol2, local_expect, tdvp_twosite
+from .measure import correlationmatrix
 
 __all__ = [
     "Context", "DMRJError", "default_context", "fermionOps", "heisenbergMPO", "hubbardMPO",
@@ -22,5 +23,5 @@ __all__ = [
     "environment", "expect", "makeEnv", "makeMPO", "makeq", "mpo_deltas", "product_state",
     "randMPS_normal", "site2_from_dense", "site_from_dense", "algvars", "dmrg", "optmps",
     "singlesite_update", "startval", "twostep", "krylovTimeEvol1", "krylovTimeEvol2", "local_expect",
-    "tdvp_twosite",
+    "tdvp_twosite", "correlationmatrix",
 ]

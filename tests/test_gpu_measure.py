@@ -1,0 +1,77 @@
+"""correlationmatrix on GPU-resident MPS (lib/MPO_MPS/correlationmatrix.jl:21-83)
+against the oracle's brute force on the full state vector and the
+reference's golden matrix (test/MPmeas_test.jl:310-315)."""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import measure as ome
+from oracle import models as om
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def test_correlationmatrix_heisenberg_golden(pkg, ctx):
+    """Dense MPS of the exact L=10 ground state (the reference's test builds it
+    with eigen(mpo) and makeMPS): the reference's own acceptance test and the
+    oracle on the same vector to 1e-12."""
+    say("case correlationmatrix L=10 golden")
+    with open(os.path.join(GOLD, "correlationmatrix_L10.json")) as f:
+        chk = np.array(json.load(f)["checkM"])
+    E, V = np.linalg.eigh(om.fullH(om.makeMPO(om.heisenbergMPO(), 2, 10)))
+    A = ome.vector_to_mps(V[:, 0], 2, 10)
+    psi = pkg.MPS(A, oc=1, ctx=ctx)
+    Sp, Sm, Sz, O, Id = pkg.spinOps()
+    M = pkg.correlationmatrix(psi, Sp, Sm)
+    want = ome.correlationmatrix(V[:, 0], 2, 10, Sp, Sm)
+    say(" max |M - oracle| %.2e, norm(checkM - M)/100 %.2e" % (np.max(np.abs(M - want)),
+                                                              np.linalg.norm(chk - M) / chk.size))
+    assert np.linalg.norm(chk - M) / chk.size < 1e-5
+    assert np.max(np.abs(M - want)) < 1e-12
+
+
+def test_correlationmatrix_u1_dmrg_state(pkg, ctx):
+    """U(1) block-sparse state from GPU DMRG (L=12, m=24, charged operators
+    S+ / S- and neutral Sz Sz) against the oracle on the same state's vector."""
+    say("case correlationmatrix U(1) L=12")
+    L = 12
+    qphys = np.array([[1], [-1]])
+    mpo = pkg.MPO(pkg.makeMPO(pkg.heisenbergMPO(), 2, L).H, qphys=qphys)
+    psi = pkg.product_state([i % 2 for i in range(L)], 2, qphys=qphys, ctx=ctx)
+    pkg.dmrg(psi, mpo, m=24, sweeps=3, cutoff=1e-12)
+    host, _ = psi.to_dense()
+    vec = ome.mps_to_vector(host)
+    Sp, Sm, Sz, O, Id = pkg.spinOps()
+    for name, (a, b) in (("S+S-", (Sp, Sm)), ("S-S+", (Sm, Sp)), ("SzSz", (Sz, Sz))):
+        M = pkg.correlationmatrix(psi, a, b)
+        want = ome.correlationmatrix(vec, 2, L, a, b)
+        err = np.max(np.abs(M - want))
+        say(" %s max |M - oracle| %.2e" % (name, err))
+        assert err < 1e-12
+
+
+def test_correlationmatrix_fermions_with_trail(pkg, ctx):
+    """Free fermions (Hubbard U = 0, mu = 0, L = 4), dense MPS of the exact
+    ground state: <c+_i c_j> with the Jordan-Wigner trail against the oracle
+    and the projector on the occupied orbitals."""
+    say("case correlationmatrix fermions L=4")
+    L = 4
+    E, V = np.linalg.eigh(om.fullH(om.makeMPO(om.hubbardMPO(t=1.0, mu=0.0, HubU=0.0), 4, L)))
+    psi = pkg.MPS(ome.vector_to_mps(V[:, 0], 4, L), oc=1, ctx=ctx)
+    Cup, Cdn, F, Nup, Ndn, Ndens, O, Id = pkg.fermionOps()
+    M = pkg.correlationmatrix(psi, Cup.T, Cup, trail=F)
+    want = ome.correlationmatrix(V[:, 0], 4, L, Cup.T, Cup, trail=F)
+    h = np.diag(np.ones(L - 1), 1) + np.diag(np.ones(L - 1), -1)
+    e, phi = np.linalg.eigh(h)
+    occ = phi[:, e < 0]
+    say(" max |M - oracle| %.2e, |M - P_occ| %.2e" % (np.max(np.abs(M - want)), np.max(np.abs(M - occ @ occ.T))))
+    assert np.max(np.abs(M - want)) < 1e-12
+    assert np.max(np.abs(M - occ @ occ.T)) < 1e-12

@@ -366,3 +366,47 @@ def test_realtime_tdvp_conserves_energy_without_truncation():
     # the evolution moved the state: the Neel pattern has decayed
     assert abs(ot.local_expect(psi, Sz, 3) + 0.5) > 1e-3
     assert np.iscomplexobj(psi[3])
+
+
+def test_correlationmatrix_golden_and_closed_forms():
+    """oracle/measure.py against the reference's golden matrix of the L=10
+    Heisenberg ground state (test/MPmeas_test.jl:310-315) and closed forms:
+    Neel product state, and the free-fermion one-body density matrix with the
+    Jordan-Wigner trail (a dropped trail or a swapped side fails)."""
+    from oracle import measure as ome
+    with open(os.path.join(GOLD, "correlationmatrix_L10.json")) as f:
+        chk = np.array(json.load(f)["checkM"])
+    E, V = ed_ground(om.makeMPO(om.heisenbergMPO(), 2, 10))
+    Sp, Sm, Sz, O, Id = om.spinOps()
+    M = ome.correlationmatrix(V[:, 0], 2, 10, Sp, Sm)
+    # the reference's own criterion; checkM itself differs from the exact ground state by up to
+    # 7e-6 (it came from the reference's eigen + makeMPS), so exact identities pin the rest:
+    # <S+_i S-_i> = 1/2 in the singlet, and E0 = sum_i <Sz_i Sz_i+1> + (<S+_i S-_i+1> + <S-_i S+_i+1>)/2
+    assert np.linalg.norm(chk - M) / chk.size < 1e-5
+    assert np.max(np.abs(np.diag(M) - 0.5)) < 1e-12
+    Mzz = ome.correlationmatrix(V[:, 0], 2, 10, Sz, Sz)
+    Mmp = ome.correlationmatrix(V[:, 0], 2, 10, Sm, Sp)
+    e_nn = sum(Mzz[i, i + 1] + 0.5 * (M[i, i + 1] + Mmp[i, i + 1]) for i in range(9))
+    assert np.isclose(e_nn, E[0], rtol=1e-13, atol=0)
+    # MPS round trip of the ground state
+    A = ome.vector_to_mps(V[:, 0], 2, 10)
+    assert np.allclose(ome.mps_to_vector(A), V[:, 0], atol=1e-13)
+    # Neel |up dn up ...>: <S+_i S-_i> = 1 on up sites (S+ S- acts on the ket), 0 elsewhere
+    neel = np.zeros(2 ** 6)
+    neel[sum(1 << i for i in range(1, 6, 2))] = 1.0      # site 1 fastest; bit set = down
+    Mn = ome.correlationmatrix(neel, 2, 6, Sp, Sm)
+    assert np.allclose(Mn, np.diag([1, 0, 1, 0, 1, 0]))
+    assert np.allclose(ome.correlationmatrix(neel, 2, 6, Sm, Sp), np.diag([0, 1, 0, 1, 0, 1]))
+    # free fermions (Hubbard U = 0, mu = 0, L = 4): <c+_i c_j> of the ground state is the
+    # projector on the occupied orbitals of the hopping matrix; the sign of the hopping
+    # follows from the energy identity E0 = 2 spins * sum_i 2 t rho[i, i+1]
+    Cup, Cdn, F, Nup, Ndn, Ndens, _, _ = om.fermionOps()
+    L = 4
+    E, V = ed_ground(om.makeMPO(om.hubbardMPO(t=1.0, mu=0.0, HubU=0.0), 4, L))
+    rho = ome.correlationmatrix(V[:, 0], 4, L, Cup.T, Cup, trail=F)
+    assert np.isclose(2 * sum(2 * rho[i, i + 1] for i in range(L - 1)), E[0], atol=1e-12)
+    h = np.diag(np.ones(L - 1), 1) + np.diag(np.ones(L - 1), -1)
+    e, phi = np.linalg.eigh(h)
+    occ = phi[:, e < 0]
+    assert np.allclose(rho, occ @ occ.T, atol=1e-12)
+    assert not np.allclose(ome.correlationmatrix(V[:, 0], 4, L, Cup.T, Cup), occ @ occ.T, atol=1e-3)
