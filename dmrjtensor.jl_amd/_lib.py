@@ -58,6 +58,8 @@ SIGNATURES = {
     "dmrj_dotc": [_p, _p, _p, _dp],
     "dmrj_tensor_free": [_p, _p],
     "dmrj_tensor_copy": [_p, _p, C.POINTER(_p)],
+    "dmrj_site_export": [_p, _p, _i64p, _i32p, _i32p, _i32p, _dp],
+    "dmrj_site_import": [_p, C.c_int, _i64, _i64, _i64, _i32p, _i32p, _i32p, _dp, _i64, C.c_int, C.POINTER(_p)],
     "dmrj_mpo_create": [_p, C.c_int, _i64, _i64, _i64, _dp, _i32p, _i32p, _i32p, C.POINTER(_p)],
     "dmrj_mpo_free": [_p, _p],
     "dmrj_env_boundary": [_p, C.c_int, C.c_int, _i32p, C.POINTER(_p)],

@@ -2322,6 +2322,65 @@ int dmrj_tensor_copy(dmrj_ctx* c, const dmrj_tensor* t, dmrj_tensor** out) {
   });
 }
 
+int dmrj_site_export(dmrj_ctx* c, const dmrj_tensor* t, int64_t* info8, int32_t* qL, int32_t* qP, int32_t* qR,
+                     double* data) {
+  return guard(c, [&] {
+    if (!t || !info8) throw Error(DMRJ_ERR_ARG, "null pointer");
+    if (t->kind != 0 || t->site.d1) throw Error(DMRJ_ERR_ARG, "site_export: single-site MPS tensors only");
+    const SiteMeta& m = t->site;
+    const int nq = t->nq;
+    info8[0] = nq;
+    info8[1] = m.left.total;
+    info8[2] = m.P;
+    info8[3] = m.right.total;
+    info8[4] = t->nstore();
+    info8[5] = t->cplx;
+    info8[6] = info8[7] = 0;
+    space_to_list(m.left, nq, qL);
+    space_to_list(m.right, nq, qR);
+    if (qP)
+      for (int s = 0; s < m.P; ++s)
+        for (int k = 0; k < nq; ++k) qP[s * nq + k] = m.qp[s].v[k];
+    if (data && t->nstore() > 0) {
+      HIP_OK(hipMemcpyAsync(data, t->data, t->nstore() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIP_OK(hipStreamSynchronize(c->stream));
+    }
+  });
+}
+
+int dmrj_site_import(dmrj_ctx* c, int nq, int64_t mL, int64_t P, int64_t mR, const int32_t* qL,
+                     const int32_t* qP, const int32_t* qR, const double* data, int64_t n, int cplx,
+                     dmrj_tensor** out) {
+  return guard(c, [&] {
+    if (!out || !data) throw Error(DMRJ_ERR_ARG, "null pointer");
+    if (nq < 0 || nq > kMaxQ) throw Error(DMRJ_ERR_ARG, "nq must be 0, 1 or 2");
+    if (mL <= 0 || P <= 0 || mR <= 0 || (nq && (!qL || !qP || !qR))) throw Error(DMRJ_ERR_ARG, "bad dimensions");
+    auto* t = new_tensor(*c, 0);
+    try {
+      t->nq = nq;
+      SiteMeta& m = t->site;
+      m.left = space_from_list(qL, nq, mL);
+      m.right = space_from_list(qR, nq, mR);
+      m.P = (int)P;
+      m.qp = nq ? charges_vec(qP, nq, P) : std::vector<Charge>(P);
+      m.build();
+      t->cplx = cplx ? 1 : 0;
+      if (n != t->nstore()) throw Error(DMRJ_ERR_ARG, "site_import: stored size does not match the charges");
+      check_finite_host(data, n);
+      alloc_data(*c, *t);
+      if (n > 0) {
+        HIP_OK(hipMemcpyAsync(t->data, data, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+      }
+    } catch (...) {
+      if (t->data) c->alloc.release(t->data);
+      delete t;
+      throw;
+    }
+    *out = t;
+  });
+}
+
 int dmrj_mpo_create(dmrj_ctx* c, int nq, int64_t DL, int64_t d, int64_t DR, const double* W,
                     const int32_t* qP, const int32_t* deltaL, const int32_t* deltaR, dmrj_mpo** out) {
   return guard(c, [&] {

@@ -154,6 +154,19 @@ int dmrj_tensor_to_dense(dmrj_ctx* ctx, const dmrj_tensor* t, double* out, int32
                          int32_t* qlast);
 int dmrj_tensor_free(dmrj_ctx* ctx, dmrj_tensor* t);
 int dmrj_tensor_copy(dmrj_ctx* ctx, const dmrj_tensor* t, dmrj_tensor** out);
+/* Checkpoint form of an MPS tensor (no counterpart in the reference's C-level
+ * API: this build's own storage, standing where the reference writes tensors
+ * to disk for largeMPS, lib/MPS/largeMPS.jl:19-81, and reloads them to resume,
+ * algs/optimizeMPS.jl:273-334).  dmrj_site_export fills info8 = {nq, mL, P,
+ * mR, stored doubles, complex, 0, 0}; qL (mL x nq), qP (P x nq), qR (mR x nq)
+ * and data (the stored doubles: block storage, real plane then imaginary
+ * plane) are filled when not NULL.  dmrj_site_import rebuilds the tensor bit
+ * for bit from the same arrays.  Single-site MPS tensors only. */
+int dmrj_site_export(dmrj_ctx* ctx, const dmrj_tensor* t, int64_t* info8, int32_t* qL, int32_t* qP, int32_t* qR,
+                     double* data);
+int dmrj_site_import(dmrj_ctx* ctx, int nq, int64_t mL, int64_t P, int64_t mR, const int32_t* qL,
+                     const int32_t* qP, const int32_t* qR, const double* data, int64_t n, int cplx,
+                     dmrj_tensor** out);
 
 /* ---- ComplexF64 tensors (the TDVP path, algs/time.jl:288-362, evolves
  * ComplexF64 MPS).  Host arrays are Julia's ComplexF64 layout: interleaved
