@@ -17,6 +17,7 @@ from .dmrg import algvars, dmrg, optmps, singlesite_update, startval, twostep
 from .tdvp import krylovTimeEvol1, krylovTimeEvol2, local_expect, tdvp_twosite
 from .measure import correlationmatrix
 from .checkpoint import load_mps, save_mps
+from .autompo import autoMPO, mpoterm
 
 __all__ = [
     "Context", "DMRJError", "default_context", "fermionOps", "heisenbergMPO", "hubbardMPO",
@@ -24,5 +25,5 @@ __all__ = [
     "environment", "expect", "makeEnv", "makeMPO", "makeq", "mpo_deltas", "product_state",
     "randMPS_normal", "site2_from_dense", "site_from_dense", "algvars", "dmrg", "optmps",
     "singlesite_update", "startval", "twostep", "krylovTimeEvol1", "krylovTimeEvol2", "local_expect",
-    "tdvp_twosite", "correlationmatrix", "save_mps", "load_mps",
+    "tdvp_twosite", "correlationmatrix", "save_mps", "load_mps", "autoMPO", "mpoterm",
 ]
