@@ -1095,7 +1095,17 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
     const double mm = (double)std::max(s.R, s.C), nn = (double)std::min(s.R, s.C);
     svd_flops += 4.0 * mm * nn * nn + 8.0 * nn * nn * nn;
   }
-  launch_svd_small(c, small, small_lds);
+  const bool small_on_side = launch_svd_small(c, small, small_lds);
+  // every exit from here on, exceptions included, orders the main stream
+  // after the side-stream kernel, so no later synchronisation of the main
+  // stream can release memory that kernel still uses
+  struct SideJoin {
+    Context& c;
+    bool on;
+    ~SideJoin() {
+      if (on) (void)hipStreamWaitEvent(c.stream, c.side_out, 0);
+    }
+  } side_join{c, small_on_side};
   // large sectors, backend 4: batched block one-sided Jacobi on M (dir > 0,
   // orthonormal U wanted) or on M^T (dir < 0, orthonormal V wanted); the
   // other factor is one grouped GEMM with M after the truncation
@@ -1358,6 +1368,7 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
     const double mm = (double)std::max(s.R, s.C), nn = (double)std::min(s.R, s.C);
     svd_flops += 4.0 * mm * nn * nn + 8.0 * nn * nn * nn;
   }
+  if (small_on_side) HIP_OK(hipStreamWaitEvent(c.stream, c.side_out, 0));  // small sectors done
   c.prof_end(psvd, KIND_SVD, svd_flops, 0.0);
   // singular values to host
   std::vector<double> hs(std::max<int64_t>(total_s, 1));
@@ -1938,6 +1949,9 @@ int dmrj_ctx_create(int device, dmrj_ctx** out) {
   try {
     HIP_OK(hipSetDevice(device));
     HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIP_OK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    HIP_OK(hipEventCreateWithFlags(&c->side_in, hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&c->side_out, hipEventDisableTiming));
     hipDeviceProp_t prop;
     HIP_OK(hipGetDeviceProperties(&prop, device));
     c->num_cu = prop.multiProcessorCount;
@@ -1965,7 +1979,11 @@ int dmrj_ctx_destroy(dmrj_ctx* c) {
   if (!c) return DMRJ_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->side) (void)hipStreamSynchronize(c->side);
   c->release_transients();
+  if (c->side_in) (void)hipEventDestroy(c->side_in);
+  if (c->side_out) (void)hipEventDestroy(c->side_out);
+  if (c->side) (void)hipStreamDestroy(c->side);
   for (auto& g : c->graph_exec)
     if (g) (void)hipGraphExecDestroy(g);
   for (auto& e : c->graph_done)

@@ -182,6 +182,11 @@ struct HostStage {   // pinned staging memory for plan uploads
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
+  // second stream (default priority) for work that overlaps the main one:
+  // the small-sector SVD kernel runs beside the batched eigensolve of the
+  // large sectors; side_in / side_out order it against the main stream
+  hipStream_t side = nullptr;
+  hipEvent_t side_in = nullptr, side_out = nullptr;
   void* rocblas = nullptr;   // rocblas_handle
   Allocator alloc;
   HostStage stage;
@@ -413,7 +418,8 @@ struct SmallSvdJob {
   double* Vt;
   int R, C;
 };
-void launch_svd_small(Context& c, const std::vector<SmallSvdJob>& jobs, size_t lds_bytes);
+// runs on c.side; join with hipStreamWaitEvent(c.stream, c.side_out) when it returns true
+bool launch_svd_small(Context& c, const std::vector<SmallSvdJob>& jobs, size_t lds_bytes);
 struct AugJob {
   const double* A;
   const double* W;

@@ -199,13 +199,23 @@ size_t small_svd_lds(int R, int C) {
   return ((size_t)np * rows + (size_t)np * n) * sizeof(double);
 }
 
-void launch_svd_small(Context& c, const std::vector<SmallSvdJob>& jobs, size_t lds_bytes) {
-  if (jobs.empty()) return;
+// Launches on the context's side stream after everything queued on the main
+// stream so far (inputs, the job table upload); the caller joins with
+// hipStreamWaitEvent(c.stream, c.side_out) before reading the results.
+// Returns whether a kernel was launched.
+bool launch_svd_small(Context& c, const std::vector<SmallSvdJob>& jobs, size_t lds_bytes) {
+  if (jobs.empty()) return false;
   auto* dj = (SmallSvdJob*)c.upload(jobs.data(), jobs.size() * sizeof(SmallSvdJob));
-  hipLaunchKernelGGL(k_svd_small, dim3((unsigned)jobs.size()), dim3(kSvdThreads), lds_bytes, c.stream, dj,
+  auto ok = [](hipError_t e) {
+    if (e != hipSuccess) throw Error(-3, std::string("HIP: ") + hipGetErrorString(e));
+  };
+  ok(hipEventRecord(c.side_in, c.stream));
+  ok(hipStreamWaitEvent(c.side, c.side_in, 0));
+  hipLaunchKernelGGL(k_svd_small, dim3((unsigned)jobs.size()), dim3(kSvdThreads), lds_bytes, c.side, dj,
                      (int)jobs.size());
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) throw Error(-3, std::string("HIP: ") + hipGetErrorString(e));
+  ok(hipGetLastError());
+  ok(hipEventRecord(c.side_out, c.side));
+  return true;
 }
 
 }  // namespace dmrj
