@@ -1767,8 +1767,14 @@ void tridiag_eig(int k, const double* a, const double* b, std::vector<double>& l
 // exp(M, prefactor) as used by krylovTimeEvol, algs/time.jl:182-202).
 // Returns the lowest Ritz value in *energy and the Krylov dimension used.
 // slot: graph slot of the iteration's kernel sequence (1: two-site, 2: one-site)
+// ritz: out is the lowest Ritz vector instead (c = eigenvector of T of the
+// lowest eigenvalue, signed so that c_1 = <v_1, out> >= 0; the prefactor is
+// ignored), the eigensolver of TensorPACK lanczos as 3S calls it
+// (algs/DMRG.jl:328)
 dmrj_tensor* krylov_expm(Context& c, const Tensor& L, const Tensor& R, const Tensor& x, const FusedMpo& f, int r,
-                         double pf_re, double pf_im, int prec, double* energy, int* kdim, int slot) {
+                         double pf_re, double pf_im, int prec, double* energy, int* kdim, int slot,
+                         bool ritz = false) {
+  if (ritz) pf_re = pf_im = 0.0;
   if (r < 1 || r > 64) throw Error(DMRJ_ERR_ARG, "Krylov dimension must be 1..64");
   const SiteMeta& m = x.site;
   const int64_t n = std::max<int64_t>(m.size, 1);
@@ -1853,14 +1859,23 @@ dmrj_tensor* krylov_expm(Context& c, const Tensor& L, const Tensor& R, const Ten
   // c = Q exp(pf Lambda) Q^T e1, pf complex
   std::vector<double> cre(k, 0.0), cim(k, 0.0);
   double emin = lam[0];
-  for (int i = 0; i < k; ++i) emin = std::min(emin, lam[i]);
-  for (int i = 0; i < k; ++i) {
+  int imin = 0;
+  for (int i = 0; i < k; ++i)
+    if (lam[i] < emin) {
+      emin = lam[i];
+      imin = i;
+    }
+  for (int i = 0; i < k && !ritz; ++i) {
     const double mag = std::exp(pf_re * lam[i]) * Q[0 + (size_t)i * k];
     const double gr = mag * std::cos(pf_im * lam[i]), gi = mag * std::sin(pf_im * lam[i]);
     for (int jj = 0; jj < k; ++jj) {
       cre[jj] += Q[jj + (size_t)i * k] * gr;
       cim[jj] += Q[jj + (size_t)i * k] * gi;
     }
+  }
+  if (ritz) {
+    const double sg = Q[0 + (size_t)imin * k] >= 0.0 ? 1.0 : -1.0;
+    for (int jj = 0; jj < k; ++jj) cre[jj] = sg * Q[jj + (size_t)imin * k];
   }
   auto* out = new dmrj_tensor();
   out->kind = 0;
@@ -1899,6 +1914,366 @@ dmrj_tensor* krylov_expm(Context& c, const Tensor& L, const Tensor& R, const Ten
   if (energy) *energy = emin;
   if (kdim) *kdim = k;
   return out;
+}
+
+// ============================================== 3S subspace expansion and move
+// Lexpand / Rexpand (algs/DMRG.jl:273-286) followed by joinindex!: the site
+// tensor A joined with alpha times the environment-side half of H_1site A
+// along the bond the centre moves across.  The result is laid out as a
+// two-site tensor with one trivial physical index, so that the truncated SVD
+// of the two-site path splits it as moveR! / moveL! do:
+//   dir > 0: [a, s, 1, b'] with b' = b + sum_u (u, b), the new column (u, b)
+//            carrying charge q_b + dR[u];  P[x,k,(u,b)] = sum L[x,w,a] A[a,j,b] W[w,j,k,u]
+//   dir < 0: [a', 1, s, b] with a' = a + sum_w (a, w), the new row (a, w)
+//            carrying charge q_a + dL[w];  P[(a,w),k,B] = sum W[w,j,k,u] A[a,j,b] R[b,u,B]
+// Inside every charge sector of the enlarged bond the original states come
+// first, so "restricted to the original bond" is a prefix of each sector.
+// Groups (u, b) / (a, w) that no MPO coefficient reaches are left out (they
+// would be zero columns / rows).  alpha == 0: A alone in the same layout.
+dmrj_tensor* expand3s(Context& c, const Tensor& E, const Tensor& A, const MpoSite& W, double alpha, int dir) {
+  if (A.kind != 0 || A.site.d1 || A.cplx || E.cplx)
+    throw Error(DMRJ_ERR_ARG, "expand3s: needs a real one-site tensor and a real environment");
+  const SiteMeta& x = A.site;
+  const Space& S = x.left;
+  const Space& T = x.right;
+  if (x.P != W.d) throw Error(DMRJ_ERR_ARG, "expand3s: MPO does not match the site tensor");
+  const bool grow = alpha != 0.0;
+  std::vector<GemmTask> t1;
+  std::vector<GemmSeg> s1;
+  std::vector<MixDst> md;
+  std::vector<MixTerm> mt;
+  struct Piece {  // one expansion block: Y block -> (tensor block, offset inside the enlarged sector)
+    int64_t yo;
+    int blk_a, blk_s;   // block (left sector, physical state) of the output tensor
+    int group;          // expansion group
+    int rows, cols;
+  };
+  std::vector<Piece> pieces;
+  double* X = nullptr;
+  double* Y = nullptr;
+  auto* t = new_tensor(c, 0);
+  t->nq = A.nq;
+  SiteMeta& m = t->site;
+  std::vector<CopyOp> ops;
+  try {
+    if (dir > 0) {
+      if (E.kind != 1) throw Error(DMRJ_ERR_ARG, "expand3s: moving right needs the left environment");
+      const EnvMeta& L = E.env;
+      if (!L.ket.same(S) || !L.bra.same(S) || L.D != W.DL)
+        throw Error(DMRJ_ERR_ARG, "expand3s: left environment does not match the site tensor / MPO");
+      const int nL = S.nsec(), nR = T.nsec();
+      std::vector<int64_t> gofs((size_t)W.DR * nR, -1);   // group (u, b): offset inside its enlarged sector
+      std::map<Charge, int64_t> dimQ;
+      for (int b = 0; b < nR; ++b) dimQ[T.q[b]] += T.dim[b];
+      if (grow) {
+        // X(w,a) = L(ap,w) * Apanel(a);  Y(ap,k,u,b) = sum_{w,j} W(w,j,k,u) X(w,a)[., (j,b)]
+        std::vector<int64_t> xoff((size_t)W.DL * nL, -1);
+        int64_t nx = 0;
+        for (int w = 0; w < W.DL; ++w)
+          for (int a = 0; a < nL; ++a) {
+            if (x.panel_cols[a] == 0) continue;
+            const int ap = S.find(S.q[a] + W.dL[w]);
+            if (ap < 0 || L.block(ap, w) < 0) continue;
+            xoff[(size_t)w * nL + a] = nx;
+            nx += (int64_t)S.dim[ap] * x.panel_cols[a];
+          }
+        int64_t ny = 0;
+        std::vector<char> used((size_t)W.DR * nR, 0);
+        for (int ap = 0; ap < nL; ++ap)
+          for (int k = 0; k < x.P; ++k) {
+            // the row (x, k) may reach charges the original bond does not
+            // hold: new sectors of the enlarged bond
+            const Charge Q = S.q[ap] + x.qp[k];
+            for (int u = 0; u < W.DR; ++u) {
+              const int b = T.find(Q - W.dR[u]);
+              if (b < 0) continue;
+              MixDst d{nullptr, (int64_t)S.dim[ap] * T.dim[b], (int)mt.size(), 0};
+              for (int w = 0; w < W.DL; ++w) {
+                const int a = S.find(S.q[ap] - W.dL[w]);
+                if (a < 0) continue;
+                const int64_t xo = xoff[(size_t)w * nL + a];
+                if (xo < 0) continue;
+                for (int j = 0; j < x.P; ++j) {
+                  const int xb = x.block(a, j);
+                  if (xb < 0 || x.blocks[xb].b != b) continue;
+                  const double cf = W.at(w, j, k, u);
+                  if (cf == 0.0) continue;
+                  mt.push_back({(const double*)(intptr_t)(xo + x.blocks[xb].colpos * S.dim[ap]), cf});
+                  d.nt++;
+                }
+              }
+              if (d.nt == 0) {
+                mt.resize(d.t0);
+                continue;
+              }
+              d.dst = (double*)(intptr_t)ny;
+              md.push_back(d);
+              pieces.push_back({ny, ap, k, u * nR + b, S.dim[ap], T.dim[b]});
+              used[(size_t)u * nR + b] = 1;
+              ny += d.n;
+            }
+          }
+        X = (double*)c.alloc.alloc(std::max<int64_t>(nx, 1) * sizeof(double));
+        Y = (double*)c.alloc.alloc(std::max<int64_t>(ny, 1) * sizeof(double));
+        for (auto& d : md) d.dst = Y + (intptr_t)d.dst;
+        for (auto& q : mt) q.src = X + (intptr_t)q.src;
+        for (int w = 0; w < W.DL; ++w)
+          for (int a = 0; a < nL; ++a) {
+            const int64_t o = xoff[(size_t)w * nL + a];
+            if (o < 0) continue;
+            const int ap = S.find(S.q[a] + W.dL[w]);
+            const EnvBlock& lb = L.blocks[L.block(ap, w)];
+            t1.push_back({X + o, S.dim[ap], S.dim[ap], (int)x.panel_cols[a], (int)s1.size(), 1, 0.0});
+            s1.push_back({E.data + lb.off, A.data + x.panel_off[a], S.dim[ap], S.dim[a], S.dim[a], 0, 1.0});
+          }
+        // groups in the reference's column order: b outer, u inner
+        for (int b = 0; b < nR; ++b)
+          for (int u = 0; u < W.DR; ++u) {
+            if (!used[(size_t)u * nR + b]) continue;
+            int64_t& cur = dimQ[T.q[b] + W.dR[u]];
+            gofs[(size_t)u * nR + b] = cur;
+            cur += T.dim[b];
+          }
+      }
+      Space Tx;
+      for (auto& kv : dimQ) {
+        Tx.q.push_back(kv.first);
+        Tx.dim.push_back((int)kv.second);
+      }
+      Tx.finish();
+      m.left = S;
+      m.right = Tx;
+      m.d1 = x.P;
+      m.d2 = 1;
+      m.qp1 = x.qp;
+      m.qp2 = std::vector<Charge>(1);
+      m.P = x.P;
+      m.qp = x.qp;
+      m.build();
+      alloc_data(c, *t);
+      launch_fill(c, t->data, m.size, 0.0);
+      for (const SiteBlock& ab : x.blocks) {
+        const int tb = m.block(ab.a, ab.s);
+        if (tb < 0 || m.right.q[m.blocks[tb].b] != T.q[ab.b]) throw Error(DMRJ_ERR_STATE, "expand3s: block table");
+        ops.push_back({A.data + ab.off, t->data + m.blocks[tb].off, S.dim[ab.a], S.dim[ab.a], S.dim[ab.a],
+                       T.dim[ab.b], nullptr, nullptr, 1.0});
+      }
+      for (const Piece& p : pieces) {
+        const int tb = m.block(p.blk_a, p.blk_s);
+        if (tb < 0) throw Error(DMRJ_ERR_STATE, "expand3s: expansion block missing");
+        const int64_t co = gofs[p.group];
+        ops.push_back({Y + p.yo, t->data + m.blocks[tb].off + co * p.rows, p.rows, p.rows, p.rows, p.cols, nullptr,
+                       nullptr, alpha});
+      }
+    } else {
+      if (E.kind != 2) throw Error(DMRJ_ERR_ARG, "expand3s: moving left needs the right environment");
+      const EnvMeta& R = E.env;
+      if (!R.ket.same(T) || !R.bra.same(T) || R.D != W.DR)
+        throw Error(DMRJ_ERR_ARG, "expand3s: right environment does not match the site tensor / MPO");
+      const int nL = S.nsec();
+      std::vector<int64_t> gofs((size_t)W.DL * nL, -1);   // group (a, w): row offset inside its enlarged sector
+      std::map<Charge, int64_t> dimQ;
+      for (int a = 0; a < nL; ++a) dimQ[S.q[a]] += S.dim[a];
+      if (grow) {
+        // X(a,j,u) = A(a,j) * Rs(B,u)^T, B = q_b + dR[u];  Y(a,w,k) = sum_{j,u} W(w,j,k,u) X(a,j,u)
+        std::vector<int64_t> xoff((size_t)nL * x.P * W.DR, -1);
+        std::vector<int> xB((size_t)nL * x.P * W.DR, -1);
+        int64_t nx = 0;
+        for (int a = 0; a < nL; ++a)
+          for (int j = 0; j < x.P; ++j) {
+            const int ab = x.block(a, j);
+            if (ab < 0) continue;
+            const int b = x.blocks[ab].b;
+            for (int u = 0; u < W.DR; ++u) {
+              const int B = T.find(T.q[b] + W.dR[u]);
+              if (B < 0) continue;
+              const int rb = R.block(B, u);
+              if (rb < 0 || R.blocks[rb].a != b) continue;
+              const size_t key = ((size_t)a * x.P + j) * W.DR + u;
+              xoff[key] = nx;
+              xB[key] = B;
+              nx += (int64_t)S.dim[a] * T.dim[B];
+            }
+          }
+        int64_t ny = 0;
+        std::vector<char> used((size_t)W.DL * nL, 0);
+        for (int a = 0; a < nL; ++a)
+          for (int w = 0; w < W.DL; ++w)
+            for (int k = 0; k < x.P; ++k) {
+              const int B = T.find(S.q[a] + W.dL[w] + x.qp[k]);
+              if (B < 0) continue;
+              MixDst d{nullptr, (int64_t)S.dim[a] * T.dim[B], (int)mt.size(), 0};
+              for (int j = 0; j < x.P; ++j)
+                for (int u = 0; u < W.DR; ++u) {
+                  const size_t key = ((size_t)a * x.P + j) * W.DR + u;
+                  if (xoff[key] < 0 || xB[key] != B) continue;
+                  const double cf = W.at(w, j, k, u);
+                  if (cf == 0.0) continue;
+                  mt.push_back({(const double*)(intptr_t)xoff[key], cf});
+                  d.nt++;
+                }
+              if (d.nt == 0) {
+                mt.resize(d.t0);
+                continue;
+              }
+              d.dst = (double*)(intptr_t)ny;
+              md.push_back(d);
+              pieces.push_back({ny, a, k, w * nL + a, S.dim[a], T.dim[B]});
+              used[(size_t)w * nL + a] = 1;
+              ny += d.n;
+            }
+        X = (double*)c.alloc.alloc(std::max<int64_t>(nx, 1) * sizeof(double));
+        Y = (double*)c.alloc.alloc(std::max<int64_t>(ny, 1) * sizeof(double));
+        for (auto& d : md) d.dst = Y + (intptr_t)d.dst;
+        for (auto& q : mt) q.src = X + (intptr_t)q.src;
+        for (int a = 0; a < nL; ++a)
+          for (int j = 0; j < x.P; ++j) {
+            const int ab = x.block(a, j);
+            if (ab < 0) continue;
+            const int b = x.blocks[ab].b;
+            for (int u = 0; u < W.DR; ++u) {
+              const size_t key = ((size_t)a * x.P + j) * W.DR + u;
+              if (xoff[key] < 0) continue;
+              const int B = xB[key];
+              const EnvBlock& rb = R.blocks[R.block(B, u)];
+              // X[n_a x n_B] = A(a,j)[n_a x n_b] * (Rs(B,u)[n_B x n_b])^T
+              t1.push_back({X + xoff[key], S.dim[a], S.dim[a], T.dim[B], (int)s1.size(), 1, 0.0});
+              s1.push_back({A.data + x.blocks[ab].off, E.data + rb.off, S.dim[a], T.dim[B], T.dim[b], 2, 1.0});
+            }
+          }
+        // groups in the reference's row order: w outer, a inner
+        for (int w = 0; w < W.DL; ++w)
+          for (int a = 0; a < nL; ++a) {
+            if (!used[(size_t)w * nL + a]) continue;
+            int64_t& cur = dimQ[S.q[a] + W.dL[w]];
+            gofs[(size_t)w * nL + a] = cur;
+            cur += S.dim[a];
+          }
+      }
+      Space Sx;
+      for (auto& kv : dimQ) {
+        Sx.q.push_back(kv.first);
+        Sx.dim.push_back((int)kv.second);
+      }
+      Sx.finish();
+      m.left = Sx;
+      m.right = T;
+      m.d1 = 1;
+      m.d2 = x.P;
+      m.qp1 = std::vector<Charge>(1);
+      m.qp2 = x.qp;
+      m.P = x.P;
+      m.qp = x.qp;
+      m.build();
+      alloc_data(c, *t);
+      launch_fill(c, t->data, m.size, 0.0);
+      for (const SiteBlock& ab : x.blocks) {
+        const int ax = Sx.find(S.q[ab.a]);
+        const int tb = ax < 0 ? -1 : m.block(ax, ab.s);
+        if (tb < 0 || m.blocks[tb].b != ab.b) throw Error(DMRJ_ERR_STATE, "expand3s: block table");
+        ops.push_back({A.data + ab.off, t->data + m.blocks[tb].off, S.dim[ab.a], Sx.dim[ax], S.dim[ab.a],
+                       T.dim[ab.b], nullptr, nullptr, 1.0});
+      }
+      for (const Piece& p : pieces) {
+        const int a = p.blk_a, k = p.blk_s, w = p.group / nL;
+        const int ax = Sx.find(S.q[a] + W.dL[w]);
+        const int tb = ax < 0 ? -1 : m.block(ax, k);
+        if (tb < 0) throw Error(DMRJ_ERR_STATE, "expand3s: expansion block missing");
+        ops.push_back({Y + p.yo, t->data + m.blocks[tb].off + gofs[p.group], p.rows, Sx.dim[ax], p.rows, p.cols,
+                       nullptr, nullptr, alpha});
+      }
+    }
+    if (grow) {
+      launch_gemm(c, t1, s1, -1);
+      launch_mix(c, md, mt);
+    }
+    launch_copy(c, ops);
+  } catch (...) {
+    c.alloc.release(X);
+    c.alloc.release(Y);
+    if (t->data) c.alloc.release(t->data);
+    delete t;
+    throw;
+  }
+  c.flops_env += gemm_flops(t1, s1);
+  // the tables above are staged asynchronously; X and Y go back to the pool,
+  // which hands them out again only to later work on the same stream
+  c.alloc.release(X);
+  c.alloc.release(Y);
+  return t;
+}
+
+// moveR! / moveL! with condition = true (lib/MPS/move.jl:124-139,175-190):
+// the new centre is the neighbour contracted with the SVD factor restricted
+// to the original bond (a prefix of every sector of the enlarged bond).
+//   dir > 0: F = D V [k, 1, b'], N = psi[iR] [b, s, c] -> [k, s, c]
+//   dir < 0: F = U D [a', 1, k], N = psi[iL] [aL, s, a] -> [aL, s, k]
+dmrj_tensor* absorb3s(Context& c, const Tensor& F, const Tensor& N, int dir) {
+  const SiteMeta& f = F.site;
+  const SiteMeta& n = N.site;
+  if (F.kind || N.kind || f.d1 || n.d1 || F.cplx || N.cplx || f.P != 1)
+    throw Error(DMRJ_ERR_ARG, "absorb3s: needs a real SVD factor with one trivial index and a real site tensor");
+  auto* t = new_tensor(c, 0);
+  t->nq = N.nq;
+  SiteMeta& m = t->site;
+  std::vector<GemmTask> tasks;
+  std::vector<GemmSeg> segs;
+  if (dir > 0) {
+    m.left = f.left;
+    m.right = n.right;
+  } else {
+    m.left = n.left;
+    m.right = f.right;
+  }
+  m.P = n.P;
+  m.qp = n.qp;
+  m.build();
+  alloc_data(c, *t);
+  try {
+    if (dir > 0) {
+      for (int K = 0; K < m.left.nsec(); ++K) {
+        if (m.panel_cols[K] == 0) continue;
+        const int dK = m.left.dim[K];
+        GemmTask tk{t->data + m.panel_off[K], dK, dK, (int)m.panel_cols[K], (int)segs.size(), 0, 0.0};
+        const int fb = f.block(K, 0);
+        const int b = n.left.find(m.left.q[K]);
+        if (fb >= 0 && b >= 0 && n.panel_cols[b] > 0) {
+          const int bx = f.blocks[fb].b;
+          if (n.panel_cols[b] != m.panel_cols[K] || f.right.dim[bx] < n.left.dim[b] ||
+              f.right.q[bx] != n.left.q[b])
+            throw Error(DMRJ_ERR_ARG, "absorb3s: factor and neighbour do not share the original bond");
+          segs.push_back({F.data + f.blocks[fb].off, N.data + n.panel_off[b], dK, n.left.dim[b], n.left.dim[b], 0,
+                          1.0});
+          tk.nseg = 1;
+        }
+        tasks.push_back(tk);
+      }
+    } else {
+      for (const SiteBlock& ob : m.blocks) {
+        const int daL = m.left.dim[ob.a];
+        GemmTask tk{t->data + ob.off, daL, daL, m.right.dim[ob.b], (int)segs.size(), 0, 0.0};
+        const Charge Q = m.left.q[ob.a] + n.qp[ob.s];
+        const int nb = n.block(ob.a, ob.s);
+        const int ax = f.left.find(Q);
+        const int fb = ax < 0 ? -1 : f.block(ax, 0);
+        if (nb >= 0 && fb >= 0) {
+          const int a = n.blocks[nb].b;
+          if (f.blocks[fb].b != ob.b || f.left.dim[ax] < n.right.dim[a])
+            throw Error(DMRJ_ERR_ARG, "absorb3s: factor and neighbour do not share the original bond");
+          segs.push_back({N.data + n.blocks[nb].off, F.data + f.blocks[fb].off, daL, f.left.dim[ax], n.right.dim[a],
+                          0, 1.0});
+          tk.nseg = 1;
+        }
+        tasks.push_back(tk);
+      }
+    }
+  } catch (...) {
+    c.alloc.release(t->data);
+    delete t;
+    throw;
+  }
+  launch_gemm(c, tasks, segs, -1);
+  return t;
 }
 
 double read_scalar(Context& c, int slot) {
@@ -2577,6 +2952,31 @@ int dmrj_krylov_expm1(dmrj_ctx* c, const dmrj_tensor* L, const dmrj_tensor* R, c
       throw Error(DMRJ_ERR_ARG, "krylov_expm1: bad handles");
     *out = krylov_expm(*c, *L, *R, *A, fuse_one(*W), krylov, prefactor_re, prefactor_im, base_prec(precision), energy,
                        kdim, 2);
+  });
+}
+
+int dmrj_lanczos1(dmrj_ctx* c, const dmrj_tensor* L, const dmrj_tensor* R, const dmrj_tensor* A, const dmrj_mpo* W,
+                  int krylov, int precision, dmrj_tensor** out, double* energy, int* kdim) {
+  return guard(c, [&] {
+    if (!L || !R || !A || !W || !out || L->kind != 1 || R->kind != 2 || A->kind != 0 || A->site.d1)
+      throw Error(DMRJ_ERR_ARG, "lanczos1: bad handles");
+    *out = krylov_expm(*c, *L, *R, *A, fuse_one(*W), krylov, 0.0, 0.0, base_prec(precision), energy, kdim, 2, true);
+  });
+}
+
+int dmrj_expand3s(dmrj_ctx* c, const dmrj_tensor* E, const dmrj_tensor* A, const dmrj_mpo* W, double alpha, int dir,
+                  dmrj_tensor** out) {
+  return guard(c, [&] {
+    if (!E || !A || !W || !out || (dir != 1 && dir != -1)) throw Error(DMRJ_ERR_ARG, "expand3s: bad arguments");
+    if (!std::isfinite(alpha)) throw Error(DMRJ_ERR_ARG, "expand3s: alpha must be finite");
+    *out = expand3s(*c, *E, *A, *W, alpha, dir);
+  });
+}
+
+int dmrj_absorb3s(dmrj_ctx* c, const dmrj_tensor* F, const dmrj_tensor* N, int dir, dmrj_tensor** out) {
+  return guard(c, [&] {
+    if (!F || !N || !out || (dir != 1 && dir != -1)) throw Error(DMRJ_ERR_ARG, "absorb3s: bad arguments");
+    *out = absorb3s(*c, *F, *N, dir);
   });
 }
 

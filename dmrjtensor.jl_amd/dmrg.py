@@ -1,8 +1,12 @@
-"""Two-site DMRG driver on the GPU path.
+"""Two-site and 3S DMRG drivers on the GPU path.
 
 Mirrors algs/DMRG.jl and algs/optimizeMPS.jl:
-  dmrg           algs/DMRG.jl:549-589 (method dispatch; "twosite" only)
+  dmrg           algs/DMRG.jl:549-589 (method dispatch: "twosite", "3S")
   dmrg_twosite   algs/DMRG.jl:675-691
+  dmrg3S         algs/DMRG.jl:655-671 (nsites = 1: the same bond pairs)
+  step3S         algs/DMRG.jl:322-360 (dmrj_lanczos1 -> dmrj_expand3s ->
+                 dmrj_normalize -> dmrj_svd_trunc2 -> dmrj_absorb3s)
+  alpha_update   algs/DMRG.jl:288-303 (noise control, on the host)
   optmps         algs/optimizeMPS.jl:759-849 (sweep driver, plugin kwargs)
   algvars        algs/optimizeMPS.jl:36-106, defaults :514-577, SvNbond :614
   twostep        algs/DMRG.jl:362-405 (one C-ABI call: normalise, lanczos2,
@@ -59,6 +63,13 @@ class algvars:
         self.efficient = False
         self.Lenv = None
         self.Renv = None
+        # 3S: Krylov vectors per site and the noise (subspace expansion)
+        # control of algvars (algs/optimizeMPS.jl:528-537)
+        self.method = "twosite"
+        self.r = 2
+        self.noise = 1.0
+        self.noise_goal = 0.3
+        self.noise_incr = 0.01
         self.history = []
         self.sweep_times = []
 
@@ -123,6 +134,93 @@ def twostep(n, j, i, iL, iR, psi, mpo, Lenv, Renv, params):
     SvNcheck(i, j, sv[:min(cap, kept.value)], len(psi), params)
 
 
+def lanczos1(Lenv, Renv, A, mpo, i, r, precision="f64"):
+    """TensorPACK lanczos(A, mpo[i]; r, m=1, updatefct=singlesite_update) as
+    step3S calls it (algs/DMRG.jl:328): (Ritz vector, lowest Ritz value)."""
+    ctx = A.ctx
+    h = C.c_void_p()
+    e = C.c_double()
+    k = C.c_int()
+    ctx.call("dmrj_lanczos1", Lenv.h, Renv.h, A.h, mpo.device(ctx, i), int(r), _lib.PREC[precision] & 15,
+             C.byref(h), C.byref(e), C.byref(k))
+    return DeviceTensor(ctx, h), e.value
+
+
+def expand3s(E, A, mpo, i, alpha, direction):
+    """Lexpand (direction 1, E = Lenv[i]) / Rexpand (direction -1, E =
+    Renv[i]) joined to A (algs/DMRG.jl:273-286); alpha = 0: A alone."""
+    ctx = A.ctx
+    h = C.c_void_p()
+    ctx.call("dmrj_expand3s", E.h, A.h, mpo.device(ctx, i), float(alpha), int(direction), C.byref(h))
+    return DeviceTensor(ctx, h)
+
+
+def absorb3s(F, N, direction):
+    """New centre of moveR!/moveL! with condition = true
+    (lib/MPS/move.jl:124-139,175-190)."""
+    ctx = F.ctx
+    h = C.c_void_p()
+    ctx.call("dmrj_absorb3s", F.h, N.h, int(direction), C.byref(h))
+    return DeviceTensor(ctx, h)
+
+
+def alpha_update(alpha, truncerr, cutoff, lastEnergy, currenergy, noise_goal, dalpha, avgE=None):
+    """algs/DMRG.jl:288-303, with IEEE division as in Julia (x/0 = +-Inf,
+    0/0 = NaN, which compares false)."""
+    avgE = currenergy if avgE is None else avgE
+    trunc_check = truncerr <= max(cutoff, 1e-12)
+    errEst = abs(currenergy - lastEnergy) <= max(abs(currenergy * cutoff), 1e-12)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ratio = np.float64(lastEnergy) / np.float64(currenergy) - 1.0
+        sitecond = bool(np.abs(np.float64(truncerr * avgE) / ratio) > noise_goal)
+    if currenergy <= lastEnergy and (sitecond or errEst or trunc_check):
+        alpha *= 1.0 - dalpha
+    else:
+        alpha *= 1.0 + dalpha
+    return max(min(alpha, 1.0), 1e-42)
+
+
+def step3S(n, j, i, iL, iR, psi, mpo, Lenv, Renv, params):
+    """One site of 3S (algs/DMRG.jl:322-360): single-site Lanczos, subspace
+    expansion while the noise exceeds noise_incr, truncated SVD of the
+    expanded tensor (scaled to unit norm) and the move of the centre."""
+    A, energy = lanczos1(Lenv[i], Renv[i], psi[i], mpo, i, params.r, params.precision)
+    params.energy = energy
+    noise = params.noise
+    alpha = noise if noise > params.noise_incr else 0.0
+    direction = 1 if j > 0 else -1
+    T = expand3s(Lenv[iL] if j > 0 else Renv[iR], A, mpo, i, alpha, direction)
+    A.free()
+    T.ctx.call("dmrj_normalize", T.h)
+    ctx = psi.ctx
+    cap = max(8, 4 * (params.maxm if params.maxm > 0 else 4096))
+    sv = np.zeros(cap)
+    nl, nr = C.c_void_p(), C.c_void_p()
+    te = C.c_double()
+    kept = C.c_int64()
+    ctx.call("dmrj_svd_trunc2", T.h, int(params.maxm), int(params.minm), float(params.cutoff), direction,
+             C.byref(nl), C.byref(nr), C.byref(te), C.byref(kept), _lib.dptr(sv), cap)
+    T.free()
+    left, right = DeviceTensor(ctx, nl), DeviceTensor(ctx, nr)
+    if j > 0:
+        newR = absorb3s(right, psi[iR], 1)
+        right.free()
+        psi[iL].free()
+        psi[iR].free()
+        psi[iL], psi[iR] = left, newR
+    else:
+        newL = absorb3s(left, psi[iL], -1)
+        left.free()
+        psi[iL].free()
+        psi[iR].free()
+        psi[iL], psi[iR] = newL, right
+    params.noise = alpha_update(noise, te.value, params.cutoff, params.lastenergy, params.energy,
+                                params.noise_goal, params.noise_incr)
+    params.truncerr = te.value
+    params.biggestm = max(params.biggestm, int(kept.value))
+    SvNcheck(i, j, sv[:min(cap, kept.value)], len(psi), params)
+
+
 def dmrgcvg(n, timer, params):
     """algs/DMRG.jl:610-652 (cvgE = true)."""
     if not params.silent:
@@ -133,7 +231,7 @@ def dmrgcvg(n, timer, params):
     params.history.append(dict(sweep=n, energy=float(params.energy), maxtrunc=float(params.maxtrunc),
                                biggestm=int(params.biggestm), entropy=float(params.entropy),
                                storeD=list(params.storeD), time=timer, precision=params.precision,
-                               residual=params.residual))
+                               residual=params.residual, noise=float(params.noise)))
     params.biggestm = 0
     params.truncerr = params.maxtrunc
     params.maxtrunc = 0.0
@@ -194,15 +292,23 @@ def _sweeps(psi, mpo, params, stepfct, cvgfct, Lenv, Renv, Nsteps, j):
 
 
 def dmrg(psi, mpo, m=0, minm=2, sweeps=1, cutoff=0.0, silent=True, goal=0.0, method="twosite",
-         SvNbond=None, precision="f64", correction=None, params=None, env_precision=None):
+         SvNbond=None, precision="f64", correction=None, params=None, env_precision=None, r=None,
+         noise=None, noise_goal=None, noise_incr=None):
     """dmrg(psi, mpo; m, minm, sweeps, cutoff, silent, goal, method)
     (algs/DMRG.jl:549-589).  Optimises ``psi`` in place and returns the
     energy; the per-sweep report is in ``params.history`` when a params
-    object is passed."""
-    if method != "twosite":
+    object is passed.  method "3S" runs dmrg3S (r, noise, noise_goal,
+    noise_incr as in the reference's keywords)."""
+    if method not in ("twosite", "3S"):
         raise ValueError("DMRG method (%s) not defined on the GPU path" % method)
     if params is None:
         params = algvars(psi)
+    params.method = method
+    for name, val in (("r", r), ("noise", noise), ("noise_goal", noise_goal), ("noise_incr", noise_incr)):
+        if val is not None:
+            setattr(params, name, val)
+    if method == "3S" and precision != "f64":
+        raise ValueError("3S runs in FP64 on the GPU path")
     params.maxm, params.minm, params.sweeps, params.cutoff = m, minm, sweeps, cutoff
     params.silent, params.goal, params.precision = silent, goal, precision
     if precision not in ("f64", "bf16x3", "f16x3"):
@@ -213,4 +319,9 @@ def dmrg(psi, mpo, m=0, minm=2, sweeps=1, cutoff=0.0, silent=True, goal=0.0, met
     params.correction = correction if correction is not None else ("none" if precision == "f64" else "sweep")
     if SvNbond is not None:
         params.SvNbond = SvNbond
-    return optmps(psi, mpo, params)
+    return optmps(psi, mpo, params, stepfct=step3S if method == "3S" else twostep)
+
+
+def dmrg3S(psi, mpo, **kw):
+    """dmrg3S (algs/DMRG.jl:655-671)."""
+    return dmrg(psi, mpo, method="3S", **kw)

@@ -256,6 +256,29 @@ int dmrj_svd_trunc2(dmrj_ctx* ctx, const dmrj_tensor* AA, int64_t m, int64_t min
                     int dir, dmrj_tensor** psiL, dmrj_tensor** psiR, double* truncerr,
                     int64_t* kept, double* svals, int64_t svals_cap);
 
+/* ---- 3S: single-site DMRG with subspace expansion (algs/DMRG.jl:322-360) - */
+/* TensorPACK lanczos(psi[i], mpo[i]; r=krylov, m=1, updatefct=singlesite_update)
+ * as called at algs/DMRG.jl:328: the lowest Ritz pair of the `krylov`-vector
+ * Krylov space of A (same Krylov process as dmrj_krylov_expm1); out = Ritz
+ * vector, unit norm, signed so that its overlap with A is >= 0. */
+int dmrj_lanczos1(dmrj_ctx* ctx, const dmrj_tensor* L, const dmrj_tensor* R, const dmrj_tensor* A,
+                  const dmrj_mpo* W, int krylov, int precision, dmrj_tensor** out, double* energy,
+                  int* kdim);
+/* Lexpand (dir > 0, E = Lenv[i]) / Rexpand (dir < 0, E = Renv[i]) with
+ * joinindex! (algs/DMRG.jl:273-286): A joined with alpha times the expansion
+ * along the right (left) bond; every sector of the enlarged bond holds the
+ * original states first.  out is a two-site tensor with a trivial second
+ * (first) physical index, ready for dmrj_svd_trunc2(out, ..., dir).
+ * alpha == 0 gives A alone in that layout (moveR!/moveL! without condition). */
+int dmrj_expand3s(dmrj_ctx* ctx, const dmrj_tensor* E, const dmrj_tensor* A, const dmrj_mpo* W,
+                  double alpha, int dir, dmrj_tensor** out);
+/* The new centre of moveR!/moveL! with condition = true (lib/MPS/move.jl:
+ * 124-139,175-190): dir > 0: F = the D*V factor of dmrj_svd_trunc2, N =
+ * psi[i+1]; dir < 0: F = the U*D factor, N = psi[i-1]; F restricted to the
+ * original bond times N. */
+int dmrj_absorb3s(dmrj_ctx* ctx, const dmrj_tensor* F, const dmrj_tensor* N, int dir,
+                  dmrj_tensor** out);
+
 /* twostep (algs/DMRG.jl:362-405) without the unused twosite_update: normalise
  * the orthogonality centre (psiL if dir > 0 else psiR, in place), lanczos2,
  * svd_trunc2.  psiL/psiR are replaced by new handles (the old ones stay

@@ -149,12 +149,13 @@ def truncation_count(s_desc, m, minm, cutoff):
     return keep
 
 
-def fsvd(AA, m=0, minm=2, cutoff=0.0, qL=None, qP=None, qR=None):
+def fsvd(AA, m=0, minm=2, cutoff=0.0, qL=None, qP=None, qR=None, qP2=None):
     """Two-site SVD split AA[(a,s1),(s2,b)] = U diag(s) V with truncation.
 
     Returns U[a,s1,k], s[k], V[k,s2,b], truncerr, qnew (charges of the new
     bond, or None).  With charges the new bond is ordered by sector charge
     (ascending, lexicographic) and by descending sigma inside a sector.
+    qP charges s1 and, unless qP2 is given, s2 as well.
     """
     a, d1, d2, b = AA.shape
     M = AA.reshape(a * d1, d2 * b)
@@ -166,8 +167,9 @@ def fsvd(AA, m=0, minm=2, cutoff=0.0, qL=None, qP=None, qR=None):
         s = s / np.linalg.norm(s)
         return U.reshape(a, d1, keep), s, Vt.reshape(keep, d2, b), truncerr, None
     nq = qP.shape[1]
+    qP2 = qP if qP2 is None else qP2
     qrow = (qL[:, None, :] + qP[None, :, :]).reshape(a * d1, nq)
-    qcol = (qR[None, :, :] - qP[:, None, :]).reshape(d2 * b, nq)
+    qcol = (qR[None, :, :] - qP2[:, None, :]).reshape(d2 * b, nq)
     keys = sorted(set(map(tuple, qrow)) & set(map(tuple, qcol)))
     sectors = []
     allS, allSec, allLoc = [], [], []
