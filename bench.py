@@ -69,6 +69,8 @@ def main():
     ap.add_argument("--no-xcd-order", action="store_true", help="plain descending-work tile order")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_k_gemm_f64.json"),
                     help="HBM bytes per k_gemm_f64 launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE")
+    ap.add_argument("--method", default="twosite", choices=["twosite", "3S"],
+                    help="DMRG method of the timed sweeps (3S: single-site with subspace expansion, FP64)")
     ap.add_argument("--mode", default="dmrg", choices=["dmrg", "tdvp", "measure"],
                     help="dmrg: two-site DMRG sweeps (default); tdvp: real-time two-site TDVP half-sweeps of the "
                          "Heisenberg chain from the Neel state (ComplexF64); measure: after the ramp, time "
@@ -140,9 +142,9 @@ def main():
         desc = "Heisenberg L=%d d=2 D=5 U(1) Sz=0" % L
     params = pkg.algvars(psi)
 
-    def sweeps(mm, n):
+    def sweeps(mm, n, method="twosite"):
         pkg.dmrg(psi, mpo, m=mm, sweeps=n, cutoff=args.cutoff, params=params, precision=args.precision,
-                 env_precision=args.env_precision)
+                 env_precision=args.env_precision, method=method)
 
     t_ramp = time.perf_counter()
     for mr in [int(x) for x in args.ramp.split(",") if x]:
@@ -156,8 +158,14 @@ def main():
         sweeps(m, 1)
         log("grow m=%d E=%.12f maxm=%d t=%.2fs" % (m, params.energy, params.history[-1]["biggestm"],
                                                    params.history[-1]["time"]))
+    # the ramp grows the bonds with two-site sweeps; 3S is timed from there
+    # with its noise back at the initial 1.0, so that the subspace expansion
+    # is active (the noise shrinks by 1% per step while the energy falls and
+    # the expansion stops below noise_incr = 0.01, i.e. after ~460 steps)
+    if args.method == "3S":
+        params.noise = 1.0
     for _ in range(args.warmup):
-        sweeps(m, 1)
+        sweeps(m, 1, args.method)
         log("warmup m=%d E=%.12f maxm=%d t=%.2fs" % (m, params.energy, params.history[-1]["biggestm"],
                                                      params.history[-1]["time"]))
     log("ramp+warmup %.1fs" % (time.perf_counter() - t_ramp))
@@ -182,7 +190,7 @@ def main():
     ctx.svd_stats(reset=True)
     ctx.marker(1)       # kernel-trace cut: start of the timed region
     t0 = time.perf_counter()
-    sweeps(m, args.steps)
+    sweeps(m, args.steps, args.method)
     ctx.sync()
     t1 = time.perf_counter()
     ctx.marker(2)       # end of the timed region
@@ -213,7 +221,7 @@ def main():
                         stats["copy"]["ms"] / 1e3, stats["svd"]["ms"] / 1e3))
 
     traffic, traffic_note = None, None
-    if os.path.exists(args.traffic_file) and args.precision == "f64":
+    if os.path.exists(args.traffic_file) and args.precision == "f64" and args.method == "twosite":
         with open(args.traffic_file) as f:
             tr = json.load(f)
         if tr.get("L") == L and tr.get("m") == m and tr.get("kernel") == "k_gemm_f64":
@@ -222,9 +230,10 @@ def main():
                             "scaled by the copy-kernel calibration (%s)" % os.path.basename(args.traffic_file))
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(pkg, psi, mpo, params, args.cpu_seconds)
+        cpu = cpu_baseline(pkg, psi, mpo, params, args.cpu_seconds, args.method)
+    mname = {"twosite": "two-site", "3S": "3S single-site with subspace expansion"}[args.method]
     out = {
-        "metric": "DMRG sweep time + H_eff*psi TFLOP/s (FP64 GPU drop-in, two-site, U(1))",
+        "metric": "DMRG sweep time + H_eff*psi TFLOP/s (FP64 GPU drop-in, %s, U(1))" % mname,
         "value": round(tflops, 4),
         "unit": "TFLOP/s",
         "n_gpus": 1,
@@ -238,8 +247,10 @@ def main():
                   "f16x3": "f16x3 per-block scaled (FP32 accumulate, FP64 Lanczos scalars/SVD/energy)"}[
                       args.precision],
         "data": "synthetic (seeded random product state, Heisenberg MPO)",
-        "config": {"workload": "%s, two-site DMRG sweep at m=%d (ramp %s), Krylov 2, cutoff %g"
-                               % (desc, m, args.ramp, args.cutoff), "model": args.model,
+        "config": {"workload": "%s, %s DMRG sweep at m=%d (ramp %s), Krylov 2, cutoff %g"
+                               % (desc, mname, m, args.ramp, args.cutoff), "model": args.model,
+                   "method": args.method, "ramp_method": "twosite",
+                   "noise_at_end": params.noise if args.method == "3S" else None,
                    "L": L, "m": m, "energy": params.energy, "precision": args.precision,
                    "env_precision": args.env_precision or args.precision,
                    "svd_backend": args.svd_backend if args.svd_backend is not None else "default",
@@ -429,10 +440,11 @@ def bench_tdvp(pkg, ctx, args):
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(pkg, psi, mpo, params, budget_s):
+def cpu_baseline(pkg, psi, mpo, params, budget_s, method="twosite"):
     """Time the oracle's block-sparse H_eff*psi at the centre bond of the
-    current (m = 1024) state on the host cores.  The GPU state is first moved
-    to the centre (untimed bond steps) so that both environments are current."""
+    current (m = 1024) state on the host cores (the single-site form at the
+    centre site for 3S).  The GPU state is first moved to the centre (untimed
+    bond steps) so that both environments are current."""
     import multiprocessing
     from oracle import blocks as ob
     L = len(psi)
@@ -451,6 +463,11 @@ def cpu_baseline(pkg, psi, mpo, params, budget_s):
     theta = np.tensordot(A1, A2, axes=(2, 0))
     plan = ob.Heff2Plan(Lh, Rh, mpo.H[c - 1], mpo.H[c], ql, psi.qphys, qr, mpo.deltas[c - 1],
                         mpo.deltas[c + 1])
+    if method == "3S":
+        Rc, _, _ = params.Renv[c].to_dense(nq)
+        _, _, qc = psi[c].to_dense(nq)
+        theta = A1
+        plan = ob.Heff1Plan(Lh, Rc, mpo.H[c - 1], ql, psi.qphys, qc, mpo.deltas[c - 1], mpo.deltas[c])
     n = 0
     t0 = time.perf_counter()
     while True:
@@ -461,12 +478,13 @@ def cpu_baseline(pkg, psi, mpo, params, budget_s):
     dt = (time.perf_counter() - t0) / n
     tf = plan.flops / dt / 1e12
     cores = int(os.environ.get("OMP_NUM_THREADS", multiprocessing.cpu_count()))
-    log("cpu baseline: %d H_eff applications at bond %d|%d (chi %d x %d), %.3f s each, %.4f TFLOP/s"
-        % (n, c, c + 1, theta.shape[0], theta.shape[3], dt, tf))
+    form = "single-site H_eff*A at the centre site %d" % c if method == "3S" else \
+        "two-site H_eff*psi at the centre bond %d|%d" % (c, c + 1)
+    log("cpu baseline: %d %s applications (chi %d x %d), %.3f s each, %.4f TFLOP/s"
+        % (n, form, theta.shape[0], theta.shape[-1], dt, tf))
     return {"value": round(tf, 5), "unit": "TFLOP/s", "cores": cores, "kind": "port",
-            "sample": "%d block-sparse H_eff*psi applications at the centre bond %d|%d (chi %dx%d) of the "
-                      "m=%d state, numpy/OpenBLAS oracle, %.3f s each" % (n, c, c + 1, theta.shape[0],
-                                                                          theta.shape[3], params.maxm, dt)}
+            "sample": "%d block-sparse %s applications (chi %dx%d) of the m=%d state, numpy/OpenBLAS oracle, "
+                      "%.3f s each" % (n, form, theta.shape[0], theta.shape[-1], params.maxm, dt)}
 
 
 if __name__ == "__main__":

@@ -994,6 +994,7 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
     // largest singular vector of that side
     double* X = nullptr;
     int bj = 0, rowsX = 0, ncolsX = 0;
+    int qr = 0;  // one-factor sector solved through its LQ / QR triangle (X: K x K factor)
     std::vector<int> perm;
     std::vector<double> hn;
     std::vector<const double*> colsrc;
@@ -1279,6 +1280,10 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
   double* augE = nullptr;
   rocblas_int* auginfo = nullptr;
   int naug = 0;
+  // LQ / QR-reduced sectors of the default backend
+  double *qrX = nullptr, *qrT = nullptr, *qaug = nullptr, *qW = nullptr, *qE = nullptr, *qVt = nullptr;
+  rocblas_int* qinfo = nullptr;
+  int nqr = 0;
   if (z) {
     // complex: every sector through one batched Hermitian eigensolve of
     // [[0, M], [M^H, 0]] (eigenvalues +-sigma)
@@ -1314,13 +1319,78 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
       launch_aug_extract_z(c, jobs);
     }
   } else if (c.svd_backend == 0) {
-    std::vector<int> big;
-    int N = 0;
+    std::vector<int> big, qrs;
+    int N = 0, Kq = 0, Lq = 0;
     for (int si : order) {
       Sec& s = secs[si];
       if (s.kmin == 0 || is_small[si]) continue;
-      big.push_back(si);
-      N = std::max<int>(N, (int)(s.R + s.C));
+      // short / long side, the short one being the side that must come out
+      // orthonormal (U for dir > 0, V for dir < 0)
+      const int sh = (int)(dir > 0 ? s.R : s.C), lg = (int)(dir > 0 ? s.C : s.R);
+      if (2 * (int64_t)lg >= 3 * (int64_t)sh) {
+        qrs.push_back(si);
+        Kq = std::max(Kq, sh);
+        Lq = std::max(Lq, lg);
+      } else {
+        big.push_back(si);
+        N = std::max<int>(N, (int)(s.R + s.C));
+      }
+    }
+    // Sectors at least 1.5 times longer than the side that must come out
+    // orthonormal (the enlarged tensors of 3S): Householder LQ (dir > 0,
+    // M = L Q: U(M) = U(L)) or QR (dir < 0, M = Q R: V(M) = V(R) = U(R^T))
+    // of the sector first, then the augmented eigensolve of the K x K
+    // triangle B = L or R^T at size 2K instead of R + C.  Backward stable like
+    // the full augmented form (sigma to eps |M| absolute); the other factor
+    // comes from the grouped GEMM with M below, as for the one-factor backends.
+    if (!qrs.empty()) {
+      nqr = (int)qrs.size();
+      const int ldq = dir > 0 ? Kq : Lq;
+      const int64_t QS = (int64_t)Kq * Lq;
+      qrX = (double*)c.alloc.alloc(QS * nqr * sizeof(double));
+      qrT = (double*)c.alloc.alloc((int64_t)Kq * nqr * sizeof(double));
+      launch_fill(c, qrX, QS * nqr, 0.0);
+      std::vector<CopyOp> place;
+      for (int b = 0; b < nqr; ++b) {
+        Sec& s = secs[qrs[b]];
+        place.push_back({s.M, qrX + b * QS, s.R, ldq, (int)s.R, (int)s.C, nullptr, nullptr, 1.0});
+      }
+      launch_copy(c, place);
+      rocblas_status st = dir > 0
+                              ? rocsolver_dgelqf_strided_batched(hb, Kq, Lq, qrX, Kq, QS, qrT, Kq, nqr)
+                              : rocsolver_dgeqrf_strided_batched(hb, Lq, Kq, qrX, Lq, QS, qrT, Kq, nqr);
+      if (st != rocblas_status_success) throw Error(DMRJ_ERR_SOLVER, "rocsolver LQ/QR (strided batched) failed");
+      const int Nq = 2 * Kq;
+      const int64_t NNq = (int64_t)Nq * Nq;
+      qaug = (double*)c.alloc.alloc(NNq * nqr * sizeof(double));
+      qW = (double*)c.alloc.alloc((int64_t)Nq * nqr * sizeof(double));
+      qE = (double*)c.alloc.alloc((int64_t)Nq * nqr * sizeof(double));
+      qVt = (double*)c.alloc.alloc((int64_t)Kq * Kq * nqr * sizeof(double));
+      qinfo = (rocblas_int*)c.alloc.alloc(sizeof(rocblas_int) * nqr);
+      launch_fill(c, qaug, NNq * nqr, 0.0);
+      std::vector<TriJob> tj;
+      for (int b = 0; b < nqr; ++b) {
+        Sec& s = secs[qrs[b]];
+        tj.push_back({qrX + b * QS, qaug + b * NNq, ldq, s.kmin, Nq, dir > 0 ? 0 : 1});
+      }
+      launch_tri_place(c, tj);
+      st = rocsolver_dsyevd_strided_batched(hb, rocblas_evect_original, rocblas_fill_upper, Nq, qaug, Nq, NNq, qW, Nq,
+                                            qE, Nq, qinfo, nqr);
+      if (st != rocblas_status_success) throw Error(DMRJ_ERR_SOLVER, "rocsolver_dsyevd_strided_batched failed");
+      std::vector<AugJob> jobs;
+      for (int b = 0; b < nqr; ++b) {
+        Sec& s = secs[qrs[b]];
+        const int K = s.kmin;
+        s.X = (double*)c.alloc.alloc((int64_t)K * K * sizeof(double));
+        jobs.push_back({qaug + b * NNq, qW + (int64_t)b * Nq, s.X, s.s, qVt + (int64_t)b * Kq * Kq, K, K, Nq, K});
+        s.bj = 1;
+        s.qr = 1;
+        s.rowsX = K;
+        is_small[qrs[b]] = 1;  // handled
+        const double kk = (double)K, ll = (double)(dir > 0 ? s.C : s.R), n2 = 2.0 * kk;
+        svd_flops += 2.0 * ll * kk * kk - 2.0 / 3.0 * kk * kk * kk + 9.0 * n2 * n2 * n2;
+      }
+      launch_aug_extract(c, jobs);
     }
     if (!big.empty()) {
       naug = (int)big.size();
@@ -1377,7 +1447,7 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
     for (int si : order) {
       Sec& s = secs[si];
       if (s.kmin == 0) continue;
-      if (s.bj) {
+      if (s.bj && !s.qr) {
         // column norms already on the host (block-Jacobi loop)
       } else {
         HIP_OK(hipMemcpyAsync(hs.data() + o, s.s, s.kmin * sizeof(double), hipMemcpyDeviceToHost, c.stream));
@@ -1397,13 +1467,20 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
   std::vector<rocblas_int> hainfo(std::max(1, naug));
   if (naug)
     HIP_OK(hipMemcpyAsync(hainfo.data(), auginfo, naug * sizeof(rocblas_int), hipMemcpyDeviceToHost, c.stream));
+  std::vector<rocblas_int> hqinfo(std::max(1, nqr));
+  if (nqr)
+    HIP_OK(hipMemcpyAsync(hqinfo.data(), qinfo, nqr * sizeof(rocblas_int), hipMemcpyDeviceToHost, c.stream));
   HIP_OK(hipStreamSynchronize(c.stream));
   c.alloc.release(aug);
   c.alloc.release(augW);
   c.alloc.release(augE);
   c.alloc.release(auginfo);
+  for (double* p : {qrX, qrT, qaug, qW, qE, qVt}) c.alloc.release(p);
+  c.alloc.release(qinfo);
   for (int i = 0; i < naug; ++i)
     if (hainfo[i] != 0) throw Error(DMRJ_ERR_SOLVER, "rocsolver symmetric eigensolver did not converge");
+  for (int i = 0; i < nqr; ++i)
+    if (hqinfo[i] != 0) throw Error(DMRJ_ERR_SOLVER, "rocsolver symmetric eigensolver (LQ/QR triangle) did not converge");
   c.release_transients();
   c.stage_reset();
   for (int i = 0; i < nsvd; ++i)
@@ -1428,6 +1505,13 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
           const double lam = hgram[(size_t)b * gramN + gramN - 1 - l];
           hs[o + l] = std::sqrt(std::max(lam, 0.0));
           s.colsrc.push_back(gramZ + b * NN + (int64_t)(gramN - 1 - l) * gramN);
+          s.coldiv.push_back(1.0);
+        }
+      } else if (s.qr) {
+        // LQ / QR-reduced sector: sigma descending from the extraction, the
+        // orthonormal factor's vectors are the columns of X (K x K)
+        for (int l = 0; l < s.kmin; ++l) {
+          s.colsrc.push_back(s.X + (int64_t)l * s.kmin);
           s.coldiv.push_back(1.0);
         }
       } else if (s.bj) {

@@ -322,6 +322,13 @@ void launch_bj_norms(Context& c, const BjJob* djobs, const std::vector<int2>& wo
 void launch_bj_gather(Context& c, const std::vector<BjGather>& g);
 void launch_transpose(Context& c, const double* M, int R, int C, double* X);
 void launch_pad_diag(Context& c, double* Z, int N, const std::vector<int2>& jobs, double v);
+struct TriJob {
+  const double* X;  // LQ / QR output, leading dimension ldx
+  double* A;        // N x N augmented matrix of this sector
+  int64_t ldx;
+  int K, N, trans;  // trans: B = R^T of a QR (else B = L of an LQ)
+};
+void launch_tri_place(Context& c, const std::vector<TriJob>& jobs);
 struct CompleteJob {
   double* Q;          // canonical factor: element r of vector c at Q[c vs + r es]
   int64_t vs, es;

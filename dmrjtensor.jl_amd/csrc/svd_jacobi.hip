@@ -223,6 +223,38 @@ void launch_pad_diag(Context& c, double* Z, int N, const std::vector<int2>& jobs
   HIP_OK(hipGetLastError());
 }
 
+// Lower-triangular K x K matrix B written into columns [K, 2K), rows [0, K)
+// of an N x N augmented matrix (the block the upper-triangle eigensolver
+// reads): B[r][c] = X[r + c ldx] (the L of an LQ factorisation) or, with
+// trans, X[c + r ldx] (the transpose of the R of a QR factorisation), zero
+// above the diagonal (where the factorisation keeps its reflectors).
+// One work item per (job, column c).
+__global__ void k_tri_place(const TriJob* __restrict__ jobs, const int2* __restrict__ work, int nwork) {
+  for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+    const int2 w = work[wi];
+    const TriJob jb = jobs[w.x];
+    const int c = w.y;
+    double* dst = jb.A + (int64_t)(jb.K + c) * jb.N;
+    for (int r = threadIdx.x; r < jb.K; r += blockDim.x) {
+      double v = 0.0;
+      if (r >= c) v = jb.trans ? jb.X[c + (int64_t)r * jb.ldx] : jb.X[r + (int64_t)c * jb.ldx];
+      dst[r] = v;
+    }
+  }
+}
+
+void launch_tri_place(Context& c, const std::vector<TriJob>& jobs) {
+  if (jobs.empty()) return;
+  std::vector<int2> work;
+  for (size_t j = 0; j < jobs.size(); ++j)
+    for (int col = 0; col < jobs[j].K; ++col) work.push_back(make_int2((int)j, col));
+  auto* dj = (TriJob*)c.upload(jobs.data(), jobs.size() * sizeof(TriJob));
+  auto* dw = (int2*)c.upload(work.data(), work.size() * sizeof(int2));
+  hipLaunchKernelGGL(k_tri_place, dim3((unsigned)std::min<size_t>(work.size(), (size_t)c.num_cu * 8)),
+                     dim3(kBjThreads), 0, c.stream, dj, dw, (int)work.size());
+  HIP_OK(hipGetLastError());
+}
+
 // Orthonormal completion of a kept canonical factor.  Singular vectors of
 // numerically zero singular values are not determined to working precision
 // by any of the solvers here (the two halves of an augmented eigenvector of

@@ -117,3 +117,70 @@ class Heff2Plan:
                 continue
             y[self.sL[ap], k1, k2, self.sR[bp]] += blk @ R[self.sR[b], u, self.sR[bp]]
         return y
+
+
+class Heff1Plan:
+    """y = H_1site A (algs/optimizeMPS.jl:503-511) by charge sectors: one dense
+    GEMM per allowed pair of sectors (the single-site form that 3S applies)."""
+
+    def __init__(self, L, R, W, qL, qP, qR, dL, dR):
+        self.L, self.R, self.W = L, R, W
+        self.sL, self.sR = sectors(qL), sectors(qR)
+        self.qP = [tuple(int(x) for x in r) for r in np.asarray(qP)]
+        self.dL = [tuple(int(x) for x in r) for r in np.asarray(dL)]
+        self.dR = [tuple(int(x) for x in r) for r in np.asarray(dR)]
+        self.d = W.shape[1]
+        self.nzW = [tuple(int(i) for i in ix) for ix in np.argwhere(W != 0.0)]
+        f = 0.0
+        for (a, j, b) in self._blocks():
+            na, nb = self._n(self.sL, a), self._n(self.sR, b)
+            for w in range(W.shape[0]):
+                ap = _add(a, self.dL[w])
+                if ap in self.sL:
+                    f += 2.0 * self._n(self.sL, ap) * na * nb
+        for ap in self.sL:
+            for k in range(self.d):
+                bp = _add(ap, self.qP[k])
+                if bp not in self.sR:
+                    continue
+                for u in range(W.shape[3]):
+                    b = _sub(bp, self.dR[u])
+                    if b in self.sR:
+                        f += 2.0 * self._n(self.sL, ap) * self._n(self.sR, b) * self._n(self.sR, bp)
+        self.flops = f
+
+    @staticmethod
+    def _n(s, key):
+        return s[key].stop - s[key].start
+
+    def _blocks(self):
+        for a in self.sL:
+            for j in range(self.d):
+                b = _add(a, self.qP[j])
+                if b in self.sR:
+                    yield a, j, b
+
+    def apply(self, A):
+        L, R, W = self.L, self.R, self.W
+        T1 = {}
+        for (a, j, b) in self._blocks():
+            blk = A[self.sL[a], j, self.sR[b]]
+            for w in range(W.shape[0]):
+                ap = _add(a, self.dL[w])
+                if ap in self.sL:
+                    T1.setdefault((w, j), []).append((ap, b, L[self.sL[ap], w, self.sL[a]] @ blk))
+        T3 = {}
+        for (w, j, k, u) in self.nzW:
+            c = W[w, j, k, u]
+            for (ap, b, blk) in T1.get((w, j), ()):
+                key = (ap, k, u, b)
+                if key in T3:
+                    T3[key] += c * blk
+                else:
+                    T3[key] = c * blk
+        y = np.zeros_like(A)
+        for (ap, k, u, b), blk in T3.items():
+            bp = _add(b, self.dR[u])
+            if bp in self.sR:
+                y[self.sL[ap], k, self.sR[bp]] += blk @ R[self.sR[b], u, self.sR[bp]]
+        return y

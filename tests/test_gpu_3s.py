@@ -163,6 +163,30 @@ def test_dmrg3s_sweeps_match_oracle(pkg, ctx, qn):
     assert abs(E - oE) <= TOL * abs(oE)
 
 
+@pytest.mark.parametrize("L,m,n", [(24, 32, 6), (20, 160, 4)], ids=["L24m32", "L20m160"])
+def test_dmrg3s_u1_longer_runs_match_oracle(pkg, ctx, L, m, n):
+    """Longer 3S runs with many small singular values kept (cutoff 1e-12):
+    per-sweep energies against the oracle.  At m = 160 the enlarged sectors
+    exceed 64 on both sides, so the large-sector SVD path runs."""
+    say("case 3S sweeps U(1) L=%d m=%d" % (L, m))
+    states = [i % 2 for i in range(L)]
+    qphys = np.array([[1], [-1]])
+    mpo_h = om.makeMPO(om.heisenbergMPO(), 2, L)
+    opsi = omps.product_mps(states, 2, qphys=qphys)
+    oE, ohist = o3.dmrg3S(opsi, mpo_h, m=m, sweeps=n, cutoff=1e-12)
+    psi = pkg.product_state(states, 2, qphys=qphys, ctx=ctx)
+    params = pkg.algvars(psi)
+    pkg.dmrg(psi, pkg.MPO(mpo_h, qphys=qphys), m=m, sweeps=n, cutoff=1e-12, method="3S", params=params)
+    for h, oh in zip(params.history, ohist):
+        rel = abs(h["energy"] - oh["energy"]) / abs(oh["energy"])
+        say(" sweep", h["sweep"], "%.15f" % h["energy"], "%.15f" % oh["energy"], "rel %.2e" % rel,
+            "m", h["biggestm"], oh["biggestm"], "maxtrunc %.2e %.2e" % (h["maxtrunc"], oh["maxtrunc"]),
+            "noise %.6f %.6f" % (h["noise"], oh["noise"]), "t=%.3fs" % h["time"])
+        assert rel <= TOL
+        assert h["biggestm"] == oh["biggestm"]
+    assert len(params.history) == len(ohist)
+
+
 def test_dmrg3s_dense_vs_qn_reference_criteria(pkg, ctx):
     """test/dmrg_test.jl:7,82-85 with method 3S: spins (Heisenberg) and
     fermions (Hubbard U(1) x U(1), half filling): dense and quantum-number

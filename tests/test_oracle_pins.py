@@ -249,6 +249,12 @@ def test_block_sparse_heff_matches_dense():
     want = od.heff2(Lenv, Renv, th, mpo[1], mpo[2])
     assert np.allclose(y, want, rtol=1e-12, atol=1e-12 * np.max(np.abs(want)))
     assert plan.flops > 0
+    # single-site form at site 2
+    R2 = od.Rupdate(Renv, A[2], mpo[2])
+    p1 = ob.Heff1Plan(Lenv, R2, mpo[1], qb[1], qphys, qb[2], deltas[1], deltas[2])
+    want = od.singlesite_update(Lenv, R2, A[1], mpo[1])
+    assert np.allclose(p1.apply(A[1]), want, rtol=1e-12, atol=1e-12 * np.max(np.abs(want)))
+    assert p1.flops > 0
 
 
 def test_krylov_expm_matches_dense_expm():
