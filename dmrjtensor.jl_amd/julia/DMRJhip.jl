@@ -3,6 +3,7 @@
 #  Keeps the reference's API for the two-site path (bakerte/DMRJtensor.jl):
 #    dmrg(psi, mpo; m, minm, sweeps, cutoff, silent, goal, method)  algs/DMRG.jl:549
 #    twostep / optmps / dmrgcvg / SvNcheck!                         algs/DMRG.jl:362,610,172
+#    step3S / alpha_update (dmrg(...; method="3S"))                  algs/DMRG.jl:322,288
 #                                                                   algs/optimizeMPS.jl:759
 #    Lupdate / Rupdate / makeEnv                                    lib/MPO_MPS/Lupdate.jl:17,
 #                                                                   Rupdate.jl:17, lib/Env/makeEnv.jl:196
@@ -14,7 +15,7 @@
 #  shared library (dmrjtensor.jl_amd/lib/libdmrjhip.so by default).
 module DMRJhip
 
-export gpuMPS, gpuMPO, gpuEnv, dmrg, makeEnv, Lupdate, Rupdate, expect, twostep, tdvp, free!
+export gpuMPS, gpuMPO, gpuEnv, dmrg, makeEnv, Lupdate, Rupdate, expect, twostep, step3S, tdvp, free!
 
 const libdmrj = get(ENV, "DMRJHIP_LIB", joinpath(@__DIR__, "..", "lib", "libdmrjhip.so"))
 
@@ -235,6 +236,11 @@ mutable struct algvars
   biggestm::Int
   entropy::Float64
   storeD::Vector{Float64}
+  # 3S: Krylov vectors per site and the subspace-expansion noise control
+  r::Int
+  noise::Float64
+  noise_goal::Float64
+  noise_incr::Float64
 end
 
 "twostep (algs/DMRG.jl:362-405) as one dmrj_twostep call"
@@ -266,14 +272,91 @@ function twostep(n, j, i, iL, iR, psi::gpuMPS, mpo::gpuMPO, Lenv, Renv, params::
   nothing
 end
 
-"dmrg (algs/DMRG.jl:549-589), method = \"twosite\" only"
+"noise control of 3S (algs/DMRG.jl:288-303); x/0 follows IEEE as in the reference"
+function alpha_update(alpha, truncerr, cutoff, lastE, currE, goal, dalpha)
+  small_trunc = truncerr <= max(cutoff, 1e-12)
+  small_step = abs(currE - lastE) <= max(abs(currE * cutoff), 1e-12)
+  site = abs((truncerr * currE) / (lastE / currE - 1)) > goal
+  alpha *= (currE <= lastE && (site || small_step || small_trunc)) ? 1 - dalpha : 1 + dalpha
+  return max(min(alpha, 1.0), 1e-42)
+end
+
+"step3S (algs/DMRG.jl:322-360): lanczos1, expansion, truncated SVD, restricted absorb"
+function step3S(n, j, i, iL, iR, psi::gpuMPS, mpo::gpuMPO, Lenv, Renv, params::algvars; correct::Bool=false)
+  ctx = context()
+  A = Ref{Ptr{Cvoid}}(C_NULL); e = Ref{Float64}(0.0); k = Ref{Cint}(0)
+  rc = ccall((:dmrj_lanczos1, libdmrj), Cint,
+             (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Ref{Ptr{Cvoid}}, Ref{Float64},
+              Ref{Cint}), ctx.h, Lenv[i].h, Renv[i].h, psi[i].h, mpo[i], params.r, 0, A, e, k)
+  check(ctx, rc, "dmrj_lanczos1")
+  noise = params.noise
+  alpha = noise > params.noise_incr ? noise : 0.0
+  dir = j > 0 ? Cint(1) : Cint(-1)
+  T = Ref{Ptr{Cvoid}}(C_NULL)
+  rc = ccall((:dmrj_expand3s, libdmrj), Cint,
+             (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Float64, Cint, Ref{Ptr{Cvoid}}),
+             ctx.h, j > 0 ? Lenv[iL].h : Renv[iR].h, A[], mpo[i], alpha, dir, T)
+  check(ctx, rc, "dmrj_expand3s")
+  ccall((:dmrj_tensor_free, libdmrj), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), ctx.h, A[])
+  check(ctx, ccall((:dmrj_normalize, libdmrj), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), ctx.h, T[]), "dmrj_normalize")
+  cap = max(8, 4 * (params.maxm > 0 ? params.maxm : 4096))
+  sv = zeros(Float64, cap)
+  nl = Ref{Ptr{Cvoid}}(C_NULL); nr = Ref{Ptr{Cvoid}}(C_NULL)
+  te = Ref{Float64}(0.0); kept = Ref{Int64}(0)
+  rc = ccall((:dmrj_svd_trunc2, libdmrj), Cint,
+             (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64, Float64, Cint, Ref{Ptr{Cvoid}}, Ref{Ptr{Cvoid}}, Ref{Float64},
+              Ref{Int64}, Ptr{Float64}, Int64),
+             ctx.h, T[], params.maxm, params.minm, params.cutoff, dir, nl, nr, te, kept, sv, cap)
+  check(ctx, rc, "dmrj_svd_trunc2")
+  ccall((:dmrj_tensor_free, libdmrj), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), ctx.h, T[])
+  out = Ref{Ptr{Cvoid}}(C_NULL)
+  F, N = j > 0 ? (nr[], psi[iR].h) : (nl[], psi[iL].h)
+  rc = ccall((:dmrj_absorb3s, libdmrj), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ref{Ptr{Cvoid}}),
+             ctx.h, F, N, dir, out)
+  check(ctx, rc, "dmrj_absorb3s")
+  ccall((:dmrj_tensor_free, libdmrj), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), ctx.h, F)
+  free!(psi[iL]); free!(psi[iR])
+  if j > 0
+    psi[iL] = gpuTens(nl[]); psi[iR] = gpuTens(out[])
+  else
+    psi[iL] = gpuTens(out[]); psi[iR] = gpuTens(nr[])
+  end
+  params.energy = e[]
+  params.noise = alpha_update(noise, te[], params.cutoff, params.lastenergy, params.energy, params.noise_goal,
+                              params.noise_incr)
+  params.truncerr = te[]
+  params.maxtrunc = max(params.maxtrunc, te[])
+  params.biggestm = max(params.biggestm, kept[])
+  if i == min(max(2, params.SvNbond + 1), length(psi)) && j < 0
+    D = sort(sv[1:min(cap, kept[])], rev=true)
+    params.entropy = -sum(x -> x > 0 ? x^2 * log(x^2) : 0.0, D)
+    params.storeD = D[1:min(8, length(D))]
+  end
+  nothing
+end
+
+"dmrg (algs/DMRG.jl:549-589), method = \"twosite\" or \"3S\""
 function dmrg(psi::gpuMPS, mpo::gpuMPO; m::Integer=0, minm::Integer=2, sweeps::Integer=1, cutoff::Float64=0.,
               silent::Bool=false, goal::Float64=0., method::String="twosite", precision::String="f64",
-              qtot=nothing)
-  method == "twosite" || error("DMRG method ($method) not defined on the GPU path")
+              qtot=nothing, r::Integer=2, noise::Float64=1.0, noise_goal::Float64=0.3, noise_incr::Float64=0.01)
+  method in ("twosite", "3S") || error("DMRG method ($method) not defined on the GPU path")
+  stepfct = method == "3S" ? step3S : twostep
   Ns = length(psi)
-  params = algvars(m, minm, sweeps, cutoff, goal, silent, cld(Ns, 2), precision, 0., 0., 0., 0., 0, 0., zeros(8))
+  params = algvars(m, minm, sweeps, cutoff, goal, silent, cld(Ns, 2), precision, 0., 0., 0., 0., 0, 0., zeros(8),
+                   r, noise, noise_goal, noise_incr)
   Lenv, Renv = makeEnv(psi, mpo; qtot=qtot)
+  # startval (algs/optimizeMPS.jl:255-258): <psi_oc|H_1site psi_oc> opens the energy record
+  ctx = context()
+  H1 = Ref{Ptr{Cvoid}}(C_NULL); e0 = Ref{Float64}(0.0)
+  oc = psi.oc
+  rc = ccall((:dmrj_heff1_apply, libdmrj), Cint,
+             (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ref{Ptr{Cvoid}}),
+             ctx.h, Lenv[oc].h, mpo[oc], Renv[oc].h, psi[oc].h, 0, H1)
+  check(ctx, rc, "dmrj_heff1_apply")
+  check(ctx, ccall((:dmrj_dot, libdmrj), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{Float64}),
+                   ctx.h, psi[oc].h, H1[], e0), "dmrj_dot")
+  ccall((:dmrj_tensor_free, libdmrj), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), ctx.h, H1[])
+  params.energy = params.lastenergy = e0[]
   ccall((:dmrj_ctx_set_env_precision, libdmrj), Cint, (Ptr{Cvoid}, Cint), context().h, PREC[precision])
   j = psi.oc == Ns ? -1 : 1
   Nsteps = 2 * (Ns - 1)
@@ -282,7 +365,7 @@ function dmrg(psi::gpuMPS, mpo::gpuMPO; m::Integer=0, minm::Integer=2, sweeps::I
     for ns = 1:Nsteps
       i = psi.oc
       iL, iR = j > 0 ? (i, i + 1) : (i - 1, i)
-      twostep(n, j, i, iL, iR, psi, mpo, Lenv, Renv, params; correct=(ns == Nsteps))
+      stepfct(n, j, i, iL, iR, psi, mpo, Lenv, Renv, params; correct=(ns == Nsteps))
       if j > 0
         Lenv[i+1] = Lupdate(Lenv[i], psi[i], mpo[i])
       else
