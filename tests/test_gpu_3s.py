@@ -187,6 +187,59 @@ def test_dmrg3s_u1_longer_runs_match_oracle(pkg, ctx, L, m, n):
     assert len(params.history) == len(ohist)
 
 
+def test_dmrg3s_hubbard_u1xu1_matches_oracle(pkg, ctx):
+    """U(1) x U(1) Hubbard L=8 at half filling (two charges, fermion signs in
+    the state), m = 24: per-sweep energies and noise against the oracle."""
+    say("case 3S Hubbard U(1)xU(1) L=8 m=24")
+    L = 8
+    Cup, Cdn, F, Nup, Ndn, Ndens, O, Id = om.fermionOps()
+    qh = np.array([[0, 0], [1, 1], [1, -1], [2, 0]])
+    hmpo = om.makeMPO(om.hubbardMPO(), 4, L)
+    o = omps.product_mps([0] * L, 4)
+    omps.apply_ops(o, list(range(1, 5)), Cup.T, trail=F)
+    omps.apply_ops(o, list(range(1, 5)), Cdn.T, trail=F)
+    omps.product_charges(o, qh)
+    psi = pkg.MPS([a.copy() for a in o.A], oc=1, qphys=qh, qbond=o.qbond, ctx=ctx, canonical=True)
+    oE, ohist = o3.dmrg3S(o, hmpo, m=24, sweeps=4, cutoff=1e-12)
+    params = pkg.algvars(psi)
+    pkg.dmrg(psi, pkg.MPO(hmpo, qphys=qh), m=24, sweeps=4, cutoff=1e-12, method="3S", params=params)
+    for h, oh in zip(params.history, ohist):
+        rel = abs(h["energy"] - oh["energy"]) / abs(oh["energy"])
+        say(" sweep", h["sweep"], "%.15f" % h["energy"], "%.15f" % oh["energy"], "rel %.2e" % rel,
+            "m", h["biggestm"], oh["biggestm"], "noise %.6f %.6f" % (h["noise"], oh["noise"]))
+        assert rel <= TOL
+        assert h["biggestm"] == oh["biggestm"]
+        assert h["noise"] == pytest.approx(oh["noise"], rel=1e-12)
+    assert len(params.history) == len(ohist)
+
+
+def test_dmrg3s_resumed_from_checkpoint(pkg, ctx, tmp_path):
+    """3S for 2 sweeps, checkpoint (the noise and the last energy go into the
+    archive's extra data), load, 2 more sweeps == 4 sweeps in one run."""
+    say("case 3S resume")
+    L = 16
+    qphys = np.array([[1], [-1]])
+    mpo = pkg.MPO(om.makeMPO(om.heisenbergMPO(), 2, L), qphys=qphys)
+    states = [i % 2 for i in range(L)]
+    full = pkg.algvars(pkg.product_state(states, 2, qphys=qphys, ctx=ctx))
+    pkg.dmrg(pkg.product_state(states, 2, qphys=qphys, ctx=ctx), mpo, m=24, sweeps=4, cutoff=1e-12,
+             method="3S", params=full)
+    psi = pkg.product_state(states, 2, qphys=qphys, ctx=ctx)
+    first = pkg.algvars(psi)
+    pkg.dmrg(psi, mpo, m=24, sweeps=2, cutoff=1e-12, method="3S", params=first)
+    p = str(tmp_path / "r3s.npz")
+    pkg.save_mps(psi, p, extra={"noise": first.noise, "energy": first.energy})
+    back, extra = pkg.load_mps(p, ctx)
+    second = pkg.algvars(back)
+    second.noise = extra["noise"]
+    second.energy = extra["energy"]
+    pkg.dmrg(back, mpo, m=24, sweeps=2, cutoff=1e-12, method="3S", params=second)
+    got = [h["energy"] for h in first.history] + [h["energy"] for h in second.history]
+    want = [h["energy"] for h in full.history]
+    say(" energies", ["%.15f" % e for e in got], "uninterrupted", ["%.15f" % e for e in want])
+    assert got == want
+
+
 def test_dmrg3s_dense_vs_qn_reference_criteria(pkg, ctx):
     """test/dmrg_test.jl:7,82-85 with method 3S: spins (Heisenberg) and
     fermions (Hubbard U(1) x U(1), half filling): dense and quantum-number
