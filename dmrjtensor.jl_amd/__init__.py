@@ -13,7 +13,8 @@ from .models import fermionOps, heisenbergMPO, hubbardMPO, j1j2_bulk, spinOps, X
 from .network import (MPO, MPS, DeviceTensor, Lupdate, Rupdate, environment, expect, makeEnv,
                       makeMPO, makeq, mpo_deltas, product_state, randMPS_normal, site2_from_dense,
                       site_from_dense)
-from .dmrg import (absorb3s, algvars, alpha_update, dmrg, dmrg3S, expand3s, lanczos1, optmps, singlesite_update,
+from .dmrg import (absorb3s, algvars, alpha_update, dmrg, dmrg3S, expand3s, lanczos1, lanczos_k, optmps,
+                   singlesite_update,
                    startval, step3S, twostep)
 from .tdvp import krylovTimeEvol1, krylovTimeEvol2, local_expect, tdvp_twosite
 from .measure import correlationmatrix
@@ -25,7 +26,7 @@ __all__ = [
     "j1j2_bulk", "spinOps", "XXZ", "MPO", "MPS", "DeviceTensor", "Lupdate", "Rupdate",
     "environment", "expect", "makeEnv", "makeMPO", "makeq", "mpo_deltas", "product_state",
     "randMPS_normal", "site2_from_dense", "site_from_dense", "algvars", "dmrg", "optmps",
-    "singlesite_update", "startval", "twostep", "dmrg3S", "step3S", "lanczos1", "expand3s", "absorb3s",
+    "singlesite_update", "startval", "twostep", "dmrg3S", "step3S", "lanczos1", "lanczos_k", "expand3s", "absorb3s",
     "alpha_update", "krylovTimeEvol1", "krylovTimeEvol2", "local_expect",
     "tdvp_twosite", "correlationmatrix", "save_mps", "load_mps", "autoMPO", "mpoterm",
 ]

@@ -74,6 +74,7 @@ SIGNATURES = {
     "dmrj_krylov_expm1": [_p, _p, _p, _p, _p, C.c_int, C.c_double, C.c_double, C.c_int, C.POINTER(_p), _dp,
                           C.POINTER(C.c_int)],
     "dmrj_lanczos1": [_p, _p, _p, _p, _p, C.c_int, C.c_int, C.POINTER(_p), _dp, C.POINTER(C.c_int)],
+    "dmrj_lanczos_k": [_p, _p, _p, _p, _p, _p, C.c_int, C.c_int, C.POINTER(_p), _dp, C.POINTER(C.c_int)],
     "dmrj_expand3s": [_p, _p, _p, _p, C.c_double, C.c_int, C.POINTER(_p)],
     "dmrj_absorb3s": [_p, _p, _p, C.c_int, C.POINTER(_p)],
     "dmrj_svd_trunc2": [_p, _p, _i64, _i64, C.c_double, C.c_int, C.POINTER(_p), C.POINTER(_p), _dp,

@@ -3048,6 +3048,18 @@ int dmrj_lanczos1(dmrj_ctx* c, const dmrj_tensor* L, const dmrj_tensor* R, const
   });
 }
 
+int dmrj_lanczos_k(dmrj_ctx* c, const dmrj_tensor* L, const dmrj_tensor* R, const dmrj_tensor* x, const dmrj_mpo* W1,
+                   const dmrj_mpo* W2, int krylov, int precision, dmrj_tensor** out, double* energy, int* kdim) {
+  return guard(c, [&] {
+    if (!L || !R || !x || !W1 || !out || L->kind != 1 || R->kind != 2 || x->kind != 0)
+      throw Error(DMRJ_ERR_ARG, "lanczos_k: bad handles");
+    const bool two = x->site.d1 != 0;
+    if (two && !W2) throw Error(DMRJ_ERR_ARG, "lanczos_k: a two-site tensor needs the second MPO tensor");
+    *out = krylov_expm(*c, *L, *R, *x, two ? fuse_two(*W1, *W2) : fuse_one(*W1), krylov, 0.0, 0.0,
+                       base_prec(precision), energy, kdim, two ? 1 : 2, true);
+  });
+}
+
 int dmrj_expand3s(dmrj_ctx* c, const dmrj_tensor* E, const dmrj_tensor* A, const dmrj_mpo* W, double alpha, int dir,
                   dmrj_tensor** out) {
   return guard(c, [&] {

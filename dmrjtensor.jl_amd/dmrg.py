@@ -146,6 +146,21 @@ def lanczos1(Lenv, Renv, A, mpo, i, r, precision="f64"):
     return DeviceTensor(ctx, h), e.value
 
 
+def lanczos_k(Lenv, Renv, x, mpo, i, r, precision="f64"):
+    """k-vector Lanczos ground state (lowest Ritz pair of the r-vector Krylov
+    space) of H_eff on a one-site tensor at site i or a two-site tensor at
+    the bond (i, i+1): simplelanczos (algs/DMRG.jl:57-90) with r vectors.
+    Returns (Ritz vector, energy, Krylov dimension used)."""
+    ctx = x.ctx
+    h = C.c_void_p()
+    e = C.c_double()
+    k = C.c_int()
+    two = x.kind == 1
+    ctx.call("dmrj_lanczos_k", Lenv.h, Renv.h, x.h, mpo.device(ctx, i), mpo.device(ctx, i + 1) if two else None,
+             int(r), _lib.PREC[precision] & 15, C.byref(h), C.byref(e), C.byref(k))
+    return DeviceTensor(ctx, h), e.value, k.value
+
+
 def expand3s(E, A, mpo, i, alpha, direction):
     """Lexpand (direction 1, E = Lenv[i]) / Rexpand (direction -1, E =
     Renv[i]) joined to A (algs/DMRG.jl:273-286); alpha = 0: A alone."""

@@ -264,6 +264,14 @@ int dmrj_svd_trunc2(dmrj_ctx* ctx, const dmrj_tensor* AA, int64_t m, int64_t min
 int dmrj_lanczos1(dmrj_ctx* ctx, const dmrj_tensor* L, const dmrj_tensor* R, const dmrj_tensor* A,
                   const dmrj_mpo* W, int krylov, int precision, dmrj_tensor** out, double* energy,
                   int* kdim);
+/* k-vector Lanczos ground state of H_eff on a one-site (W2 may be NULL) or
+ * two-site tensor x: the lowest Ritz pair of the `krylov`-vector Krylov space
+ * (full FP64 reorthogonalisation), the generalisation of simplelanczos
+ * (algs/DMRG.jl:57-90, fixed at 2 vectors) that TensorPACK's lanczos
+ * provides for convergence studies and TDVP. */
+int dmrj_lanczos_k(dmrj_ctx* ctx, const dmrj_tensor* L, const dmrj_tensor* R, const dmrj_tensor* x,
+                   const dmrj_mpo* W1, const dmrj_mpo* W2, int krylov, int precision, dmrj_tensor** out,
+                   double* energy, int* kdim);
 /* Lexpand (dir > 0, E = Lenv[i]) / Rexpand (dir < 0, E = Renv[i]) with
  * joinindex! (algs/DMRG.jl:273-286): A joined with alpha times the expansion
  * along the right (left) bond; every sector of the enlarged bond holds the

@@ -98,6 +98,37 @@ def test_lanczos1_and_expansion_match_oracle(pkg, ctx, qn):
 
 
 @pytest.mark.parametrize("qn", [False, True], ids=["dense", "u1"])
+def test_lanczos_k_matches_oracle(pkg, ctx, qn):
+    """dmrj_lanczos_k: two-site r = 2 equals simplelanczos (algs/DMRG.jl:57-90),
+    two-site r = 10 and one-site r = 6 equal the oracle's Ritz pairs."""
+    say("case lanczos_k", "u1" if qn else "dense")
+    i = 4
+    opsi, mpo_h, oL, oR = _oracle_state(10, 8, i, qn)
+    psi, mpo, L, R = _upload(pkg, ctx, opsi, mpo_h)
+    th = pkg.tdvp._theta(ctx, psi[i], psi[i + 1])
+    W1, W2 = mpo_h[i - 1], mpo_h[i]
+    oth = np.tensordot(opsi[i], opsi[i + 1], axes=(2, 0))
+    e2, _ = od.simplelanczos(oL[i], oR[i + 1], opsi[i], opsi[i + 1], W1, W2)
+    for r, want in ((2, e2), (10, None)):
+        out, e, k = pkg.lanczos_k(L[i], R[i + 1], th, mpo, i, r)
+        oe, ov = o3.lanczos_ritz(lambda t: od.heff2(oL[i], oR[i + 1], t, W1, W2), oth, r)
+        gv = out.to_dense(psi.nq)[0].reshape(ov.shape, order="F")
+        say(" two-site r", r, "k", k, "energy", e, oe, "simplelanczos", want, "vector rel", relerr(gv, ov))
+        assert abs(e - oe) <= TOL * abs(oe)
+        if want is not None:
+            assert abs(e - want) <= TOL * abs(want)
+        assert relerr(gv, ov) <= 1e-10
+        out.free()
+    out, e, k = pkg.lanczos_k(L[i], R[i], psi[i], mpo, i, 6)
+    oe, ov = o3.lanczos_ritz(lambda t: od.singlesite_update(oL[i], oR[i], t, W1), opsi[i], 6)
+    say(" one-site r 6 energy", e, oe)
+    assert abs(e - oe) <= TOL * abs(oe)
+    assert relerr(out.to_dense(psi.nq)[0], ov) <= 1e-10
+    out.free()
+    th.free()
+
+
+@pytest.mark.parametrize("qn", [False, True], ids=["dense", "u1"])
 @pytest.mark.parametrize("j", [1, -1], ids=["right", "left"])
 def test_step3s_matches_oracle(pkg, ctx, qn, j):
     """One step3S: energy, the two-site state after the move (gauge-invariant
