@@ -1122,8 +1122,9 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
       if (s.kmin == 0 || is_small[si]) continue;
       const int rows = (int)(dir > 0 ? s.R : s.C), ncols = (int)(dir > 0 ? s.C : s.R);
       int bw = 16;
-      while (bw > 4 && bj_lds_bytes(rows, bw) > c.lds_max) bw >>= 1;
-      if (bj_lds_bytes(rows, bw) > c.lds_max) continue;  // too tall for LDS: augmented solver below
+      const size_t lds_dyn = c.lds_max - kBjStaticLds;  // the kernel's static LDS comes on top
+      while (bw > 4 && bj_lds_bytes(rows, bw) > lds_dyn) bw >>= 1;
+      if (bj_lds_bytes(rows, bw) > lds_dyn) continue;  // too tall for LDS: augmented solver below
       s.X = (double*)c.alloc.alloc((int64_t)rows * ncols * sizeof(double));
       if (dir > 0)
         HIP_OK(hipMemcpyAsync(s.X, s.M, (int64_t)rows * ncols * sizeof(double), hipMemcpyDeviceToDevice, c.stream));

@@ -316,6 +316,9 @@ struct BjGather {
   int rows, trans;
 };
 size_t bj_lds_bytes(int rows, int bw);
+// static shared memory of k_bj_step, reserved out of the per-work-group LDS
+// limit (lds_max counts static and dynamic LDS together)
+constexpr size_t kBjStaticLds = 256;
 void launch_bj_sweep(Context& c, const BjJob* djobs, const std::vector<int2>& work, int nsteps, int inner_max,
                      size_t lds);
 void launch_bj_norms(Context& c, const BjJob* djobs, const std::vector<int2>& work);

@@ -12,9 +12,10 @@
 // among themselves with inner circle rounds (one group of lanes per column
 // pair, 3 dot products and a plane rotation), then writes them back.  Rounds
 // repeat as outer sweeps until a sweep rotates nothing beyond
-// |x_p . x_q| <= tol sqrt(|x_p|^2 |x_q|^2), tol = sqrt(rows) eps (the
-// relative threshold of LAPACK's dgesvj), which keeps every column pair
-// orthogonal to working precision.  Columns whose norm fell to the rounding
+// |x_p . x_q| <= tol sqrt(|x_p|^2 |x_q|^2), tol = rows eps (one of the
+// relative thresholds of LAPACK's dgesvj; the rounding of a computed dot
+// product of two orthogonal columns is of that order, so a stricter threshold
+// keeps rotating pairs that are already orthogonal).  Columns whose norm fell to the rounding
 // level of the whole tensor (8 eps sqrt(rows) |AA|) are numerically zero and
 // take no further rotations.  Before every sweep the host orders the columns
 // by decreasing norm (de Rijk's pivoting: about half the sweeps on DMRG
@@ -90,7 +91,10 @@ __global__ __launch_bounds__(kBjThreads) void k_bj_step(const BjJob* __restrict_
     int G = 64;
     while (G > 1 && G * P > kBjThreads) G >>= 1;
     const int grp = tid / G, gl = tid % G;
-    const double tol = 2.220446049250313e-16 * sqrt((double)rows);
+    // rotate only pairs whose cosine exceeds rows * eps: a computed dot
+    // product of two orthogonal columns carries rounding of that order, so a
+    // stricter threshold keeps rotating already orthogonal pairs
+    const double tol = 2.220446049250313e-16 * (double)rows;
     bool any_step = false;
     for (int sweep = 0; sweep < inner_max; ++sweep) {
       if (tid == 0) rot_any = 0;
@@ -545,9 +549,11 @@ size_t bj_lds_bytes(int rows, int bw) { return (size_t)2 * bw * rows * sizeof(do
 void launch_bj_sweep(Context& c, const BjJob* djobs, const std::vector<int2>& work, int nsteps, int inner_max,
                      size_t lds) {
   if (work.empty() || nsteps <= 0) return;
+  if (lds + kBjStaticLds > c.lds_max) throw Error(-2, "block Jacobi panel exceeds the work-group LDS");
   static bool attr_set = false;
-  if (!attr_set) {  // allow dynamic LDS up to the device limit
-    (void)hipFuncSetAttribute((const void*)k_bj_step, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.lds_max);
+  if (!attr_set) {  // allow dynamic LDS up to the device limit less the kernel's static LDS
+    (void)hipFuncSetAttribute((const void*)k_bj_step, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)(c.lds_max - kBjStaticLds));
     (void)hipGetLastError();
     attr_set = true;
   }
