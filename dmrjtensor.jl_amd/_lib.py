@@ -56,6 +56,7 @@ SIGNATURES = {
     "dmrj_tensor_is_complex": [_p],
     "dmrj_tensor_to_complex": [_p, _p],
     "dmrj_dotc": [_p, _p, _p, _dp],
+    "dmrj_dotu": [_p, _p, _p, _dp],
     "dmrj_tensor_free": [_p, _p],
     "dmrj_tensor_copy": [_p, _p, C.POINTER(_p)],
     "dmrj_site_export": [_p, _p, _i64p, _i32p, _i32p, _i32p, _dp],

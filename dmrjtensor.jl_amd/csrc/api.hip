@@ -3150,6 +3150,43 @@ int dmrj_dot(dmrj_ctx* c, const dmrj_tensor* x, const dmrj_tensor* y, double* ou
   });
 }
 
+int dmrj_dotu(dmrj_ctx* c, const dmrj_tensor* x, const dmrj_tensor* y, double* out2) {
+  return guard(c, [&] {
+    if (!x || !y || !out2) throw Error(DMRJ_ERR_ARG, "null pointer");
+    if (x->nelem() != y->nelem()) throw Error(DMRJ_ERR_ARG, "dotu: size mismatch");
+    const int64_t n = x->nelem();
+    const double* xr = x->data;
+    const double* yr = y->data;
+    // sum x y without conjugation; a real operand has no imaginary plane
+    double re = 0.0, im = 0.0;
+    {
+      const double* xs[1] = {xr};
+      const double* ys[1] = {yr};
+      launch_dots(*c, 1, xs, ys, n, c->scal + S_TT);
+      re = read_scalar(*c, S_TT);
+    }
+    if (x->cplx && y->cplx) {
+      const double* xs[3] = {xr + n, xr, xr + n};
+      const double* ys[3] = {yr + n, yr + n, yr};
+      launch_dots(*c, 3, xs, ys, n, c->scal + S_A1);  // S_A1, S_CAXPY, S_B1
+      re -= read_scalar(*c, S_A1);
+      im = read_scalar(*c, S_CAXPY) + read_scalar(*c, S_B1);
+    } else if (y->cplx) {
+      const double* xs[1] = {xr};
+      const double* ys[1] = {yr + n};
+      launch_dots(*c, 1, xs, ys, n, c->scal + S_A1);
+      im = read_scalar(*c, S_A1);
+    } else if (x->cplx) {
+      const double* xs[1] = {xr + n};
+      const double* ys[1] = {yr};
+      launch_dots(*c, 1, xs, ys, n, c->scal + S_A1);
+      im = read_scalar(*c, S_A1);
+    }
+    out2[0] = re;
+    out2[1] = im;
+  });
+}
+
 int dmrj_dotc(dmrj_ctx* c, const dmrj_tensor* x, const dmrj_tensor* y, double* out2) {
   return guard(c, [&] {
     if (!x || !y || !out2) throw Error(DMRJ_ERR_ARG, "null pointer");

@@ -1,6 +1,7 @@
 """Measurements on a GPU-resident MPS beyond expect.
 
-correlationmatrix   lib/MPO_MPS/correlationmatrix.jl:21-83 (single-state form)
+correlationmatrix   lib/MPO_MPS/correlationmatrix.jl:21-83 (single-state form;
+                    real or ComplexF64 MPS, real operators)
 
 rho[i, j] = <psi| (Cc F)_i F_{i+1} ... F_{j-1} (Ca)_j |psi> for i < j (the
 trail F only when given), rho[i, i] = <psi| (Cc Ca)_i |psi>, rho[j, i] =
@@ -9,8 +10,10 @@ environment update on the GPU with a one-site operator in place of the MPO
 tensor: left environments of the identity from site 1, right environments of
 the identity from site L, then for each i one left chain through i .. L that
 carries the operator string, closed at every j against the right environment
-of site j+1 by an inner product of the two environment tensors (they share
-the bond, the operator link and therefore the block layout).  O(L^2)
+of site j+1 by the bilinear sum of the two environment tensors (they share
+the bond, the operator link and therefore the block layout; no conjugation:
+the bra's conjugate is inside the environments, so the sum is complex for a
+complex state).  O(L^2)
 environment updates, like the reference's loop.  The result is divided by
 <psi|psi> (the reference assumes a normalised state).
 """
@@ -74,14 +77,19 @@ def _close(Lx, Rx):
     """Full contraction of a left and a right environment of the same bond."""
     if Lx.info()[1:] != Rx.info()[1:] or Lx.nelem != Rx.nelem:
         raise RuntimeError("correlationmatrix: environment layouts differ")
+    if Lx.is_complex or Rx.is_complex:
+        return Lx.dotu(Rx)
     return Lx.dot(Rx)
 
 
 def correlationmatrix(psi, Cc, Ca, trail=None):
     """correlationmatrix(psi, Cc, Ca; trail) (lib/MPO_MPS/correlationmatrix.jl:21-83)
-    for a real MPS on the GPU; returns the L x L matrix on the host."""
-    if any(t.is_complex for t in psi.A):
-        raise NotImplementedError("correlationmatrix: real MPS only")
+    for a real or ComplexF64 MPS on the GPU (real operators); returns the
+    L x L matrix on the host (complex for a complex state)."""
+    for o in (Cc, Ca) + (() if trail is None else (trail,)):
+        if np.iscomplexobj(o):
+            raise NotImplementedError("correlationmatrix: real operators only")
+    cplx = any(t.is_complex for t in psi.A)
     ctx = psi.ctx
     L = len(psi)
     ops = _Ops(psi, Cc, Ca, trail)
@@ -97,7 +105,7 @@ def correlationmatrix(psi, Cc, Ca, trail=None):
     last_left = _left(ctx, Lb[L], psi[L], ops.ident)
     norm2 = _close(last_left, Rb[L + 1])
     last_left.free()
-    rho = np.zeros((L, L))
+    rho = np.zeros((L, L), dtype=np.complex128 if cplx else np.float64)
     tmp = []
     for i in range(1, L + 1):
         X = _left(ctx, Lb[i], psi[i], ops.onsite)
@@ -110,7 +118,7 @@ def correlationmatrix(psi, Cc, Ca, trail=None):
             v = _close(Y, Rb[j + 1])
             Y.free()
             rho[i - 1, j - 1] = v
-            rho[j - 1, i - 1] = v
+            rho[j - 1, i - 1] = np.conj(v)
             if j < L:
                 X2 = _left(ctx, X, psi[j], ops.trail)
                 X.free()
@@ -119,4 +127,4 @@ def correlationmatrix(psi, Cc, Ca, trail=None):
     for e in Lb + Rb + tmp:
         if e is not None:
             e.free()
-    return rho / norm2
+    return rho / (norm2.real if cplx else norm2)

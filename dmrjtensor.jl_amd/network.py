@@ -100,6 +100,12 @@ class DeviceTensor:
         self.ctx.call("dmrj_dotc", self.h, other.h, _lib.dptr(out))
         return complex(out[0], out[1])
 
+    def dotu(self, other):
+        """sum self other without conjugation (complex)."""
+        out = np.zeros(2)
+        self.ctx.call("dmrj_dotu", self.h, other.h, _lib.dptr(out))
+        return complex(out[0], out[1])
+
     def free(self):
         if self.h:
             self.ctx.lib.dmrj_tensor_free(self.ctx.h, self.h)

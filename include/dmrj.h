@@ -99,7 +99,10 @@ int dmrj_ctx_graph_stats(dmrj_ctx* ctx, double* out2);
 int dmrj_ctx_marker(dmrj_ctx* ctx, int id);
 /* SVD backend of dmrj_svd_trunc2 / dmrj_twostep: 0 (default) = batched
  * one-sided Jacobi for sectors with min(R,C) <= 64 and one batched symmetric
- * eigensolve of the augmented matrices [[0,M],[M^T,0]] for the larger ones;
+ * eigensolve of the augmented matrices [[0,M],[M^T,0]] for the larger ones
+ * (a sector at least 1.5 times longer than the side that must come out
+ * orthonormal is first reduced by a batched Householder LQ / QR to its K x K
+ * triangle, whose augmented matrix has size 2K);
  * 1 = rocSOLVER gesdd per sector, 2 = gesvd per sector, 3 = batched Jacobi +
  * gesvd, 4 = batched Jacobi for small sectors + batched block one-sided
  * Jacobi (columns of M or M^T in LDS-resident block pairs) for the larger
@@ -191,6 +194,11 @@ int dmrj_tensor_is_complex(const dmrj_tensor* t);
 int dmrj_tensor_to_complex(dmrj_ctx* ctx, dmrj_tensor* t);
 /* complex inner product, out2 = {Re, Im} of sum conj(x) y */
 int dmrj_dotc(dmrj_ctx* ctx, const dmrj_tensor* x, const dmrj_tensor* y, double* out2);
+/* bilinear sum x y (no conjugation), out2 = {Re, Im}: the closing contraction
+ * of a left and a right environment of the same bond (both stored [bra, w,
+ * ket]), the last contract of correlationmatrix (lib/MPO_MPS/
+ * correlationmatrix.jl:21-83) on complex states */
+int dmrj_dotu(dmrj_ctx* ctx, const dmrj_tensor* x, const dmrj_tensor* y, double* out2);
 
 /* ---- MPO ------------------------------------------------------------- */
 /* One MPO site W[w,j,k,u] (lib/MPO/makeMPO.jl:59-92; charges as

@@ -75,3 +75,31 @@ def test_correlationmatrix_fermions_with_trail(pkg, ctx):
     say(" max |M - oracle| %.2e, |M - P_occ| %.2e" % (np.max(np.abs(M - want)), np.max(np.abs(M - occ @ occ.T))))
     assert np.max(np.abs(M - want)) < 1e-12
     assert np.max(np.abs(M - occ @ occ.T)) < 1e-12
+
+
+@pytest.mark.parametrize("qn", [False, True], ids=["dense", "u1"])
+def test_correlationmatrix_complex_tdvp_state(pkg, ctx, qn):
+    """ComplexF64 state after real-time TDVP from the Neel state (L=8):
+    <S+_i S-_j> and <Sz_i Sz_j> against the oracle's brute force on the
+    complex amplitude vector (1e-12), Hermitian, not real."""
+    say("case correlationmatrix complex", "u1" if qn else "dense")
+    L = 8
+    qphys = np.array([[1], [-1]]) if qn else None
+    mpo_h = om.makeMPO(om.heisenbergMPO(), 2, L)
+    mpo = pkg.MPO(mpo_h, qphys=qphys) if qn else pkg.MPO(mpo_h)
+    psi = pkg.product_state([i % 2 for i in range(L)], 2, qphys=qphys, ctx=ctx)
+    Le = Re = None
+    for _ in range(6):
+        _, Le, Re = pkg.tdvp_twosite(psi, mpo, m=16, cutoff=1e-12, r=8, prefactor=-1.0j, Lenv=Le, Renv=Re)
+    assert any(t.is_complex for t in psi.A)
+    host, _ = psi.to_dense()
+    vec = ome.mps_to_vector([np.asarray(a, dtype=np.complex128) for a in host])
+    Sp, Sm, Sz, O, Id = pkg.spinOps()
+    for Cc, Ca in ((Sp, Sm), (Sz, Sz)):
+        M = pkg.correlationmatrix(psi, Cc, Ca)
+        want = ome.correlationmatrix(vec, 2, L, Cc, Ca)
+        err = np.max(np.abs(M - want))
+        say(" max |M - oracle| %.2e, max |Im M| %.2e" % (err, np.max(np.abs(M.imag))))
+        assert err < 1e-12 * max(1.0, np.max(np.abs(want)))
+        assert np.allclose(M, M.conj().T, atol=1e-13)
+    assert np.max(np.abs(pkg.correlationmatrix(psi, Sp, Sm).imag)) > 1e-6
