@@ -310,3 +310,39 @@ def test_dmrg3s_dense_vs_qn_reference_criteria(pkg, ctx):
     say(" Hubbard L=10: dense", out[0], "U(1)xU(1)", out[1])
     assert abs(out[0] - out[1]) < 1e-4
     assert abs(out[1] - (-25.380567747816)) < 1e-4
+
+
+def test_3s_entry_points_reject_bad_arguments(pkg, ctx):
+    """Error behaviour of the 3S and k-Lanczos entry points: a clean
+    DMRJ_ERR_ARG (raised as DMRJError) for the wrong environment side, a
+    neighbour that does not share the bond, a two-site tensor without its
+    second MPO tensor, and a direction other than +-1."""
+    say("case 3S argument errors")
+    i = 4
+    opsi, mpo_h, oL, oR = _oracle_state(10, 8, i, True)
+    psi, mpo, L, R = _upload(pkg, ctx, opsi, mpo_h)
+    with pytest.raises(pkg.DMRJError):
+        pkg.expand3s(R[i], psi[i], mpo, i, 0.5, 1)        # moving right needs the left environment
+    with pytest.raises(pkg.DMRJError):
+        pkg.expand3s(L[i], psi[i], mpo, i, 0.5, -1)       # moving left needs the right environment
+    with pytest.raises(pkg.DMRJError):
+        pkg.expand3s(L[i], psi[i], mpo, i, 0.5, 2)
+    T = pkg.expand3s(L[i], psi[i], mpo, i, 0.5, 1)
+    T.ctx.call("dmrj_normalize", T.h)
+    import ctypes as C
+    nl, nr = C.c_void_p(), C.c_void_p()
+    te, kept = C.c_double(), C.c_int64()
+    sv = np.zeros(64)
+    ctx.call("dmrj_svd_trunc2", T.h, 8, 2, 0.0, 1, C.byref(nl), C.byref(nr), C.byref(te), C.byref(kept),
+             pkg._lib.dptr(sv), 64)
+    left, right = pkg.DeviceTensor(ctx, nl), pkg.DeviceTensor(ctx, nr)
+    with pytest.raises(pkg.DMRJError):
+        pkg.absorb3s(psi[i], psi[i + 1], 1)               # not an SVD factor with a trivial index
+    good = pkg.absorb3s(right, psi[i + 1], 1)
+    assert good.shape[0] == right.shape[0]
+    th = pkg.tdvp._theta(ctx, psi[i], psi[i + 1])
+    with pytest.raises(pkg.DMRJError):
+        ctx.call("dmrj_lanczos_k", L[i].h, R[i + 1].h, th.h, mpo.device(ctx, i), None, 2, 0, C.byref(C.c_void_p()),
+                 C.byref(C.c_double()), C.byref(C.c_int()))
+    for t in (T, left, right, good, th):
+        t.free()
