@@ -446,7 +446,10 @@ def bench_tdvp(pkg, ctx, args):
                    "contraction_tflops": (fl_heff + fl_env) / elapsed / 1e12},
         "graph": ctx.graph_stats() if args.graph else None,
         "roofline": {"bound": "mfma", "kernel": "k_gemm_f64 (complexified plans)", "achieved": round(g_tflops, 3),
-                     "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": round(g_tflops / PEAK_FP64, 4), "traffic": None},
+                     "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": round(g_tflops / PEAK_FP64, 4), "traffic": None,
+                     "note": "launch durations from two profiled half-sweeps right after the timed region (events "
+                             "on every launch would slow the timed half-sweeps by ~40 %); at bond dimension 4 "
+                             "the launches are latency-bound"},
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
