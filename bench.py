@@ -88,8 +88,8 @@ def main():
                     help="only run one block-copy kernel over this many doubles (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md); run with --gpus 1",
+        print("bench.py measures one GPU: there is no multi-GPU path (one GPU per process; the "
+              "sharding of the bond step across GPUs is a design, not code); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
 

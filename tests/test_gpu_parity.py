@@ -1,7 +1,7 @@
 """GPU parity of the hot path against the numpy oracle, through the C ABI.
 
 Tolerance: FP64 path, max |gpu - oracle| <= 1e-12 * max |oracle| for
-contractions and energies (north_star: <= 1e-12 relative); spectra compared
+contractions and energies (the FP64 drop-in bar: <= 1e-12 relative); spectra compared
 after sorting, gauge-dependent tensors compared through gauge-invariant
 products.
 """

@@ -1,7 +1,7 @@
 """Whole two-site DMRG runs on the GPU path against the oracle and the
 reference's constants.
 
-Per-sweep energies: <= 1e-12 relative to the oracle (north_star).  Centre
+Per-sweep energies: <= 1e-12 relative to the oracle (the FP64 drop-in bar).  Centre
 bond spectra (top-8 singular values) and the largest truncation error per
 sweep are compared within 1e-8 relative (gauge- and tie-robust quantities).
 """
