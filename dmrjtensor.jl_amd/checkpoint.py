@@ -3,7 +3,7 @@
 The reference keeps long runs restartable by writing tensors to disk
 (largeMPS / tensor2disc, lib/MPS/largeMPS.jl:19-81) and by handing the
 environments back through params.Lenv / params.Renv
-(algs/optimizeMPS.jl:273-334).  This build stores an MPS in its own format:
+(algs/optimizeMPS.jl:273-334).  This package stores an MPS in its own format:
 one NumPy ``.npz`` archive (no pickled objects) holding, per site, the block
 storage of the device tensor exactly as the library keeps it (real plane then
 imaginary plane) and the per-index charges of both bonds and of the physical

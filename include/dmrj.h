@@ -158,7 +158,7 @@ int dmrj_tensor_to_dense(dmrj_ctx* ctx, const dmrj_tensor* t, double* out, int32
 int dmrj_tensor_free(dmrj_ctx* ctx, dmrj_tensor* t);
 int dmrj_tensor_copy(dmrj_ctx* ctx, const dmrj_tensor* t, dmrj_tensor** out);
 /* Checkpoint form of an MPS tensor (no counterpart in the reference's C-level
- * API: this build's own storage, standing where the reference writes tensors
+ * API: the library's own storage, standing where the reference writes tensors
  * to disk for largeMPS, lib/MPS/largeMPS.jl:19-81, and reloads them to resume,
  * algs/optimizeMPS.jl:273-334).  dmrj_site_export fills info8 = {nq, mL, P,
  * mR, stored doubles, complex, 0, 0}; qL (mL x nq), qP (P x nq), qR (mR x nq)
