@@ -5,7 +5,8 @@ reference's two-site DMRG inner loop (algs/DMRG.jl, algs/optimizeMPS.jl,
 lib/MPO_MPS/{Lupdate,Rupdate,expect}.jl, lib/Env/makeEnv.jl, lib/models.jl,
 lib/MPO/makeMPO.jl, algs/time.jl).  It is imported only by the tests, by
 ``__graft_entry__.smoke()`` and by the ``cpu_baseline`` leg of ``bench.py``,
-always as the checker and never as the thing measured or shipped.  The product
+always as what GPU results are compared against, never as the thing measured
+or shipped (the CPU baseline times it as the CPU reference point).  The product
 path (the HIP library behind include/dmrj.h) never imports it.
 
 Pinning (what each part is checked against, see tests/test_oracle_pins.py):
