@@ -1892,11 +1892,18 @@ dmrj_tensor* krylov_expm(Context& c, const Tensor& L, const Tensor& R, const Ten
   // the whole iteration is a fixed kernel sequence on device scalars: with
   // graphs on it is captured and launched as one graph
   const bool graph = c.graph_mode != 0;
+  // short vectors: everything after H_eff in one single-work-group launch,
+  // which also writes v_{j+1} into the fixed input buffer
+  const bool fused = ne <= kKrylovFusedMax;
   auto iterate = [&]() {
   for (int j = 0; j < r; ++j) {
     double* vj = V + (size_t)j * nv;
-    if (j > 0) HIP_OK(hipMemcpyAsync(xin, vj, ne * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    if (j > 0 && !fused) HIP_OK(hipMemcpyAsync(xin, vj, ne * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
     issue_heff(c, hl, !graph);
+    if (fused) {
+      launch_krylov_tail(c, V, nv, j, w, m.size, zv ? 1 : 0, sc, xin, j == r - 1 ? 1 : 0);
+      continue;
+    }
     // alpha_j = Re <v_j, w> (real for a Hermitian H_eff)
     {
       const double* xs[1] = {vj};

@@ -359,6 +359,12 @@ void launch_marker(Context& c, int id);
 // coef: 2 k doubles.
 void launch_cgs_pass(Context& c, double* w, const double* V, int64_t nv, int64_t n, int k, int cplx,
                      double* partial, double* coef);
+// One Krylov iteration after the H_eff application in a single work-group
+// (alpha_j to sc[j]; unless last: two Gram-Schmidt passes against v_0..v_j,
+// beta_j to sc[64 + j], v_{j+1} to V and xin) for short vectors.
+void launch_krylov_tail(Context& c, double* V, int64_t nv, int j, double* w, int64_t n, int cplx, double* sc,
+                        double* xin, int last);
+constexpr int64_t kKrylovFusedMax = 16384;  // stored elements of a Krylov vector for the one-work-group path
 void launch_dots(Context& c, int npairs, const double* const* xs, const double* const* ys,
                  int64_t n, double* out);
 void launch_axpy_dev(Context& c, double* y, const double* x, int64_t n, const double* coef);

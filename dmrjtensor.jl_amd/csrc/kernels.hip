@@ -333,6 +333,106 @@ __global__ void k_cgs_sub(double* __restrict__ w, const double* __restrict__ V, 
   }
 }
 
+// One Krylov iteration after the H_eff application, in one work-group, for
+// short vectors (the latency of a dozen dependent launches dominates there):
+// alpha_j = Re <v_j, w> -> sc[j]; unless last: two classical Gram-Schmidt
+// passes of w against v_0..v_j (coefficients of a pass from the same w, then
+// one subtraction, as the multi-launch passes), beta_j = |w| -> sc[64 + j],
+// v_{j+1} = w / beta_j (zero when beta_j <= 1e-10) written to V and to the
+// fixed H_eff input xin.  Complex vectors: real plane then imaginary plane of
+// n elements each, <v, w> = sum conj(v) w.
+__global__ __launch_bounds__(kThreads) void k_krylov_tail(double* __restrict__ V, int64_t nv, int j,
+                                                          double* __restrict__ w, int64_t n, int cplx,
+                                                          double* __restrict__ sc, double* __restrict__ xin,
+                                                          int last) {
+  __shared__ double red[2][kThreads / 64];
+  __shared__ double coef[2 * 64];
+  const int tid = threadIdx.x;
+  const int64_t ne = cplx ? 2 * n : n;
+  auto block_sum2 = [&](double a, double b, double& A, double& B) {
+    a = wave_sum(a);
+    b = wave_sum(b);
+    if ((tid & 63) == 0) {
+      red[0][tid >> 6] = a;
+      red[1][tid >> 6] = b;
+    }
+    __syncthreads();
+    A = 0.0;
+    B = 0.0;
+    for (int i = 0; i < kThreads / 64; ++i) {
+      A += red[0][i];
+      B += red[1][i];
+    }
+    __syncthreads();
+  };
+  double A, B;
+  {
+    const double* vj = V + (int64_t)j * nv;
+    double a = 0.0;
+    for (int64_t e = tid; e < ne; e += kThreads) a += vj[e] * w[e];
+    block_sum2(a, 0.0, A, B);
+    if (tid == 0) sc[j] = A;
+  }
+  if (last) return;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int i = 0; i <= j; ++i) {
+      const double* v = V + (int64_t)i * nv;
+      double re = 0.0, im = 0.0;
+      for (int64_t e = tid; e < n; e += kThreads) {
+        if (cplx) {
+          const double vr = v[e], vi = v[n + e], wr = w[e], wi = w[n + e];
+          re += vr * wr + vi * wi;
+          im += vr * wi - vi * wr;
+        } else {
+          re += v[e] * w[e];
+        }
+      }
+      block_sum2(re, im, A, B);
+      if (tid == 0) {
+        coef[2 * i] = A;
+        coef[2 * i + 1] = B;
+      }
+    }
+    __syncthreads();
+    for (int64_t e = tid; e < n; e += kThreads) {
+      double ar = 0.0, ai = 0.0;
+      for (int i = 0; i <= j; ++i) {
+        const double* v = V + (int64_t)i * nv;
+        const double cr = coef[2 * i];
+        if (cplx) {
+          const double ci = coef[2 * i + 1], vr = v[e], vi = v[n + e];
+          ar += cr * vr - ci * vi;
+          ai += cr * vi + ci * vr;
+        } else {
+          ar += cr * v[e];
+        }
+      }
+      w[e] -= ar;
+      if (cplx) w[n + e] -= ai;
+    }
+    __syncthreads();
+  }
+  double s2 = 0.0;
+  for (int64_t e = tid; e < ne; e += kThreads) s2 += w[e] * w[e];
+  block_sum2(s2, 0.0, A, B);
+  const double b = sqrt(A);
+  const double invb = b > 1e-10 ? 1.0 / b : 0.0;
+  if (tid == 0) sc[64 + j] = b;
+  double* vn = V + (int64_t)(j + 1) * nv;
+  for (int64_t e = tid; e < ne; e += kThreads) {
+    const double x = w[e] * invb;
+    vn[e] = x;
+    xin[e] = x;
+  }
+}
+
+void launch_krylov_tail(Context& c, double* V, int64_t nv, int j, double* w, int64_t n, int cplx, double* sc,
+                        double* xin, int last) {
+  if (j < 0 || j >= 64) throw Error(-1, "krylov tail: vector index out of range");
+  hipLaunchKernelGGL(k_krylov_tail, dim3(1), dim3(kThreads), 0, c.stream, V, nv, j, w, n, cplx, sc, xin, last);
+  HIP_OK(hipGetLastError());
+}
+
 __global__ void k_axpy_dev(double* y, const double* x, int64_t n, const double* coef) {
   const double c = *coef;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
