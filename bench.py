@@ -67,6 +67,8 @@ def main():
                          "environment-update GEMMs only)")
     ap.add_argument("--no-autotune", action="store_true", help="fixed tile-shape heuristic instead of the autotuner")
     ap.add_argument("--no-xcd-order", action="store_true", help="plain descending-work tile order")
+    ap.add_argument("--gemm-cfg", type=int, default=-1,
+                    help="force one grouped-GEMM tile shape (dmrj_ctx_set_gemm_cfg, 0..4; -1 = autotuner)")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_k_gemm_f64.json"),
                     help="HBM bytes per k_gemm_f64 launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE")
     ap.add_argument("--method", default="twosite", choices=["twosite", "3S"],
@@ -105,6 +107,8 @@ def main():
         ctx.set_autotune(0)
     if args.no_xcd_order:
         ctx.set_tile_order(0)
+    if args.gemm_cfg >= 0:
+        ctx.set_gemm_cfg(args.gemm_cfg)
     if args.graph:
         ctx.set_graph(1)
     if args.mode == "tdvp":

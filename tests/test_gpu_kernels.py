@@ -55,6 +55,27 @@ def test_grouped_gemm_f64(pkg, ctx, cfg, tA, tB, M, N, K):
     assert err <= 1e-13, err
 
 
+@pytest.mark.parametrize("tA,tB", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("M,N,K", [(33, 70, 17), (130, 129, 200), (64, 31, 333)])
+def test_gemm_shapes_same_bits(pkg, ctx, tA, tB, M, N, K):
+    """Every tile shape accumulates each output element in the same K order:
+    identical bits."""
+    say("gemm bits tA", tA, "tB", tB, "MNK", M, N, K)
+    rng = np.random.default_rng(M + 5 * N + 11 * K)
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((K, N))
+    As = np.asfortranarray(A.T if tA else A)
+    Bs = np.asfortranarray(B.T if tB else B)
+    out = []
+    for cfg in range(5):
+        C = np.zeros((M, N), order="F")
+        ctx.call("dmrj_selftest_gemm", cfg, M, N, K, tA, tB, pkg._lib.dptr(As.ravel(order="F")),
+                 pkg._lib.dptr(Bs.ravel(order="F")), C.ctypes.data_as(pkg._lib._dp))
+        out.append(C)
+    for cfg in range(1, 5):
+        assert np.array_equal(out[0], out[cfg]), cfg
+
+
 @pytest.mark.parametrize("backend", [0, pytest.param(1, marks=pytest.mark.xfail(
     reason="rocSOLVER gesdd loses the small singular values of graded spectra; kept only as a "
            "selectable backend, never the default", strict=False)), 2, 3, 4, 5])
