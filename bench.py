@@ -67,6 +67,8 @@ def main():
                          "environment-update GEMMs only)")
     ap.add_argument("--no-autotune", action="store_true", help="fixed tile-shape heuristic instead of the autotuner")
     ap.add_argument("--no-xcd-order", action="store_true", help="plain descending-work tile order")
+    ap.add_argument("--eig-batching", type=int, default=None, choices=[0, 1, 2],
+                    help="SVD eigensolve batching (dmrj_ctx_set_eig_batching; default 1 = size classes)")
     ap.add_argument("--gemm-cfg", type=int, default=-1,
                     help="force one grouped-GEMM tile shape (dmrj_ctx_set_gemm_cfg, 0..4; -1 = autotuner)")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_k_gemm_f64.json"),
@@ -109,6 +111,8 @@ def main():
         ctx.set_tile_order(0)
     if args.gemm_cfg >= 0:
         ctx.set_gemm_cfg(args.gemm_cfg)
+    if args.eig_batching is not None:
+        ctx.set_eig_batching(args.eig_batching)
     if args.graph:
         ctx.set_graph(1)
     if args.mode == "tdvp":

@@ -30,6 +30,7 @@ SIGNATURES = {
     "dmrj_ctx_sync": [_p],
     "dmrj_ctx_memstats": [_p, _i64p],
     "dmrj_ctx_set_gemm_cfg": [_p, C.c_int],
+    "dmrj_ctx_set_eig_batching": [_p, C.c_int],
     "dmrj_ctx_set_autotune": [_p, C.c_int],
     "dmrj_ctx_set_tile_order": [_p, C.c_int],
     "dmrj_ctx_svd_stats": [_p, _dp, C.c_int],
@@ -182,6 +183,9 @@ class Context:
 
     def set_gemm_cfg(self, cfg):
         self.call("dmrj_ctx_set_gemm_cfg", int(cfg))
+
+    def set_eig_batching(self, mode):
+        self.call("dmrj_ctx_set_eig_batching", int(mode))
 
     def svd_stats(self, reset=False):
         """Block-Jacobi SVD: calls and total sweeps since the last reset."""

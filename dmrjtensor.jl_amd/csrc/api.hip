@@ -971,6 +971,46 @@ struct SvdResult {
   std::vector<double> svals;  // kept, descending
 };
 
+// Cost model of one step of rocSOLVER's batched symmetric tridiagonalisation
+// on MI355X, from its kernel trace on DMRG sectors: about 20 us of fixed
+// latency per step (five small kernels) plus about 1e-6 us per element of
+// the padded N x N matrices of the batch.  It only groups sectors into
+// batches; every grouping gives a backward-stable result.
+constexpr double kEigColUs = 20.0;
+constexpr double kEigElemUs = 1.0e-6;
+
+// Batches for a list of eigenproblem sizes sorted in descending order:
+// contiguous classes, each solved as one batch padded to its first (largest)
+// size N, chosen to minimise the sum over classes of
+// N * (kEigColUs + kEigElemUs * count * N^2), so a class is split off only
+// when the padding it no longer carries outweighs its extra steps.  Returns
+// the first index of every class followed by n.size().
+static std::vector<int> eig_classes(const std::vector<int>& n, int mode) {
+  const int cnt = (int)n.size();
+  std::vector<int> cls;
+  if (mode != 1) {  // 0: one batch; 2: one batch per sector
+    for (int i = 0; i < cnt; i += mode == 0 ? std::max(cnt, 1) : 1) cls.push_back(i);
+    cls.push_back(cnt);
+    return cls;
+  }
+  std::vector<double> best(cnt + 1, 0.0);
+  std::vector<int> cut(cnt + 1, cnt);
+  for (int i = cnt - 1; i >= 0; --i) {
+    const double Ni = (double)n[i];
+    best[i] = 1e300;
+    for (int j = i + 1; j <= cnt; ++j) {
+      const double cost = Ni * (kEigColUs + kEigElemUs * (j - i) * Ni * Ni) + best[j];
+      if (cost < best[i]) {
+        best[i] = cost;
+        cut[i] = j;
+      }
+    }
+  }
+  for (int i = 0; i < cnt; i = cut[i]) cls.push_back(i);
+  cls.push_back(cnt);
+  return cls;
+}
+
 SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, double cutoff, int dir) {
   const SiteMeta& x = AA.site;
   if (AA.kind != 0 || x.d1 == 0) throw Error(DMRJ_ERR_ARG, "svd_trunc2 needs a two-site tensor");
@@ -1227,36 +1267,54 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
   double* gramW = nullptr;
   double* gramE = nullptr;
   rocblas_int* graminfo = nullptr;
-  int ngram = 0, gramN = 0;
-  std::vector<int> gramsec;
+  int ngram = 0;
+  int64_t gramWtot = 0;
+  std::vector<int> gramsec, gramNb;       // per Gram sector: its batch's padded size
+  std::vector<int64_t> gramZoff, gramWoff;  // and its offsets in gramZ / gramW
   if (!z && c.svd_backend == 5) {
     for (int si : order) {
       Sec& s = secs[si];
       if (s.kmin == 0 || is_small[si]) continue;
       gramsec.push_back(si);
-      gramN = std::max<int>(gramN, (int)(dir > 0 ? s.R : s.C));
     }
     if (!gramsec.empty()) {
+      auto gsize = [&](int si) { return (int)(dir > 0 ? secs[si].R : secs[si].C); };
+      std::stable_sort(gramsec.begin(), gramsec.end(), [&](int a, int b) { return gsize(a) > gsize(b); });
       ngram = (int)gramsec.size();
-      const int64_t NN = (int64_t)gramN * gramN;
-      gramZ = (double*)c.alloc.alloc(NN * ngram * sizeof(double));
-      gramW = (double*)c.alloc.alloc((int64_t)gramN * ngram * sizeof(double));
-      gramE = (double*)c.alloc.alloc((int64_t)gramN * ngram * sizeof(double));
+      std::vector<int> gn(ngram);
+      for (int b = 0; b < ngram; ++b) gn[b] = gsize(gramsec[b]);
+      const std::vector<int> gcls = eig_classes(gn, c.eig_batching);  // batches by size
+      gramNb.resize(ngram);
+      gramZoff.resize(ngram);
+      gramWoff.resize(ngram);
+      int64_t ztot = 0;
+      for (size_t k = 0; k + 1 < gcls.size(); ++k)
+        for (int b = gcls[k]; b < gcls[k + 1]; ++b) {
+          gramNb[b] = gn[gcls[k]];
+          gramZoff[b] = ztot;
+          gramWoff[b] = gramWtot;
+          ztot += (int64_t)gramNb[b] * gramNb[b];
+          gramWtot += gramNb[b];
+        }
+      gramZ = (double*)c.alloc.alloc(ztot * sizeof(double));
+      gramW = (double*)c.alloc.alloc(gramWtot * sizeof(double));
+      gramE = (double*)c.alloc.alloc(gramWtot * sizeof(double));
       graminfo = (rocblas_int*)c.alloc.alloc(sizeof(rocblas_int) * ngram);
-      launch_fill(c, gramZ, NN * ngram, 0.0);
+      launch_fill(c, gramZ, ztot, 0.0);
       std::vector<GemmTask> tasks;
       std::vector<GemmSeg> segs;
-      std::vector<int2> padjobs;
+      std::vector<std::vector<int2>> padjobs(gcls.size() - 1);
+      for (size_t k = 0; k + 1 < gcls.size(); ++k)
+        for (int b = gcls[k]; b < gcls[k + 1]; ++b) padjobs[k].push_back(make_int2(b - gcls[k], gn[b]));
       for (int b = 0; b < ngram; ++b) {
         Sec& s = secs[gramsec[b]];
         const int n = (int)(dir > 0 ? s.R : s.C);
-        double* Zb = gramZ + b * NN;
-        tasks.push_back({Zb, gramN, n, n, (int)segs.size(), 1, 0.0});
+        double* Zb = gramZ + gramZoff[b];
+        tasks.push_back({Zb, gramNb[b], n, n, (int)segs.size(), 1, 0.0});
         if (dir > 0)
           segs.push_back({s.M, s.M, s.R, s.R, (int)s.C, 2, 1.0});  // M M^T
         else
           segs.push_back({s.M, s.M, s.R, s.R, (int)s.R, 1, 1.0});  // M^T M
-        padjobs.push_back(make_int2(b, n));
         s.bj = 1;
         s.rowsX = n;
         is_small[gramsec[b]] = 1;  // handled
@@ -1266,11 +1324,14 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
       launch_gemm(c, tasks, segs, -1);
       // padding rows / columns: -1 on the diagonal keeps their eigenvalues
       // below every sigma^2 >= 0 and their eigenvectors outside the sector
-      launch_pad_diag(c, gramZ, gramN, padjobs, -1.0);
-      rocblas_status st = rocsolver_dsyevd_strided_batched(hb, rocblas_evect_original, rocblas_fill_upper, gramN,
-                                                           gramZ, gramN, NN, gramW, gramN, gramE, gramN, graminfo,
-                                                           ngram);
-      if (st != rocblas_status_success) throw Error(DMRJ_ERR_SOLVER, "rocsolver_dsyevd_strided_batched failed");
+      for (size_t k = 0; k + 1 < gcls.size(); ++k) {
+        const int b0 = gcls[k], Nk = gramNb[b0];
+        launch_pad_diag(c, gramZ + gramZoff[b0], Nk, padjobs[k], -1.0);
+        rocblas_status st = rocsolver_dsyevd_strided_batched(
+            hb, rocblas_evect_original, rocblas_fill_upper, Nk, gramZ + gramZoff[b0], Nk, (int64_t)Nk * Nk,
+            gramW + gramWoff[b0], Nk, gramE + gramWoff[b0], Nk, graminfo + b0, gcls[k + 1] - b0);
+        if (st != rocblas_status_success) throw Error(DMRJ_ERR_SOLVER, "rocsolver_dsyevd_strided_batched failed");
+      }
     }
   }
   // large sectors, default backend: one batched symmetric eigensolve of the
@@ -1394,29 +1455,51 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
       launch_aug_extract(c, jobs);
     }
     if (!big.empty()) {
+      // Size classes (eig_classes), each one batched eigensolve padded to its
+      // own largest augmented size: the eigensolve takes about N steps, each
+      // a fixed latency plus a pass over the padded matrices of the batch.
+      std::stable_sort(big.begin(), big.end(),
+                       [&](int a, int b) { return secs[a].R + secs[a].C > secs[b].R + secs[b].C; });
       naug = (int)big.size();
-      const int64_t NN = (int64_t)N * N;
-      aug = (double*)c.alloc.alloc(NN * naug * sizeof(double));
-      augW = (double*)c.alloc.alloc((int64_t)N * naug * sizeof(double));
-      augE = (double*)c.alloc.alloc((int64_t)N * naug * sizeof(double));
+      std::vector<int> nsz(naug);
+      for (int b = 0; b < naug; ++b) nsz[b] = (int)(secs[big[b]].R + secs[big[b]].C);
+      const std::vector<int> cls = eig_classes(nsz, c.eig_batching);
+      const int ncls = (int)cls.size() - 1;
+      std::vector<int> cN(ncls);
+      std::vector<int64_t> aoff(ncls + 1, 0), voff(ncls + 1, 0);
+      for (int k = 0; k < ncls; ++k) {
+        cN[k] = (int)(secs[big[cls[k]]].R + secs[big[cls[k]]].C);
+        aoff[k + 1] = aoff[k] + (int64_t)cN[k] * cN[k] * (cls[k + 1] - cls[k]);
+        voff[k + 1] = voff[k] + (int64_t)cN[k] * (cls[k + 1] - cls[k]);
+      }
+      aug = (double*)c.alloc.alloc(aoff[ncls] * sizeof(double));
+      augW = (double*)c.alloc.alloc(voff[ncls] * sizeof(double));
+      augE = (double*)c.alloc.alloc(voff[ncls] * sizeof(double));
       auginfo = (rocblas_int*)c.alloc.alloc(sizeof(rocblas_int) * naug);
-      launch_fill(c, aug, NN * naug, 0.0);
+      launch_fill(c, aug, aoff[ncls], 0.0);
       std::vector<CopyOp> place;
-      for (int b = 0; b < naug; ++b) {
-        Sec& s = secs[big[b]];
-        place.push_back({s.M, aug + b * NN + s.R * (int64_t)N, s.R, N, (int)s.R, (int)s.C, nullptr, nullptr, 1.0});
+      std::vector<AugJob> jobs;
+      for (int k = 0; k < ncls; ++k) {
+        const int Nk = cN[k];
+        const int64_t NNk = (int64_t)Nk * Nk;
+        for (int b = cls[k]; b < cls[k + 1]; ++b) {
+          Sec& s = secs[big[b]];
+          double* a = aug + aoff[k] + (b - cls[k]) * NNk;
+          place.push_back({s.M, a + s.R * (int64_t)Nk, s.R, Nk, (int)s.R, (int)s.C, nullptr, nullptr, 1.0});
+          jobs.push_back({a, augW + voff[k] + (int64_t)(b - cls[k]) * Nk, s.U, s.s, s.Vt, (int)s.R, (int)s.C, Nk,
+                          s.kmin});
+          is_small[big[b]] = 1;  // handled
+          const double n2 = (double)Nk;
+          svd_flops += 9.0 * n2 * n2 * n2;
+        }
       }
       launch_copy(c, place);
-      rocblas_status st = rocsolver_dsyevd_strided_batched(hb, rocblas_evect_original, rocblas_fill_upper, N, aug, N,
-                                                           NN, augW, N, augE, N, auginfo, naug);
-      if (st != rocblas_status_success) throw Error(DMRJ_ERR_SOLVER, "rocsolver_dsyevd_strided_batched failed");
-      std::vector<AugJob> jobs;
-      for (int b = 0; b < naug; ++b) {
-        Sec& s = secs[big[b]];
-        jobs.push_back({aug + b * NN, augW + (int64_t)b * N, s.U, s.s, s.Vt, (int)s.R, (int)s.C, N, s.kmin});
-        is_small[big[b]] = 1;  // handled
-        const double n2 = (double)N;
-        svd_flops += 9.0 * n2 * n2 * n2;
+      for (int k = 0; k < ncls; ++k) {
+        const int Nk = cN[k];
+        rocblas_status st = rocsolver_dsyevd_strided_batched(
+            hb, rocblas_evect_original, rocblas_fill_upper, Nk, aug + aoff[k], Nk, (int64_t)Nk * Nk, augW + voff[k],
+            Nk, augE + voff[k], Nk, auginfo + cls[k], cls[k + 1] - cls[k]);
+        if (st != rocblas_status_success) throw Error(DMRJ_ERR_SOLVER, "rocsolver_dsyevd_strided_batched failed");
       }
       launch_aug_extract(c, jobs);
     }
@@ -1456,10 +1539,10 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
       o += s.kmin;
     }
   }
-  std::vector<double> hgram((size_t)std::max(1, ngram * gramN));
+  std::vector<double> hgram((size_t)std::max<int64_t>(1, gramWtot));
   std::vector<rocblas_int> hginfo(std::max(1, ngram));
   if (ngram) {
-    HIP_OK(hipMemcpyAsync(hgram.data(), gramW, (size_t)ngram * gramN * sizeof(double), hipMemcpyDeviceToHost,
+    HIP_OK(hipMemcpyAsync(hgram.data(), gramW, (size_t)gramWtot * sizeof(double), hipMemcpyDeviceToHost,
                           c.stream));
     HIP_OK(hipMemcpyAsync(hginfo.data(), graminfo, ngram * sizeof(rocblas_int), hipMemcpyDeviceToHost, c.stream));
   }
@@ -1500,12 +1583,11 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
       auto g = gram_of.find(si);
       if (g != gram_of.end()) {
         // Gram sector: eigenvalues ascending, sigma_k = sqrt(lambda_{N-1-k}), unit eigenvectors
-        const int b = g->second;
-        const int64_t NN = (int64_t)gramN * gramN;
+        const int b = g->second, Nb = gramNb[b];
         for (int l = 0; l < s.kmin; ++l) {
-          const double lam = hgram[(size_t)b * gramN + gramN - 1 - l];
+          const double lam = hgram[(size_t)gramWoff[b] + Nb - 1 - l];
           hs[o + l] = std::sqrt(std::max(lam, 0.0));
-          s.colsrc.push_back(gramZ + b * NN + (int64_t)(gramN - 1 - l) * gramN);
+          s.colsrc.push_back(gramZ + gramZoff[b] + (int64_t)(Nb - 1 - l) * Nb);
           s.coldiv.push_back(1.0);
         }
       } else if (s.qr) {
@@ -2510,6 +2592,13 @@ int dmrj_ctx_set_svd_backend(dmrj_ctx* c, int mode) {
   return guard(c, [&] {
     if (mode < 0 || mode > 5) throw Error(DMRJ_ERR_ARG, "svd backend must be 0..5");
     c->svd_backend = mode;
+  });
+}
+
+int dmrj_ctx_set_eig_batching(dmrj_ctx* c, int mode) {
+  return guard(c, [&] {
+    if (mode < 0 || mode > 2) throw Error(DMRJ_ERR_ARG, "eigensolve batching mode must be 0..2");
+    c->eig_batching = mode;
   });
 }
 

@@ -188,6 +188,10 @@ struct Context {
   hipStream_t side = nullptr;
   hipEvent_t side_in = nullptr, side_out = nullptr;
   void* rocblas = nullptr;   // rocblas_handle
+  // batching of the SVD's batched symmetric eigensolves: 1 = size classes by
+  // the cost model, 0 = one batch padded to the largest sector, 2 = one batch
+  // per sector (dmrj_ctx_set_eig_batching)
+  int eig_batching = 1;
   Allocator alloc;
   HostStage stage;
   std::vector<void*> transient;  // device copies of plan tables, freed at sync points

@@ -64,6 +64,15 @@ int dmrj_ctx_sync(dmrj_ctx* ctx);
 int dmrj_version(void);
 /* memory statistics: bytes live, bytes cached, peak bytes */
 int dmrj_ctx_memstats(dmrj_ctx* ctx, int64_t* out3);
+/* Batching of the SVD's batched symmetric eigensolves (backends 0 and 5; no
+ * counterpart in the reference, whose SVD runs sector by sector): 1 (default)
+ * = sectors sorted by size and cut into batches, each padded to its own
+ * largest size, where a cost model of the tridiagonalisation (fixed latency
+ * per step against the padded elements read per step) says the saved padding
+ * outweighs the extra steps; 0 = one batch padded to the largest sector;
+ * 2 = one batch per sector.  Every mode gives a backward-stable result; the
+ * bits differ between modes. */
+int dmrj_ctx_set_eig_batching(dmrj_ctx* ctx, int mode);
 /* force a GEMM tile shape (0: 32x32, 1: 64x64, 2: 128x128, 3: 64x128,
  * 4: 128x64, -1: automatic) */
 int dmrj_ctx_set_gemm_cfg(dmrj_ctx* ctx, int cfg);

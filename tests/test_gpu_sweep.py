@@ -163,6 +163,43 @@ def test_autotune_is_result_neutral(pkg, ctx):
             assert list(a["storeD"]) == list(b["storeD"])
 
 
+@pytest.mark.parametrize("backend", [0, 5])
+def test_eig_batching_modes_agree(pkg, ctx, backend):
+    """The SVD's batched eigensolves give the same sweeps whether the sectors
+    form one batch padded to the largest, one batch each, or the cost-model
+    size classes (which keep one batch at these sizes).  L=16 keeps every
+    bond below m, so the cutoff alone truncates and no rounding-level change
+    can move a degenerate multiplet across the bond-dimension cut.  Per-sweep
+    energies within 1e-12 relative for the augmented form (backend 0), 1e-9
+    for the Gram form (backend 5, sigma^2 to eps |M|^2); spectra within
+    1e-12 / 1e-8 absolute."""
+    say("case eigensolve batching modes, backend", backend)
+    L = 16
+    qphys = np.array([[1], [-1]])
+    states = [i % 2 for i in range(L)]
+    qmpo = pkg.MPO(pkg.makeMPO(pkg.heisenbergMPO(), 2, L).H, qphys=qphys)
+    runs = []
+    for mode in (0, 2, 1):
+        ctx.set_eig_batching(mode)
+        ctx.set_svd_backend(backend)
+        try:
+            psi = pkg.product_state(states, 2, qphys=qphys, ctx=ctx)
+            params = pkg.algvars(psi)
+            pkg.dmrg(psi, qmpo, m=512, sweeps=3, cutoff=1e-10, params=params)
+        finally:
+            ctx.set_eig_batching(1)
+            ctx.set_svd_backend(0)
+        say(" mode", mode, "E %.15f" % params.energy)
+        runs.append(params.history)
+    tol_e, tol_s = (1e-12, 1e-12) if backend == 0 else (1e-9, 1e-8)
+    for other in runs[1:]:
+        for a, b in zip(runs[0], other):
+            assert abs(a["energy"] - b["energy"]) <= tol_e * abs(a["energy"])
+            sa, sb = np.asarray(a["storeD"]), np.asarray(b["storeD"])
+            n = min(len(sa), len(sb))
+            assert np.max(np.abs(sa[:n] - sb[:n])) <= tol_s
+
+
 def test_graph_captured_lanczos_matches_direct(pkg, ctx):
     """Lanczos steps captured into hipGraphs (instantiated once, then updated
     in place bond after bond) give the same per-sweep energies, truncation
