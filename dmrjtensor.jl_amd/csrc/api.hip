@@ -1382,7 +1382,7 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
     }
   } else if (c.svd_backend == 0) {
     std::vector<int> big, qrs;
-    int N = 0, Kq = 0, Lq = 0;
+    int N = 0;
     for (int si : order) {
       Sec& s = secs[si];
       if (s.kmin == 0 || is_small[si]) continue;
@@ -1391,8 +1391,6 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
       const int sh = (int)(dir > 0 ? s.R : s.C), lg = (int)(dir > 0 ? s.C : s.R);
       if (2 * (int64_t)lg >= 3 * (int64_t)sh) {
         qrs.push_back(si);
-        Kq = std::max(Kq, sh);
-        Lq = std::max(Lq, lg);
       } else {
         big.push_back(si);
         N = std::max<int>(N, (int)(s.R + s.C));
@@ -1406,51 +1404,92 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
     // the full augmented form (sigma to eps |M| absolute); the other factor
     // comes from the grouped GEMM with M below, as for the one-factor backends.
     if (!qrs.empty()) {
+      // batches by size as for the augmented form below (eig_classes on the
+      // 2K augmented sizes), each padded to its own largest K and L
+      const int sdir = dir;
+      auto shortside = [&](int si) { return (int)(sdir > 0 ? secs[si].R : secs[si].C); };
+      auto longside = [&](int si) { return (int)(sdir > 0 ? secs[si].C : secs[si].R); };
+      std::stable_sort(qrs.begin(), qrs.end(), [&](int a, int b) { return shortside(a) > shortside(b); });
       nqr = (int)qrs.size();
-      const int ldq = dir > 0 ? Kq : Lq;
-      const int64_t QS = (int64_t)Kq * Lq;
-      qrX = (double*)c.alloc.alloc(QS * nqr * sizeof(double));
-      qrT = (double*)c.alloc.alloc((int64_t)Kq * nqr * sizeof(double));
-      launch_fill(c, qrX, QS * nqr, 0.0);
+      std::vector<int> n2k(nqr);
+      for (int b = 0; b < nqr; ++b) n2k[b] = 2 * shortside(qrs[b]);
+      const std::vector<int> qcls = eig_classes(n2k, c.eig_batching);
+      const int nqc = (int)qcls.size() - 1;
+      std::vector<int> cK(nqc), cL(nqc, 0);
+      std::vector<int64_t> xo(nqc + 1, 0), to(nqc + 1, 0), ao(nqc + 1, 0), wo(nqc + 1, 0), vo(nqc + 1, 0);
+      for (int k = 0; k < nqc; ++k) {
+        const int cnt = qcls[k + 1] - qcls[k];
+        cK[k] = shortside(qrs[qcls[k]]);
+        for (int b = qcls[k]; b < qcls[k + 1]; ++b) cL[k] = std::max(cL[k], longside(qrs[b]));
+        xo[k + 1] = xo[k] + (int64_t)cK[k] * cL[k] * cnt;
+        to[k + 1] = to[k] + (int64_t)cK[k] * cnt;
+        ao[k + 1] = ao[k] + (int64_t)4 * cK[k] * cK[k] * cnt;
+        wo[k + 1] = wo[k] + (int64_t)2 * cK[k] * cnt;
+        vo[k + 1] = vo[k] + (int64_t)cK[k] * cK[k] * cnt;
+      }
+      qrX = (double*)c.alloc.alloc(xo[nqc] * sizeof(double));
+      qrT = (double*)c.alloc.alloc(to[nqc] * sizeof(double));
+      launch_fill(c, qrX, xo[nqc], 0.0);
       std::vector<CopyOp> place;
-      for (int b = 0; b < nqr; ++b) {
-        Sec& s = secs[qrs[b]];
-        place.push_back({s.M, qrX + b * QS, s.R, ldq, (int)s.R, (int)s.C, nullptr, nullptr, 1.0});
+      for (int k = 0; k < nqc; ++k) {
+        const int ldq = dir > 0 ? cK[k] : cL[k];
+        const int64_t QS = (int64_t)cK[k] * cL[k];
+        for (int b = qcls[k]; b < qcls[k + 1]; ++b) {
+          Sec& s = secs[qrs[b]];
+          place.push_back({s.M, qrX + xo[k] + (b - qcls[k]) * QS, s.R, ldq, (int)s.R, (int)s.C, nullptr, nullptr, 1.0});
+        }
       }
       launch_copy(c, place);
-      rocblas_status st = dir > 0
-                              ? rocsolver_dgelqf_strided_batched(hb, Kq, Lq, qrX, Kq, QS, qrT, Kq, nqr)
-                              : rocsolver_dgeqrf_strided_batched(hb, Lq, Kq, qrX, Lq, QS, qrT, Kq, nqr);
-      if (st != rocblas_status_success) throw Error(DMRJ_ERR_SOLVER, "rocsolver LQ/QR (strided batched) failed");
-      const int Nq = 2 * Kq;
-      const int64_t NNq = (int64_t)Nq * Nq;
-      qaug = (double*)c.alloc.alloc(NNq * nqr * sizeof(double));
-      qW = (double*)c.alloc.alloc((int64_t)Nq * nqr * sizeof(double));
-      qE = (double*)c.alloc.alloc((int64_t)Nq * nqr * sizeof(double));
-      qVt = (double*)c.alloc.alloc((int64_t)Kq * Kq * nqr * sizeof(double));
+      for (int k = 0; k < nqc; ++k) {
+        const int Kk = cK[k], Lk = cL[k], cnt = qcls[k + 1] - qcls[k];
+        const int64_t QS = (int64_t)Kk * Lk;
+        rocblas_status st =
+            dir > 0 ? rocsolver_dgelqf_strided_batched(hb, Kk, Lk, qrX + xo[k], Kk, QS, qrT + to[k], Kk, cnt)
+                    : rocsolver_dgeqrf_strided_batched(hb, Lk, Kk, qrX + xo[k], Lk, QS, qrT + to[k], Kk, cnt);
+        if (st != rocblas_status_success) throw Error(DMRJ_ERR_SOLVER, "rocsolver LQ/QR (strided batched) failed");
+      }
+      qaug = (double*)c.alloc.alloc(ao[nqc] * sizeof(double));
+      qW = (double*)c.alloc.alloc(wo[nqc] * sizeof(double));
+      qE = (double*)c.alloc.alloc(wo[nqc] * sizeof(double));
+      qVt = (double*)c.alloc.alloc(vo[nqc] * sizeof(double));
       qinfo = (rocblas_int*)c.alloc.alloc(sizeof(rocblas_int) * nqr);
-      launch_fill(c, qaug, NNq * nqr, 0.0);
+      launch_fill(c, qaug, ao[nqc], 0.0);
       std::vector<TriJob> tj;
-      for (int b = 0; b < nqr; ++b) {
-        Sec& s = secs[qrs[b]];
-        tj.push_back({qrX + b * QS, qaug + b * NNq, ldq, s.kmin, Nq, dir > 0 ? 0 : 1});
+      for (int k = 0; k < nqc; ++k) {
+        const int Nq = 2 * cK[k], ldq = dir > 0 ? cK[k] : cL[k];
+        const int64_t QS = (int64_t)cK[k] * cL[k], NNq = (int64_t)Nq * Nq;
+        for (int b = qcls[k]; b < qcls[k + 1]; ++b) {
+          Sec& s = secs[qrs[b]];
+          tj.push_back({qrX + xo[k] + (b - qcls[k]) * QS, qaug + ao[k] + (b - qcls[k]) * NNq, ldq, s.kmin, Nq,
+                        dir > 0 ? 0 : 1});
+        }
       }
       launch_tri_place(c, tj);
-      st = rocsolver_dsyevd_strided_batched(hb, rocblas_evect_original, rocblas_fill_upper, Nq, qaug, Nq, NNq, qW, Nq,
-                                            qE, Nq, qinfo, nqr);
-      if (st != rocblas_status_success) throw Error(DMRJ_ERR_SOLVER, "rocsolver_dsyevd_strided_batched failed");
+      for (int k = 0; k < nqc; ++k) {
+        const int Nq = 2 * cK[k];
+        rocblas_status st = rocsolver_dsyevd_strided_batched(
+            hb, rocblas_evect_original, rocblas_fill_upper, Nq, qaug + ao[k], Nq, (int64_t)Nq * Nq, qW + wo[k], Nq,
+            qE + wo[k], Nq, qinfo + qcls[k], qcls[k + 1] - qcls[k]);
+        if (st != rocblas_status_success) throw Error(DMRJ_ERR_SOLVER, "rocsolver_dsyevd_strided_batched failed");
+      }
       std::vector<AugJob> jobs;
-      for (int b = 0; b < nqr; ++b) {
-        Sec& s = secs[qrs[b]];
-        const int K = s.kmin;
-        s.X = (double*)c.alloc.alloc((int64_t)K * K * sizeof(double));
-        jobs.push_back({qaug + b * NNq, qW + (int64_t)b * Nq, s.X, s.s, qVt + (int64_t)b * Kq * Kq, K, K, Nq, K});
-        s.bj = 1;
-        s.qr = 1;
-        s.rowsX = K;
-        is_small[qrs[b]] = 1;  // handled
-        const double kk = (double)K, ll = (double)(dir > 0 ? s.C : s.R), n2 = 2.0 * kk;
-        svd_flops += 2.0 * ll * kk * kk - 2.0 / 3.0 * kk * kk * kk + 9.0 * n2 * n2 * n2;
+      for (int k = 0; k < nqc; ++k) {
+        const int Kk = cK[k], Nq = 2 * Kk;
+        const int64_t NNq = (int64_t)Nq * Nq;
+        for (int b = qcls[k]; b < qcls[k + 1]; ++b) {
+          Sec& s = secs[qrs[b]];
+          const int K = s.kmin;
+          const int64_t j = b - qcls[k];
+          s.X = (double*)c.alloc.alloc((int64_t)K * K * sizeof(double));
+          jobs.push_back({qaug + ao[k] + j * NNq, qW + wo[k] + j * Nq, s.X, s.s, qVt + vo[k] + j * Kk * Kk, K, K, Nq,
+                          K});
+          s.bj = 1;
+          s.qr = 1;
+          s.rowsX = K;
+          is_small[qrs[b]] = 1;  // handled
+          const double kk = (double)K, ll = (double)longside(qrs[b]), n2 = 2.0 * kk;
+          svd_flops += 2.0 * ll * kk * kk - 2.0 / 3.0 * kk * kk * kk + 9.0 * n2 * n2 * n2;
+        }
       }
       launch_aug_extract(c, jobs);
     }

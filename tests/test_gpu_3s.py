@@ -194,12 +194,15 @@ def test_dmrg3s_sweeps_match_oracle(pkg, ctx, qn):
     assert abs(E - oE) <= TOL * abs(oE)
 
 
+@pytest.mark.parametrize("batch", [1, 2], ids=["classes", "per_sector"])
 @pytest.mark.parametrize("L,m,n", [(24, 32, 6), (20, 160, 4)], ids=["L24m32", "L20m160"])
-def test_dmrg3s_u1_longer_runs_match_oracle(pkg, ctx, L, m, n):
+def test_dmrg3s_u1_longer_runs_match_oracle(pkg, ctx, L, m, n, batch):
     """Longer 3S runs with many small singular values kept (cutoff 1e-12):
     per-sweep energies against the oracle.  At m = 160 the enlarged sectors
-    exceed 64 on both sides, so the large-sector SVD path runs."""
-    say("case 3S sweeps U(1) L=%d m=%d" % (L, m))
+    exceed 64 on both sides, so the large-sector SVD path runs (its LQ/QR
+    form), with the eigensolves batched by the cost model (one batch at these
+    sizes) or one batch per sector."""
+    say("case 3S sweeps U(1) L=%d m=%d eigensolve batching %d" % (L, m, batch))
     states = [i % 2 for i in range(L)]
     qphys = np.array([[1], [-1]])
     mpo_h = om.makeMPO(om.heisenbergMPO(), 2, L)
@@ -207,7 +210,11 @@ def test_dmrg3s_u1_longer_runs_match_oracle(pkg, ctx, L, m, n):
     oE, ohist = o3.dmrg3S(opsi, mpo_h, m=m, sweeps=n, cutoff=1e-12)
     psi = pkg.product_state(states, 2, qphys=qphys, ctx=ctx)
     params = pkg.algvars(psi)
-    pkg.dmrg(psi, pkg.MPO(mpo_h, qphys=qphys), m=m, sweeps=n, cutoff=1e-12, method="3S", params=params)
+    ctx.set_eig_batching(batch)
+    try:
+        pkg.dmrg(psi, pkg.MPO(mpo_h, qphys=qphys), m=m, sweeps=n, cutoff=1e-12, method="3S", params=params)
+    finally:
+        ctx.set_eig_batching(1)
     for h, oh in zip(params.history, ohist):
         rel = abs(h["energy"] - oh["energy"]) / abs(oh["energy"])
         say(" sweep", h["sweep"], "%.15f" % h["energy"], "%.15f" % oh["energy"], "rel %.2e" % rel,
