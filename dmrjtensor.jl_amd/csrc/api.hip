@@ -1347,37 +1347,55 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
   rocblas_int* qinfo = nullptr;
   int nqr = 0;
   if (z) {
-    // complex: every sector through one batched Hermitian eigensolve of
-    // [[0, M], [M^H, 0]] (eigenvalues +-sigma)
+    // complex: every sector through batched Hermitian eigensolves of
+    // [[0, M], [M^H, 0]] (eigenvalues +-sigma), batched by size (eig_classes)
     std::vector<int> big;
-    int N = 0;
     for (int si : order) {
       Sec& s = secs[si];
       if (s.kmin == 0) continue;
       big.push_back(si);
-      N = std::max<int>(N, (int)(s.R + s.C));
     }
     if (!big.empty()) {
+      std::stable_sort(big.begin(), big.end(),
+                       [&](int a, int b) { return secs[a].R + secs[a].C > secs[b].R + secs[b].C; });
       naug = (int)big.size();
-      const int64_t NN = (int64_t)N * N;
-      aug = (double*)c.alloc.alloc(2 * NN * naug * sizeof(double));
-      augW = (double*)c.alloc.alloc((int64_t)N * naug * sizeof(double));
-      augE = (double*)c.alloc.alloc((int64_t)N * naug * sizeof(double));
+      std::vector<int> nsz(naug);
+      for (int b = 0; b < naug; ++b) nsz[b] = (int)(secs[big[b]].R + secs[big[b]].C);
+      const std::vector<int> cls = eig_classes(nsz, c.eig_batching);
+      const int ncls = (int)cls.size() - 1;
+      std::vector<int64_t> aoff(ncls + 1, 0), voff(ncls + 1, 0);
+      for (int k = 0; k < ncls; ++k) {
+        const int64_t Nk = nsz[cls[k]], cnt = cls[k + 1] - cls[k];
+        aoff[k + 1] = aoff[k] + 2 * Nk * Nk * cnt;  // interleaved complex
+        voff[k + 1] = voff[k] + Nk * cnt;
+      }
+      aug = (double*)c.alloc.alloc(aoff[ncls] * sizeof(double));
+      augW = (double*)c.alloc.alloc(voff[ncls] * sizeof(double));
+      augE = (double*)c.alloc.alloc(voff[ncls] * sizeof(double));
       auginfo = (rocblas_int*)c.alloc.alloc(sizeof(rocblas_int) * naug);
-      launch_fill(c, aug, 2 * NN * naug, 0.0);
+      launch_fill(c, aug, aoff[ncls], 0.0);
       std::vector<AugJobZ> jobs;
-      for (int b = 0; b < naug; ++b) {
-        Sec& s = secs[big[b]];
-        jobs.push_back({s.M, aug + 2 * b * NN, augW + (int64_t)b * N, s.U, s.s, s.Vt, (int)s.R, (int)s.C, N, s.kmin});
-        is_small[big[b]] = 1;  // handled
-        const double n2 = (double)N;
-        svd_flops += 4.0 * 9.0 * n2 * n2 * n2;
+      for (int k = 0; k < ncls; ++k) {
+        const int Nk = nsz[cls[k]];
+        const int64_t NNk = (int64_t)Nk * Nk;
+        for (int b = cls[k]; b < cls[k + 1]; ++b) {
+          Sec& s = secs[big[b]];
+          const int64_t j = b - cls[k];
+          jobs.push_back({s.M, aug + aoff[k] + 2 * j * NNk, augW + voff[k] + j * Nk, s.U, s.s, s.Vt, (int)s.R,
+                          (int)s.C, Nk, s.kmin});
+          is_small[big[b]] = 1;  // handled
+          const double n2 = (double)Nk;
+          svd_flops += 4.0 * 9.0 * n2 * n2 * n2;
+        }
       }
       launch_aug_place_z(c, jobs);
-      rocblas_status st = rocsolver_zheevd_strided_batched(hb, rocblas_evect_original, rocblas_fill_upper, N,
-                                                           (rocblas_double_complex*)aug, N, NN, augW, N, augE, N,
-                                                           auginfo, naug);
-      if (st != rocblas_status_success) throw Error(DMRJ_ERR_SOLVER, "rocsolver_zheevd_strided_batched failed");
+      for (int k = 0; k < ncls; ++k) {
+        const int Nk = nsz[cls[k]];
+        rocblas_status st = rocsolver_zheevd_strided_batched(
+            hb, rocblas_evect_original, rocblas_fill_upper, Nk, (rocblas_double_complex*)(aug + aoff[k]), Nk,
+            (int64_t)Nk * Nk, augW + voff[k], Nk, augE + voff[k], Nk, auginfo + cls[k], cls[k + 1] - cls[k]);
+        if (st != rocblas_status_success) throw Error(DMRJ_ERR_SOLVER, "rocsolver_zheevd_strided_batched failed");
+      }
       launch_aug_extract_z(c, jobs);
     }
   } else if (c.svd_backend == 0) {

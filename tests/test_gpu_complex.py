@@ -103,11 +103,13 @@ def test_complex_envs_heff_theta(pkg, ctx, mixed):
     assert abs(lhs.imag) <= 1e-10 * abs(lhs)
 
 
-def test_complex_svd_truncation(pkg, ctx):
+@pytest.mark.parametrize("batch", [1, 2, 0])
+def test_complex_svd_truncation(pkg, ctx, batch):
     """Complex two-site SVD with charges: kept singular values against numpy
     per sector, left factor orthonormal, product equal to the truncated
-    tensor."""
-    say("case complex SVD")
+    tensor; the sectors' Hermitian eigensolves batched by the cost model,
+    one per sector, or all in one."""
+    say("case complex SVD, eigensolve batching", batch)
     rng = np.random.default_rng(9)
     q = lambda n, c: np.array([[c]] * n)
     qL = np.vstack([q(3, -1), q(5, 1)])
@@ -127,8 +129,12 @@ def test_complex_svd_truncation(pkg, ctx):
         te = C.c_double()
         k = C.c_int64()
         sv = np.zeros(64)
-        ctx.call("dmrj_svd_trunc2", dev.h, m, 2, 0.0, 1, C.byref(l), C.byref(r), C.byref(te), C.byref(k),
-                 pkg._lib.dptr(sv), 64)
+        ctx.set_eig_batching(batch)
+        try:
+            ctx.call("dmrj_svd_trunc2", dev.h, m, 2, 0.0, 1, C.byref(l), C.byref(r), C.byref(te), C.byref(k),
+                     pkg._lib.dptr(sv), 64)
+        finally:
+            ctx.set_eig_batching(1)
         U, s, V, terr, qn = od.fsvd(AA, m, 2, 0.0, qL, qP, qR)
         kk = k.value
         say(" m", m, "kept", kk, "truncerr %.3e %.3e" % (te.value, terr))
