@@ -240,8 +240,11 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(pkg, psi, mpo, params, args.cpu_seconds, args.method)
     mname = {"twosite": "two-site", "3S": "3S single-site with subspace expansion"}[args.method]
+    form = "FP64 GPU drop-in" if args.precision == "f64" else "%s H_eff*psi" % args.precision
+    sym = "U(1)xU(1)" if args.model == "hubbard" else "U(1)"
+    mpo_name = {"heisenberg": "Heisenberg", "hubbard": "Fermi-Hubbard", "j1j2": "J1-J2 cylinder"}[args.model]
     out = {
-        "metric": "DMRG sweep time + H_eff*psi TFLOP/s (FP64 GPU drop-in, %s, U(1))" % mname,
+        "metric": "DMRG sweep time + H_eff*psi TFLOP/s (%s, %s, %s)" % (form, mname, sym),
         "value": round(tflops, 4),
         "unit": "TFLOP/s",
         "n_gpus": 1,
@@ -254,7 +257,7 @@ def main():
         "dtype": {"f64": "f64", "bf16x3": "bf16x3 (FP32 accumulate, FP64 Lanczos scalars/SVD/energy)",
                   "f16x3": "f16x3 per-block scaled (FP32 accumulate, FP64 Lanczos scalars/SVD/energy)"}[
                       args.precision],
-        "data": "synthetic (seeded random product state, Heisenberg MPO)",
+        "data": "synthetic (seeded random product state, %s MPO)" % mpo_name,
         "config": {"workload": "%s, %s DMRG sweep at m=%d (ramp %s), Krylov 2, cutoff %g"
                                % (desc, mname, m, args.ramp, args.cutoff), "model": args.model,
                    "method": args.method, "ramp_method": "twosite",
