@@ -353,6 +353,14 @@ def bench_compare(pkg, ctx, args, psi, mpo, desc, L, m, base_params):
     unless the state is converged; the centre-bond Lanczos step from one
     state isolates the arithmetic."""
     variants = [("f64", None, 0), ("f64", None, 5), ("bf16x3", "f64", 5), ("f16x3", "f64", 5)]
+    # the centre-bond step needs the ramp's environments; free them afterwards
+    # so that only one variant's environments are resident at a time
+    centre = centre_step_energies(pkg, ctx, psi, mpo, base_params)
+    for env in (base_params.Lenv, base_params.Renv):
+        for e in (env.V if env is not None else []):
+            if e is not None:
+                e.free()
+    base_params.Lenv = base_params.Renv = None
     rows = []
     for prec, envp, backend in variants:
         st = psi.copy()
@@ -394,7 +402,7 @@ def bench_compare(pkg, ctx, args, psi, mpo, desc, L, m, base_params):
         "data": "synthetic (seeded random product state)",
         "config": {"workload": "%s, two-site DMRG sweeps at m=%d from one state, Krylov 2, cutoff %g"
                                % (desc, m, args.cutoff), "L": L, "m": m},
-        "variants": rows, "centre_step": centre_step_energies(pkg, ctx, psi, mpo, base_params),
+        "variants": rows, "centre_step": centre,
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
