@@ -84,6 +84,7 @@ SIGNATURES = {
     "dmrj_twostep": [_p, _p, _p, _p, _p, _p, _p, C.c_int, _i64, _i64, C.c_double, C.c_int,
                      C.POINTER(_p), C.POINTER(_p), _dp, _dp, _i64p, _dp, _i64],
     "dmrj_dot": [_p, _p, _p, _dp],
+    "dmrj_dots_noalloc": [_p, C.c_int, C.POINTER(_p), C.POINTER(_p), _p],
     "dmrj_norm": [_p, _p, _dp],
     "dmrj_scale": [_p, _p, C.c_double],
     "dmrj_axpy": [_p, C.c_double, _p, _p],

@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 
 #include "../../include/dmrj.h"
@@ -2539,6 +2540,35 @@ std::vector<Charge> charges_vec(const int32_t* q, int nq, int64_t n) {
 
 }  // namespace
 
+// Scratch of dmrj_dots_noalloc: the per-work-group partial sums of up to
+// kNoAllocPairs inner products (32 KiB).  Every GPU the library runs on holds
+// its own copy until the process ends.  Host code never passes the name
+// `scratch` to a kernel: it is a host-side stand-in; the device address comes
+// from hipGetSymbolAddress, looked up once per device at context creation.
+constexpr int kNoAllocPairs = 16;
+static __device__ __align__(256) double scratch[kNoAllocPairs * kDotPartials];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (which must be `device`),
+// looked up on the first call for that device.
+double* noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) throw Error(DMRJ_ERR_ARG, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw Error(DMRJ_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = (double*)p;
+  }
+  return g_scratch_ptr[device];
+}
+}  // namespace
+
 // =================================================================== C ABI
 extern "C" {
 
@@ -2569,7 +2599,8 @@ int dmrj_ctx_create(int device, dmrj_ctx** out) {
     HIP_OK(hipMalloc((void**)&c->scal, S_COUNT * sizeof(double)));
     HIP_OK(hipMemset(c->scal, 0, S_COUNT * sizeof(double)));
     HIP_OK(hipHostMalloc((void**)&c->hscal, S_COUNT * sizeof(double), hipHostMallocDefault));
-    HIP_OK(hipMalloc((void**)&c->partial, 4 * 256 * sizeof(double)));
+    HIP_OK(hipMalloc((void**)&c->partial, 4 * kDotPartials * sizeof(double)));
+    c->noalloc = noalloc_scratch(device);
     c->stage.cap = 8 << 20;
     HIP_OK(hipHostMalloc((void**)&c->stage.base, c->stage.cap, hipHostMallocDefault));
     for (auto& e : c->graph_done) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -3300,6 +3331,28 @@ int dmrj_dot(dmrj_ctx* c, const dmrj_tensor* x, const dmrj_tensor* y, double* ou
     const double* ys[1] = {y->data};
     launch_dots(*c, 1, xs, ys, x->nstore(), c->scal + S_TT);  // Re <x|y> for complex data
     *out = read_scalar(*c, S_TT);
+  });
+}
+
+int dmrj_dots_noalloc(dmrj_ctx* c, int npairs, const dmrj_tensor* const* xs, const dmrj_tensor* const* ys,
+                      double* out_dev) {
+  return guard(c, [&] {
+    if (!xs || !ys || !out_dev) throw Error(DMRJ_ERR_ARG, "null pointer");
+    if (npairs < 1 || npairs > kNoAllocPairs)
+      throw Error(DMRJ_ERR_ARG, "dots_noalloc: npairs must be 1.." + std::to_string(kNoAllocPairs) +
+                                    " (the scratch holds the partial sums of at most that many inner products)");
+    for (int p = 0; p < npairs; ++p) {
+      if (!xs[p] || !ys[p]) throw Error(DMRJ_ERR_ARG, "null tensor");
+      if (xs[p]->nstore() != ys[p]->nstore()) throw Error(DMRJ_ERR_ARG, "dots_noalloc: size mismatch");
+    }
+    if (!c->noalloc) throw Error(DMRJ_ERR_STATE, "no scratch address for this device");
+    // one pair per launch pair, as dmrj_dot: identical partial sums and
+    // summation order, so each result equals dmrj_dot's bit for bit
+    for (int p = 0; p < npairs; ++p) {
+      const double* xp[1] = {xs[p]->data};
+      const double* yp[1] = {ys[p]->data};
+      launch_dots_buf(*c, 1, xp, yp, xs[p]->nstore(), out_dev + p, c->noalloc + (int64_t)p * kDotPartials);
+    }
   });
 }
 

@@ -199,6 +199,7 @@ struct Context {
   double* scal = nullptr;       // device scalars for Lanczos
   double* hscal = nullptr;      // pinned mirror
   double* partial = nullptr;    // reduction partials
+  double* noalloc = nullptr;    // device address of dmrj_dots_noalloc's scratch on this device
   int num_cu = 256;
   size_t lds_max = 65536;  // dynamic LDS per work-group (device property)
   // tuning: preferred GEMM tile config per operation (-1 = automatic)
@@ -369,6 +370,10 @@ void launch_cgs_pass(Context& c, double* w, const double* V, int64_t nv, int64_t
 void launch_krylov_tail(Context& c, double* V, int64_t nv, int j, double* w, int64_t n, int cplx, double* sc,
                         double* xin, int last);
 constexpr int64_t kKrylovFusedMax = 16384;  // stored elements of a Krylov vector for the one-work-group path
+constexpr int kDotPartials = 256;  // per-pair partial sums of launch_dots
+// launch_dots with the partials in `partial` (npairs * kDotPartials doubles)
+void launch_dots_buf(Context& c, int npairs, const double* const* xs, const double* const* ys, int64_t n,
+                     double* out, double* partial);
 void launch_dots(Context& c, int npairs, const double* const* xs, const double* const* ys,
                  int64_t n, double* out);
 void launch_axpy_dev(Context& c, double* y, const double* x, int64_t n, const double* coef);

@@ -700,19 +700,25 @@ void launch_fill(Context& c, double* p, int64_t n, double v) {
   HIP_OK(hipGetLastError());
 }
 
-void launch_dots(Context& c, int npairs, const double* const* xs, const double* const* ys,
-                 int64_t n, double* out) {
+static_assert(kDotBlocks == kDotPartials, "partials per pair");
+
+void launch_dots_buf(Context& c, int npairs, const double* const* xs, const double* const* ys, int64_t n,
+                     double* out, double* partial) {
   if (npairs < 1 || npairs > 4) throw Error(-2, "launch_dots: 1..4 pairs");
   DotArgs a{};
   for (int i = 0; i < npairs; ++i) {
     a.x[i] = xs[i];
     a.y[i] = ys[i];
   }
-  hipLaunchKernelGGL(k_dot_partial, dim3(kDotBlocks, npairs), dim3(kThreads), 0, c.stream, a, n,
-                     c.partial);
+  hipLaunchKernelGGL(k_dot_partial, dim3(kDotBlocks, npairs), dim3(kThreads), 0, c.stream, a, n, partial);
   HIP_OK(hipGetLastError());
-  hipLaunchKernelGGL(k_dot_final, dim3(npairs), dim3(kThreads), 0, c.stream, c.partial, out);
+  hipLaunchKernelGGL(k_dot_final, dim3(npairs), dim3(kThreads), 0, c.stream, partial, out);
   HIP_OK(hipGetLastError());
+}
+
+void launch_dots(Context& c, int npairs, const double* const* xs, const double* const* ys,
+                 int64_t n, double* out) {
+  launch_dots_buf(c, npairs, xs, ys, n, out, c.partial);
 }
 
 void launch_axpy_dev(Context& c, double* y, const double* x, int64_t n, const double* coef) {

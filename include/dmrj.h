@@ -321,6 +321,19 @@ int dmrj_scale(dmrj_ctx* ctx, dmrj_tensor* x, double a);
 int dmrj_axpy(dmrj_ctx* ctx, double a, const dmrj_tensor* x, dmrj_tensor* y);
 /* div!(psi[oc], norm(psi[oc])) without a host round trip (algs/DMRG.jl:365-369). */
 int dmrj_normalize(dmrj_ctx* ctx, dmrj_tensor* x);
+/* Allocation-free inner products.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).
+ * out_dev[p] = Re <xs[p]|ys[p]> for p < npairs, written to DEVICE memory the
+ * caller owns (npairs doubles), enqueued on the context stream without any
+ * synchronisation; each result equals dmrj_dot's bit for bit.  The partial
+ * sums live in a fixed per-GPU scratch array of the library (32 KiB, held
+ * until the process ends): npairs must be 1..16 (larger counts are rejected
+ * with DMRJ_ERR_ARG).  Calls that overlap on one GPU (other contexts or
+ * streams) share that scratch and race; calls on one context are ordered by
+ * its stream.  No hipMalloc, hipFree or allocating library routine runs. */
+int dmrj_dots_noalloc(dmrj_ctx* ctx, int npairs, const dmrj_tensor* const* xs, const dmrj_tensor* const* ys,
+                      double* out_dev);
 
 /* ---- kernel self-tests (no counterpart in the reference) -------------- */
 /* One grouped-GEMM launch on host arrays: C[MxN] = op(A) op(B) with the tile
