@@ -84,6 +84,10 @@ SIGNATURES = {
     "dmrj_twostep": [_p, _p, _p, _p, _p, _p, _p, C.c_int, _i64, _i64, C.c_double, C.c_int,
                      C.POINTER(_p), C.POINTER(_p), _dp, _dp, _i64p, _dp, _i64],
     "dmrj_dot": [_p, _p, _p, _dp],
+    "dmrj_bj_panel_width": [C.c_int, _i64, _i64],
+    "dmrj_env_left2": [_p, _p, _p, _p, _p, C.POINTER(_p)],
+    "dmrj_env_right2": [_p, _p, _p, _p, _p, C.POINTER(_p)],
+    "dmrj_site_apply_op": [_p, _p, _dp, _i32p, _i32p, C.POINTER(_p)],
     "dmrj_dots_noalloc": [_p, C.c_int, C.POINTER(_p), C.POINTER(_p), _p],
     "dmrj_norm": [_p, _p, _dp],
     "dmrj_scale": [_p, _p, C.c_double],

@@ -54,28 +54,52 @@ def save_mps(psi, path, extra=None):
         np.savez(f, **arrays)
 
 
+def _site_arrays(z, path, i, nq_head):
+    """One site's arrays from the archive, checked against its info record
+    before anything reaches the library (which reads mL*nq, P*nq and mR*nq
+    charges and n values through raw pointers)."""
+    info = np.asarray(z["A%d_info" % i])
+    if info.ndim != 1 or info.size < 6:
+        raise ValueError("%s: site %d: info record has %d entries, expected 8" % (path, i, info.size))
+    nq, mL, P, mR, n, cplx = (int(x) for x in info[:6])
+    if nq != nq_head or not 0 <= nq <= 2:
+        raise ValueError("%s: site %d carries nq=%d, the header says %d" % (path, i, nq, nq_head))
+    if min(mL, P, mR) < 1 or n < 0 or cplx not in (0, 1):
+        raise ValueError("%s: site %d: bad dimensions %s" % (path, i, info[:6].tolist()))
+    w = max(nq, 1)
+    out = []
+    for name, rows in (("qL", mL), ("qP", P), ("qR", mR)):
+        q = np.ascontiguousarray(z["A%d_%s" % (i, name)], dtype=np.int32)
+        if q.shape != (rows, w):
+            raise ValueError("%s: site %d: %s has shape %s, expected %s" % (path, i, name, q.shape, (rows, w)))
+        out.append(q)
+    data = np.ascontiguousarray(z["A%d_data" % i], dtype=np.float64)
+    if data.ndim != 1 or data.size != n:
+        raise ValueError("%s: site %d holds %d values, expected %d" % (path, i, data.size, n))
+    return (nq, mL, P, mR, n, cplx), out[0], out[1], out[2], data
+
+
 def load_mps(path, ctx=None):
-    """Read an MPS written by save_mps onto the device; returns (psi, extra)."""
-    ctx = ctx or _lib.default_context()
+    """Read an MPS written by save_mps onto the device; returns (psi, extra).
+    Every array is checked against the header and the site records first; a
+    truncated or inconsistent archive raises ValueError before any device
+    call."""
     with np.load(path, allow_pickle=False) as z:
         head = json.loads(bytes(z["header"]).decode())
         if head.get("format") != FORMAT:
             raise ValueError("%s: not a %s archive" % (path, FORMAT))
-        tensors = []
-        for i in range(1, head["L"] + 1):
-            info = z["A%d_info" % i]
-            nq, mL, P, mR, n, cplx = (int(x) for x in info[:6])
-            qL = np.ascontiguousarray(z["A%d_qL" % i], dtype=np.int32)
-            qP = np.ascontiguousarray(z["A%d_qP" % i], dtype=np.int32)
-            qR = np.ascontiguousarray(z["A%d_qR" % i], dtype=np.int32)
-            data = np.ascontiguousarray(z["A%d_data" % i], dtype=np.float64)
-            if data.size != n:
-                raise ValueError("%s: site %d holds %d values, expected %d" % (path, i, data.size, n))
-            buf = data if n else np.zeros(1)
-            h = C.c_void_p()
-            ctx.call("dmrj_site_import", nq, mL, P, mR, _lib.iptr(qL), _lib.iptr(qP), _lib.iptr(qR),
-                     _lib.dptr(buf), n, cplx, C.byref(h))
-            tensors.append(DeviceTensor(ctx, h))
+        L = int(head["L"])
+        if L < 1 or len(head.get("d", [])) != L:
+            raise ValueError("%s: header length and physical dimensions disagree" % path)
+        sites = [_site_arrays(z, path, i, int(head["nq"])) for i in range(1, L + 1)]
+    ctx = ctx or _lib.default_context()
+    tensors = []
+    for (nq, mL, P, mR, n, cplx), qL, qP, qR, data in sites:
+        buf = data if n else np.zeros(1)
+        h = C.c_void_p()
+        ctx.call("dmrj_site_import", nq, mL, P, mR, _lib.iptr(qL), _lib.iptr(qP), _lib.iptr(qR),
+                 _lib.dptr(buf), n, cplx, C.byref(h))
+        tensors.append(DeviceTensor(ctx, h))
     psi = MPS.__new__(MPS)
     psi.ctx = ctx
     psi.qphys = None if head["qphys"] is None else np.asarray(head["qphys"], dtype=np.int64)

@@ -682,42 +682,50 @@ struct EnvPlan {
   double flops = 0.0;
 };
 
-// Lupdate: Lout on A.right with D = W.DR.
+// Lupdate: Lout on A.right with D = W.DR.  The bra tensor is B (bra bond
+// spaces B.left / B.right) and the ket tensor A; B == A for the environments
+// of one state, where every step below is the one-state plan.
 EnvPlan plan_left(Context& c, const EnvMeta& L, const double* Ld, const SiteMeta& A,
-                  const double* Ad, const MpoSite& W, EnvMeta& Lo, double* Lod, int zmask = 0) {
-  // zmask: bit 0 = L complex, bit 1 = A complex (then Lod holds two planes)
+                  const double* Ad, const MpoSite& W, EnvMeta& Lo, double* Lod, int zmask = 0,
+                  const SiteMeta* Bp = nullptr, const double* Bd = nullptr) {
+  // zmask: bit 0 = L complex, bit 1 = A complex, bit 2 = B complex (then Lod
+  // holds two planes)
+  const SiteMeta& B = Bp ? *Bp : A;
+  if (!Bp) Bd = Ad;
   const int zf = zmask ? 2 : 1;
-  if (!L.ket.same(A.left) || !L.bra.same(A.left))
-    throw Error(DMRJ_ERR_ARG, "left environment does not match the MPS tensor");
-  if (L.D != W.DL || A.P != W.d) throw Error(DMRJ_ERR_ARG, "MPO does not match environment/MPS");
+  if (!L.ket.same(A.left) || !L.bra.same(B.left))
+    throw Error(DMRJ_ERR_ARG, "left environment does not match the MPS tensors");
+  if (L.D != W.DL || A.P != W.d || B.P != W.d) throw Error(DMRJ_ERR_ARG, "MPO does not match environment/MPS");
   EnvPlan p;
-  const Space& S = A.left;
+  const Space& S = A.left;   // ket sectors
   const Space& T = A.right;
-  const int nL = S.nsec(), nR = T.nsec();
+  const Space& Sb = B.left;  // bra sectors
+  const Space& Tb = B.right;
+  const int nL = S.nsec(), nLb = Sb.nsec();
   // step 1 X(w,a) = L(ap,w) * Apanel(a)
   std::vector<int64_t> xoff((size_t)W.DL * nL, -1);
   int64_t nx = 0;
   for (int w = 0; w < W.DL; ++w)
     for (int a = 0; a < nL; ++a) {
       if (A.panel_cols[a] == 0) continue;
-      const int ap = S.find(S.q[a] + W.dL[w]);
+      const int ap = Sb.find(S.q[a] + W.dL[w]);
       if (ap < 0 || L.block(ap, w) < 0) continue;
       xoff[(size_t)w * nL + a] = nx;
-      nx += (int64_t)S.dim[ap] * A.panel_cols[a];
+      nx += (int64_t)Sb.dim[ap] * A.panel_cols[a];
     }
-  // step 2 Y(ap,k,u,b): blocks n_ap x n_b
-  std::vector<int64_t> yoff((size_t)nL * A.P * W.DR, -1);
+  // step 2 Y(ap,k,u,b): blocks n_ap x n_b (ap a bra sector, b a ket sector)
+  std::vector<int64_t> yoff((size_t)nLb * B.P * W.DR, -1);
   int64_t ny = 0;
-  for (int ap = 0; ap < nL; ++ap)
-    for (int k = 0; k < A.P; ++k) {
-      const int ab = A.block(ap, k);
+  for (int ap = 0; ap < nLb; ++ap)
+    for (int k = 0; k < B.P; ++k) {
+      const int ab = B.block(ap, k);
       if (ab < 0) continue;
-      const int bp = A.blocks[ab].b;
+      const int bp = B.blocks[ab].b;
       for (int u = 0; u < W.DR; ++u) {
-        const int b = T.find(T.q[bp] - W.dR[u]);
+        const int b = T.find(Tb.q[bp] - W.dR[u]);
         if (b < 0) continue;
-        yoff[((size_t)ap * A.P + k) * W.DR + u] = ny;
-        ny += (int64_t)S.dim[ap] * T.dim[b];
+        yoff[((size_t)ap * B.P + k) * W.DR + u] = ny;
+        ny += (int64_t)Sb.dim[ap] * T.dim[b];
       }
     }
   p.X = (double*)c.alloc.alloc(std::max<int64_t>(nx, 1) * zf * sizeof(double));
@@ -726,23 +734,23 @@ EnvPlan plan_left(Context& c, const EnvMeta& L, const double* Ld, const SiteMeta
     for (int a = 0; a < nL; ++a) {
       const int64_t o = xoff[(size_t)w * nL + a];
       if (o < 0) continue;
-      const int ap = S.find(S.q[a] + W.dL[w]);
+      const int ap = Sb.find(S.q[a] + W.dL[w]);
       const EnvBlock& lb = L.blocks[L.block(ap, w)];
-      p.t1.push_back({p.X + o, S.dim[ap], S.dim[ap], (int)A.panel_cols[a], (int)p.s1.size(), 1, 0.0});
-      p.s1.push_back({Ld + lb.off, Ad + A.panel_off[a], S.dim[ap], S.dim[a], S.dim[a], 0, 1.0});
+      p.t1.push_back({p.X + o, Sb.dim[ap], Sb.dim[ap], (int)A.panel_cols[a], (int)p.s1.size(), 1, 0.0});
+      p.s1.push_back({Ld + lb.off, Ad + A.panel_off[a], Sb.dim[ap], S.dim[a], S.dim[a], 0, 1.0});
     }
-  for (int ap = 0; ap < nL; ++ap)
-    for (int k = 0; k < A.P; ++k) {
-      const int ab = A.block(ap, k);
+  for (int ap = 0; ap < nLb; ++ap)
+    for (int k = 0; k < B.P; ++k) {
+      const int ab = B.block(ap, k);
       if (ab < 0) continue;
-      const int bp = A.blocks[ab].b;
+      const int bp = B.blocks[ab].b;
       for (int u = 0; u < W.DR; ++u) {
-        const int64_t yo = yoff[((size_t)ap * A.P + k) * W.DR + u];
+        const int64_t yo = yoff[((size_t)ap * B.P + k) * W.DR + u];
         if (yo < 0) continue;
-        const int b = T.find(T.q[bp] - W.dR[u]);
-        MixDst d{p.Y + yo, (int64_t)S.dim[ap] * T.dim[b], (int)p.mt.size(), 0};
+        const int b = T.find(Tb.q[bp] - W.dR[u]);
+        MixDst d{p.Y + yo, (int64_t)Sb.dim[ap] * T.dim[b], (int)p.mt.size(), 0};
         for (int w = 0; w < W.DL; ++w) {
-          const int a = S.find(S.q[ap] - W.dL[w]);
+          const int a = S.find(Sb.q[ap] - W.dL[w]);
           if (a < 0) continue;
           const int64_t xo = xoff[(size_t)w * nL + a];
           if (xo < 0) continue;
@@ -751,7 +759,7 @@ EnvPlan plan_left(Context& c, const EnvMeta& L, const double* Ld, const SiteMeta
             if (xb < 0 || A.blocks[xb].b != b) continue;
             const double cf = W.at(w, j, k, u);
             if (cf == 0.0) continue;
-            p.mt.push_back({p.X + xo + A.blocks[xb].colpos * S.dim[ap], cf});
+            p.mt.push_back({p.X + xo + A.blocks[xb].colpos * Sb.dim[ap], cf});
             d.nt++;
           }
         }
@@ -759,25 +767,24 @@ EnvPlan plan_left(Context& c, const EnvMeta& L, const double* Ld, const SiteMeta
         p.md.push_back(d);
       }
     }
-  // step 3: Lout(bp,u) = sum_k A(ap_k,k)^T * Y(ap_k,k,u,b)
-  Lo.bra = T;
+  // step 3: Lout(bp,u) = sum_k B(ap_k,k)^T * Y(ap_k,k,u,b)
+  Lo.bra = Tb;
   Lo.ket = T;
   Lo.D = W.DR;
   Lo.delta = W.dR;
   Lo.build();
-  (void)nR;
   for (const EnvBlock& ob : Lo.blocks) {
     const int bp = ob.ap, u = ob.w, b = ob.a;
-    GemmTask t{Lod + ob.off, T.dim[bp], T.dim[bp], T.dim[b], (int)p.s3.size(), 0, 0.0};
-    for (int k = 0; k < A.P; ++k) {
-      const int ap = S.find(T.q[bp] - A.qp[k]);
+    GemmTask t{Lod + ob.off, Tb.dim[bp], Tb.dim[bp], T.dim[b], (int)p.s3.size(), 0, 0.0};
+    for (int k = 0; k < B.P; ++k) {
+      const int ap = Sb.find(Tb.q[bp] - B.qp[k]);
       if (ap < 0) continue;
-      const int ab = A.block(ap, k);
-      if (ab < 0 || A.blocks[ab].b != bp) continue;
-      const int64_t yo = yoff[((size_t)ap * A.P + k) * W.DR + u];
+      const int ab = B.block(ap, k);
+      if (ab < 0 || B.blocks[ab].b != bp) continue;
+      const int64_t yo = yoff[((size_t)ap * B.P + k) * W.DR + u];
       if (yo < 0) continue;
       // bra tensor: enters complex-conjugated
-      p.s3.push_back({Ad + A.blocks[ab].off, p.Y + yo, S.dim[ap], S.dim[ap], S.dim[ap], 1 | kSegConjA, 1.0});
+      p.s3.push_back({Bd + B.blocks[ab].off, p.Y + yo, Sb.dim[ap], Sb.dim[ap], Sb.dim[ap], 1 | kSegConjA, 1.0});
       t.nseg++;
     }
     p.t3.push_back(t);
@@ -787,6 +794,7 @@ EnvPlan plan_left(Context& c, const EnvMeta& L, const double* Ld, const SiteMeta
     BufMap bm;
     bm.add(Ld, L.size, zmask & 1);
     bm.add(Ad, A.size, (zmask & 2) != 0);
+    if (Bd != Ad) bm.add(Bd, B.size, (zmask & 4) != 0);
     bm.add(p.X, nx, true);
     bm.add(p.Y, ny, true);
     bm.add(Lod, Lo.size, true);
@@ -798,68 +806,75 @@ EnvPlan plan_left(Context& c, const EnvMeta& L, const double* Ld, const SiteMeta
   return p;
 }
 
-// Rupdate: Rout (stored [bra, w, ket]) on A.left with D = W.DL.
+// Rupdate: Rout (stored [bra, w, ket]) on A.left with D = W.DL; the bra
+// tensor is B (B == A for the environments of one state).
 EnvPlan plan_right(Context& c, const EnvMeta& R, const double* Rd, const SiteMeta& A,
-                   const double* Ad, const MpoSite& W, EnvMeta& Ro, double* Rod, int zmask = 0) {
-  // zmask: bit 0 = R complex, bit 1 = A complex (then Rod holds two planes)
+                   const double* Ad, const MpoSite& W, EnvMeta& Ro, double* Rod, int zmask = 0,
+                   const SiteMeta* Bp = nullptr, const double* Bd = nullptr) {
+  // zmask: bit 0 = R complex, bit 1 = A complex, bit 2 = B complex (then Rod
+  // holds two planes)
+  const SiteMeta& B = Bp ? *Bp : A;
+  if (!Bp) Bd = Ad;
   const int zf = zmask ? 2 : 1;
-  if (!R.ket.same(A.right) || !R.bra.same(A.right))
-    throw Error(DMRJ_ERR_ARG, "right environment does not match the MPS tensor");
-  if (R.D != W.DR || A.P != W.d) throw Error(DMRJ_ERR_ARG, "MPO does not match environment/MPS");
+  if (!R.ket.same(A.right) || !R.bra.same(B.right))
+    throw Error(DMRJ_ERR_ARG, "right environment does not match the MPS tensors");
+  if (R.D != W.DR || A.P != W.d || B.P != W.d) throw Error(DMRJ_ERR_ARG, "MPO does not match environment/MPS");
   EnvPlan p;
-  const Space& S = A.left;
+  const Space& S = A.left;   // ket sectors
   const Space& T = A.right;
-  const int nL = S.nsec();
-  // step 1 X(ap,k) = A(ap,k) * Rpanel(bp): blocks over u as in R's panel
-  std::vector<int64_t> xpan((size_t)nL * A.P, -1);
+  const Space& Sb = B.left;  // bra sectors
+  const Space& Tb = B.right;
+  const int nLb = Sb.nsec();
+  // step 1 X(ap,k) = B(ap,k) * Rpanel(bp): blocks over u as in R's panel
+  std::vector<int64_t> xpan((size_t)nLb * B.P, -1);
   int64_t nx = 0;
-  for (int ap = 0; ap < nL; ++ap)
-    for (int k = 0; k < A.P; ++k) {
-      const int ab = A.block(ap, k);
+  for (int ap = 0; ap < nLb; ++ap)
+    for (int k = 0; k < B.P; ++k) {
+      const int ab = B.block(ap, k);
       if (ab < 0) continue;
-      const int bp = A.blocks[ab].b;
+      const int bp = B.blocks[ab].b;
       if (R.panel_cols[bp] == 0) continue;
-      xpan[(size_t)ap * A.P + k] = nx;
-      nx += (int64_t)S.dim[ap] * R.panel_cols[bp];
+      xpan[(size_t)ap * B.P + k] = nx;
+      nx += (int64_t)Sb.dim[ap] * R.panel_cols[bp];
     }
   // step 2 Y(ap,w): aligned with A's row panel of a = ap - dL[w]
-  std::vector<int64_t> ypan((size_t)nL * W.DL, -1);
+  std::vector<int64_t> ypan((size_t)nLb * W.DL, -1);
   int64_t ny = 0;
-  for (int ap = 0; ap < nL; ++ap)
+  for (int ap = 0; ap < nLb; ++ap)
     for (int w = 0; w < W.DL; ++w) {
-      const int a = S.find(S.q[ap] - W.dL[w]);
+      const int a = S.find(Sb.q[ap] - W.dL[w]);
       if (a < 0) continue;
       ypan[(size_t)ap * W.DL + w] = ny;
-      ny += (int64_t)S.dim[ap] * A.panel_cols[a];
+      ny += (int64_t)Sb.dim[ap] * A.panel_cols[a];
     }
   p.X = (double*)c.alloc.alloc(std::max<int64_t>(nx, 1) * zf * sizeof(double));
   p.Y = (double*)c.alloc.alloc(std::max<int64_t>(ny, 1) * zf * sizeof(double));
-  for (int ap = 0; ap < nL; ++ap)
-    for (int k = 0; k < A.P; ++k) {
-      const int64_t o = xpan[(size_t)ap * A.P + k];
+  for (int ap = 0; ap < nLb; ++ap)
+    for (int k = 0; k < B.P; ++k) {
+      const int64_t o = xpan[(size_t)ap * B.P + k];
       if (o < 0) continue;
-      const SiteBlock& ab = A.blocks[A.block(ap, k)];
+      const SiteBlock& ab = B.blocks[B.block(ap, k)];
       const int bp = ab.b;
-      p.t1.push_back({p.X + o, S.dim[ap], S.dim[ap], (int)R.panel_cols[bp], (int)p.s1.size(), 1, 0.0});
+      p.t1.push_back({p.X + o, Sb.dim[ap], Sb.dim[ap], (int)R.panel_cols[bp], (int)p.s1.size(), 1, 0.0});
       // bra tensor: enters complex-conjugated
-      p.s1.push_back({Ad + ab.off, Rd + R.panel_off[bp], S.dim[ap], T.dim[bp], T.dim[bp], kSegConjA, 1.0});
+      p.s1.push_back({Bd + ab.off, Rd + R.panel_off[bp], Sb.dim[ap], Tb.dim[bp], Tb.dim[bp], kSegConjA, 1.0});
     }
-  for (int ap = 0; ap < nL; ++ap)
+  for (int ap = 0; ap < nLb; ++ap)
     for (int w = 0; w < W.DL; ++w) {
       const int64_t yo = ypan[(size_t)ap * W.DL + w];
       if (yo < 0) continue;
-      const int a = S.find(S.q[ap] - W.dL[w]);
-      const int nap = S.dim[ap];
+      const int a = S.find(Sb.q[ap] - W.dL[w]);
+      const int nap = Sb.dim[ap];
       for (int j = 0; j < A.P; ++j) {
         const int jb = A.block(a, j);
         if (jb < 0) continue;
         const SiteBlock& abj = A.blocks[jb];
         const int b = abj.b;
         MixDst d{p.Y + yo + abj.colpos * nap, (int64_t)nap * T.dim[b], (int)p.mt.size(), 0};
-        for (int k = 0; k < A.P; ++k) {
-          const int64_t xo = xpan[(size_t)ap * A.P + k];
+        for (int k = 0; k < B.P; ++k) {
+          const int64_t xo = xpan[(size_t)ap * B.P + k];
           if (xo < 0) continue;
-          const int bp = A.blocks[A.block(ap, k)].b;
+          const int bp = B.blocks[B.block(ap, k)].b;
           for (int u = 0; u < W.DR; ++u) {
             const int rbi = R.block(bp, u);
             if (rbi < 0 || R.blocks[rbi].a != b) continue;
@@ -873,17 +888,17 @@ EnvPlan plan_right(Context& c, const EnvMeta& R, const double* Rd, const SiteMet
         p.md.push_back(d);
       }
     }
-  Ro.bra = S;
+  Ro.bra = Sb;
   Ro.ket = S;
   Ro.D = W.DL;
   Ro.delta = W.dL;
   Ro.build();
   for (const EnvBlock& ob : Ro.blocks) {
     const int ap = ob.ap, w = ob.w, a = ob.a;
-    GemmTask t{Rod + ob.off, S.dim[ap], S.dim[ap], S.dim[a], (int)p.s3.size(), 0, 0.0};
+    GemmTask t{Rod + ob.off, Sb.dim[ap], Sb.dim[ap], S.dim[a], (int)p.s3.size(), 0, 0.0};
     const int64_t yo = ypan[(size_t)ap * W.DL + w];
     if (yo >= 0 && A.panel_cols[a] > 0) {
-      p.s3.push_back({p.Y + yo, Ad + A.panel_off[a], S.dim[ap], S.dim[a], (int)A.panel_cols[a], 2, 1.0});
+      p.s3.push_back({p.Y + yo, Ad + A.panel_off[a], Sb.dim[ap], S.dim[a], (int)A.panel_cols[a], 2, 1.0});
       t.nseg = 1;
     }
     p.t3.push_back(t);
@@ -893,6 +908,7 @@ EnvPlan plan_right(Context& c, const EnvMeta& R, const double* Rd, const SiteMet
     BufMap bm;
     bm.add(Rd, R.size, zmask & 1);
     bm.add(Ad, A.size, (zmask & 2) != 0);
+    if (Bd != Ad) bm.add(Bd, B.size, (zmask & 4) != 0);
     bm.add(p.X, nx, true);
     bm.add(p.Y, ny, true);
     bm.add(Rod, Ro.size, true);
@@ -989,8 +1005,9 @@ constexpr double kEigElemUs = 1.0e-6;
 static std::vector<int> eig_classes(const std::vector<int>& n, int mode) {
   const int cnt = (int)n.size();
   std::vector<int> cls;
-  if (mode != 1) {  // 0: one batch; 2: one batch per sector
-    for (int i = 0; i < cnt; i += mode == 0 ? std::max(cnt, 1) : 1) cls.push_back(i);
+  if (mode != 1) {  // 0: one batch; 2: one batch per sector; 3: batches of two sectors
+    const int step = mode == 0 ? std::max(cnt, 1) : mode == 2 ? 1 : 2;
+    for (int i = 0; i < cnt; i += step) cls.push_back(i);
     cls.push_back(cnt);
     return cls;
   }
@@ -1162,10 +1179,9 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
       Sec& s = secs[si];
       if (s.kmin == 0 || is_small[si]) continue;
       const int rows = (int)(dir > 0 ? s.R : s.C), ncols = (int)(dir > 0 ? s.C : s.R);
-      int bw = 16;
-      const size_t lds_dyn = c.lds_max - kBjStaticLds;  // the kernel's static LDS comes on top
-      while (bw > 4 && bj_lds_bytes(rows, bw) > lds_dyn) bw >>= 1;
-      if (bj_lds_bytes(rows, bw) > lds_dyn) continue;  // too tall for LDS: augmented solver below
+      // the kernel's static LDS comes on top of the panel
+      const int bw = bj_panel_width(rows, c.lds_max, bj_static_lds());
+      if (bw == 0) continue;  // too tall for LDS: augmented solver below
       s.X = (double*)c.alloc.alloc((int64_t)rows * ncols * sizeof(double));
       if (dir > 0)
         HIP_OK(hipMemcpyAsync(s.X, s.M, (int64_t)rows * ncols * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
@@ -1760,20 +1776,32 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
       hoff += s.kmin;
     }
     launch_bj_gather(c, gat);
-    // canonical factor: orthonormal completion for numerically zero singular
-    // values (sigma <= 1e-10 sigma_max); they carry no weight
+    // canonical factor: kept vectors of small singular values (sigma <
+    // 1e-4 sigma_max, whose extracted vectors are orthogonal to the rest only
+    // to about eps / sigma) re-orthonormalised in place, then an orthonormal
+    // completion for numerically zero singular values (sigma <= 1e-10
+    // sigma_max); the latter carry no weight
     double smax = 0.0;
     for (int64_t i = 0; i < n; ++i) smax = std::max(smax, ents[i].v);
     const double szero = 1e-10 * smax;
-    std::vector<CompleteJob> cj;
+    const double ssmall = 1e-4 * smax;
+    std::vector<CompleteJob> cj, rj;
     hoff = 0;
     for (int si : order) {
       Sec& s = secs[si];
       if (s.keep > 0) {
         int c0 = s.keep;
         while (c0 > 0 && hs[hoff + c0 - 1] <= szero) --c0;
+        int c1 = c0;
+        while (c1 > 1 && hs[hoff + c1 - 1] < ssmall) --c1;
+        const int nv = (int)(dir > 0 ? s.R : s.C);
+        if (c1 < c0 && c1 >= 1) {
+          if (dir > 0)
+            rj.push_back({s.U, s.R, 1, nv, c1, c0});
+          else
+            rj.push_back({s.Vt, 1, s.kmin, nv, c1, c0});
+        }
         if (c0 < s.keep) {
-          const int nv = (int)(dir > 0 ? s.R : s.C);
           if (dir > 0)
             cj.push_back({s.U, s.R, 1, nv, c0, s.keep});
           else
@@ -1782,7 +1810,10 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
       }
       hoff += s.kmin;
     }
-    if (!z) complete_basis(c, cj);
+    if (!z) {
+      reorth_basis(c, rj);
+      complete_basis(c, cj);
+    }
     launch_gemm(c, tasks, segs, -1);
   }
   double* dscal = (double*)c.alloc.alloc(std::max<size_t>(1, scal.size()) * sizeof(double));
@@ -2685,7 +2716,7 @@ int dmrj_ctx_set_svd_backend(dmrj_ctx* c, int mode) {
 
 int dmrj_ctx_set_eig_batching(dmrj_ctx* c, int mode) {
   return guard(c, [&] {
-    if (mode < 0 || mode > 2) throw Error(DMRJ_ERR_ARG, "eigensolve batching mode must be 0..2");
+    if (mode < 0 || mode > 3) throw Error(DMRJ_ERR_ARG, "eigensolve batching mode must be 0..3");
     c->eig_batching = mode;
   });
 }
@@ -2715,7 +2746,7 @@ int dmrj_ctx_set_tile_order(dmrj_ctx* c, int mode) {
 
 int dmrj_ctx_set_autotune(dmrj_ctx* c, int mode) {
   return guard(c, [&] {
-    if (mode < 0 || mode > 2) throw Error(DMRJ_ERR_ARG, "autotune mode must be 0, 1 or 2");
+    if (mode < 0 || mode > 3) throw Error(DMRJ_ERR_ARG, "autotune mode must be 0, 1, 2 or 3");
     if (mode == 2) {
       c->tune_cache.clear();
       c->tune_hits = c->tune_misses = c->tune_ms = c->tune_skipped = 0;
@@ -3140,6 +3171,56 @@ int dmrj_env_right(dmrj_ctx* c, const dmrj_tensor* Rin, const dmrj_tensor* A, co
   });
 }
 
+int dmrj_env_left2(dmrj_ctx* c, const dmrj_tensor* Lin, const dmrj_tensor* B, const dmrj_tensor* A,
+                   const dmrj_mpo* W, dmrj_tensor** Lout) {
+  return guard(c, [&] {
+    if (!Lin || !A || !B || !W || !Lout || Lin->kind != 1 || A->kind != 0 || A->site.d1 || B->kind != 0 ||
+        B->site.d1)
+      throw Error(DMRJ_ERR_ARG, "env_left2: bad handles");
+    if (A->nq != B->nq) throw Error(DMRJ_ERR_ARG, "env_left2: bra and ket carry different charge types");
+    auto* t = new_tensor(*c, 1);
+    t->nq = A->nq;
+    EnvMeta lo;
+    lo.bra = B->site.right;
+    lo.ket = A->site.right;
+    lo.D = W->DR;
+    lo.delta = W->dR;
+    lo.build();
+    t->env = lo;
+    const int zm = (Lin->cplx ? 1 : 0) | (A->cplx ? 2 : 0) | (B->cplx ? 4 : 0);
+    t->cplx = zm ? 1 : 0;
+    alloc_data(*c, *t);
+    EnvPlan p = plan_left(*c, Lin->env, Lin->data, A->site, A->data, *W, t->env, t->data, zm, &B->site, B->data);
+    run_env(*c, p);
+    *Lout = t;
+  });
+}
+
+int dmrj_env_right2(dmrj_ctx* c, const dmrj_tensor* Rin, const dmrj_tensor* B, const dmrj_tensor* A,
+                    const dmrj_mpo* W, dmrj_tensor** Rout) {
+  return guard(c, [&] {
+    if (!Rin || !A || !B || !W || !Rout || Rin->kind != 2 || A->kind != 0 || A->site.d1 || B->kind != 0 ||
+        B->site.d1)
+      throw Error(DMRJ_ERR_ARG, "env_right2: bad handles");
+    if (A->nq != B->nq) throw Error(DMRJ_ERR_ARG, "env_right2: bra and ket carry different charge types");
+    auto* t = new_tensor(*c, 2);
+    t->nq = A->nq;
+    EnvMeta ro;
+    ro.bra = B->site.left;
+    ro.ket = A->site.left;
+    ro.D = W->DL;
+    ro.delta = W->dL;
+    ro.build();
+    t->env = ro;
+    const int zm = (Rin->cplx ? 1 : 0) | (A->cplx ? 2 : 0) | (B->cplx ? 4 : 0);
+    t->cplx = zm ? 1 : 0;
+    alloc_data(*c, *t);
+    EnvPlan p = plan_right(*c, Rin->env, Rin->data, A->site, A->data, *W, t->env, t->data, zm, &B->site, B->data);
+    run_env(*c, p);
+    *Rout = t;
+  });
+}
+
 int dmrj_theta2(dmrj_ctx* c, const dmrj_tensor* psiL, const dmrj_tensor* psiR, dmrj_tensor** theta) {
   return guard(c, [&] {
     if (!psiL || !psiR || !theta) throw Error(DMRJ_ERR_ARG, "null pointer");
@@ -3356,6 +3437,76 @@ int dmrj_dots_noalloc(dmrj_ctx* c, int npairs, const dmrj_tensor* const* xs, con
   });
 }
 
+int dmrj_site_apply_op(dmrj_ctx* c, const dmrj_tensor* A, const double* op, const int32_t* dqL, const int32_t* dqR,
+                       dmrj_tensor** out) {
+  return guard(c, [&] {
+    if (!A || !op || !out) throw Error(DMRJ_ERR_ARG, "null pointer");
+    if (A->kind != 0 || A->site.d1) throw Error(DMRJ_ERR_ARG, "site_apply_op: single-site MPS tensors only");
+    const SiteMeta& x = A->site;
+    const int nq = A->nq;
+    const int P = x.P;
+    Charge sl, sr;
+    if (nq) {
+      if (!dqL || !dqR) throw Error(DMRJ_ERR_ARG, "site_apply_op: charge shifts required for a charged tensor");
+      for (int k = 0; k < nq; ++k) {
+        sl.v[k] = dqL[k];
+        sr.v[k] = dqR[k];
+      }
+    }
+    // every non-zero O[k, j] must change the charge by q(k) - q(j) = sr - sl
+    for (int j = 0; j < P; ++j)
+      for (int k = 0; k < P; ++k) {
+        const double v = op[k + (int64_t)P * j];
+        if (!std::isfinite(v)) throw Error(DMRJ_ERR_NUMERIC, "site_apply_op: operator holds NaN/Inf");
+        if (v != 0.0 && x.qp[k] - x.qp[j] != sr - sl)
+          throw Error(DMRJ_ERR_ARG, "site_apply_op: operator entry changes the charge by other than dqR - dqL");
+      }
+    auto shifted = [](const Space& s, const Charge& d) {
+      Space r = s;
+      for (auto& q : r.q) q = q + d;
+      r.finish();
+      return r;
+    };
+    auto* n = new dmrj_tensor();
+    n->kind = 0;
+    n->nq = nq;
+    n->cplx = A->cplx;
+    n->site.P = P;
+    n->site.qp = x.qp;
+    n->site.left = shifted(x.left, sl);
+    n->site.right = shifted(x.right, sr);
+    n->site.build();
+    try {
+      alloc_data(*c, *n);
+      // block (a, k) of the result = sum_j O[k, j] * block (a, j) of A: with
+      // the shifts both blocks sit on the same right sector index
+      std::vector<MixDst> dsts;
+      std::vector<MixTerm> terms;
+      const int planes = A->cplx ? 2 : 1;
+      for (int pl = 0; pl < planes; ++pl)
+        for (const SiteBlock& ob : n->site.blocks) {
+          const int64_t ne = (int64_t)n->site.left.dim[ob.a] * n->site.right.dim[ob.b];
+          MixDst d{n->data + pl * n->site.size + ob.off, ne, (int)terms.size(), 0};
+          for (int j = 0; j < P; ++j) {
+            const int xb = x.block(ob.a, j);
+            if (xb < 0 || x.blocks[xb].b != ob.b) continue;
+            const double cf = op[ob.s + (int64_t)P * j];
+            if (cf == 0.0) continue;
+            terms.push_back({A->data + pl * x.size + x.blocks[xb].off, cf});
+            d.nt++;
+          }
+          dsts.push_back(d);
+        }
+      launch_mix(*c, dsts, terms);
+    } catch (...) {
+      if (n->data) c->alloc.release(n->data);
+      delete n;
+      throw;
+    }
+    *out = n;
+  });
+}
+
 int dmrj_dotu(dmrj_ctx* c, const dmrj_tensor* x, const dmrj_tensor* y, double* out2) {
   return guard(c, [&] {
     if (!x || !y || !out2) throw Error(DMRJ_ERR_ARG, "null pointer");
@@ -3481,6 +3632,11 @@ int dmrj_selftest_gemm(dmrj_ctx* c, int cfg, int64_t M, int64_t N, int64_t K, in
     c->alloc.release(dB);
     c->alloc.release(dC);
   });
+}
+
+int dmrj_bj_panel_width(int rows, int64_t lds_max, int64_t static_lds) {
+  if (rows < 1 || lds_max < 0 || static_lds < 0) return DMRJ_ERR_ARG;
+  return bj_panel_width(rows, (size_t)lds_max, (size_t)static_lds);
 }
 
 int dmrj_selftest_copy(dmrj_ctx* c, int64_t n) {

@@ -210,6 +210,10 @@ int autotune_gemm(Context& c, const std::vector<GemmTask>& tasks, const std::vec
     c.tune_hits += 1;
     return it->second;
   }
+  if (c.autotune == 3) {  // frozen: keys not tuned yet take the heuristic, no trials
+    c.tune_skipped += 1;
+    return choose_gemm_cfg(tasks, c.num_cu);
+  }
   if (!repeatable(tasks, segs)) {
     c.tune_skipped += 1;
     return choose_gemm_cfg(tasks, c.num_cu);

@@ -321,9 +321,13 @@ struct BjGather {
   int rows, trans;
 };
 size_t bj_lds_bytes(int rows, int bw);
-// static shared memory of k_bj_step, reserved out of the per-work-group LDS
-// limit (lds_max counts static and dynamic LDS together)
-constexpr size_t kBjStaticLds = 256;
+// static shared memory of k_bj_step (from its compiled attributes), reserved
+// out of the per-work-group LDS limit (lds_max counts static and dynamic LDS
+// together)
+size_t bj_static_lds();
+// column-block width of the block-Jacobi panel (rows x 2bw doubles in
+// dynamic LDS) for `rows` rows: 16, 8 or 4, or 0 when even 4 does not fit
+int bj_panel_width(int rows, size_t lds_max, size_t static_lds);
 void launch_bj_sweep(Context& c, const BjJob* djobs, const std::vector<int2>& work, int nsteps, int inner_max,
                      size_t lds);
 void launch_bj_norms(Context& c, const BjJob* djobs, const std::vector<int2>& work);
@@ -350,6 +354,9 @@ struct CompleteJob {
 };
 // Orthonormal completion (svd_jacobi.hip); allocates and releases its scratch.
 void complete_basis(Context& c, std::vector<CompleteJob>& jobs);
+// Re-orthonormalisation of the kept vectors c0 .. keep-1 against the vectors
+// before them and among themselves (svd_jacobi.hip); c0 >= 1.
+void reorth_basis(Context& c, std::vector<CompleteJob>& jobs);
 void launch_aug_extract_z(Context& c, const std::vector<AugJobZ>& jobs);
 
 // ---- kernel launchers (kernels.hip) ----

@@ -544,16 +544,183 @@ void complete_basis(Context& c, std::vector<CompleteJob>& jobs) {
   for (double* p : scratch) c.alloc.release(p);
 }
 
+// Re-orthonormalisation of kept vectors c0 .. keep-1 of a canonical factor
+// whose singular values are small (their vectors, taken from an augmented
+// eigenvector or a Jacobi column, are orthogonal to the rest only to about
+// eps / sigma): blocks of kCB vectors, two classical Gram-Schmidt passes
+// against all vectors before the block on the grouped GEMM, then inside the
+// block two Cholesky-QR passes (G = W^T W = R^T R, W <- W R^-1; G is within
+// ~1e-6 of the identity here, so the factorisation is benign).  Each vector
+// moves by its own non-orthogonal part only.  b < 0: load block 0 into W;
+// b >= 0: finish block b (projected against vectors 0 .. c0 + kCB b - 1),
+// store it, load block b + 1.
+__device__ void rb_load(const CompleteJob& jb, int c, int s) {
+  const int n = jb.n;
+  if (jb.es == 1) {
+    for (int u = 0; u < s; ++u)
+      for (int r = threadIdx.x; r < n; r += kBjThreads)
+        jb.W[(int64_t)u * n + r] = jb.Q[(int64_t)(c + u) * jb.vs + r];
+  } else {
+    for (int64_t e = threadIdx.x; e < (int64_t)s * n; e += kBjThreads) {
+      const int u = (int)(e % s), r = (int)(e / s);
+      jb.W[(int64_t)u * n + r] = jb.Q[(int64_t)(c + u) * jb.vs + (int64_t)r * jb.es];
+    }
+  }
+  __syncthreads();
+}
+
+// One Cholesky-QR pass on the nb columns of W (n rows each): G = W^T W
+// (pairs spread over the threads), G = R^T R (one thread: a 32 x 32
+// factorisation), then every row w of W is replaced by w R^-1 (a forward
+// substitution per row, one row per thread).
+__device__ void rb_cholqr(const CompleteJob& jb, int nb, double (*G)[kCB + 1]) {
+  const int n = jb.n;
+  const int npairs = nb * (nb + 1) / 2;
+  for (int pi = threadIdx.x; pi < npairs; pi += kBjThreads) {
+    int t = 0;
+    while ((t + 1) * (t + 2) / 2 <= pi) ++t;
+    const int u = pi - t * (t + 1) / 2;  // u <= t
+    const double* wu = jb.W + (int64_t)u * n;
+    const double* wt = jb.W + (int64_t)t * n;
+    double a = 0.0;
+    for (int r = 0; r < n; ++r) a += wu[r] * wt[r];
+    G[u][t] = a;
+    G[t][u] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // upper Cholesky factor in place: G[u][t] (u <= t) becomes R[u][t]
+    for (int j = 0; j < nb; ++j) {
+      double d = G[j][j];
+      for (int u = 0; u < j; ++u) d -= G[u][j] * G[u][j];
+      d = sqrt(fmax(d, 1e-300));
+      G[j][j] = d;
+      for (int t = j + 1; t < nb; ++t) {
+        double a = G[j][t];
+        for (int u = 0; u < j; ++u) a -= G[u][j] * G[u][t];
+        G[j][t] = a / d;
+      }
+    }
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < n; r += kBjThreads) {
+    double w[kCB];
+    for (int t = 0; t < nb; ++t) w[t] = jb.W[(int64_t)t * n + r];
+    for (int t = 0; t < nb; ++t) {
+      double a = w[t];
+      for (int u = 0; u < t; ++u) a -= w[u] * G[u][t];
+      w[t] = a / G[t][t];
+    }
+    for (int t = 0; t < nb; ++t) jb.W[(int64_t)t * n + r] = w[t];
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kBjThreads) void k_reorth_block(const CompleteJob* __restrict__ jobs, int njobs, int b) {
+  __shared__ double G[kCB][kCB + 1];
+  for (int ji = blockIdx.x; ji < njobs; ji += gridDim.x) {
+    const CompleteJob jb = jobs[ji];
+    const int n = jb.n;
+    int cnext = jb.c0;
+    if (b >= 0) {
+      const int cb = jb.c0 + kCB * b;
+      if (cb >= jb.keep) continue;
+      const int nb = min(kCB, jb.keep - cb);
+      rb_cholqr(jb, nb, G);
+      rb_cholqr(jb, nb, G);
+      if (jb.es == 1) {
+        for (int u = 0; u < nb; ++u)
+          for (int r = threadIdx.x; r < n; r += kBjThreads)
+            jb.Q[(int64_t)(cb + u) * jb.vs + r] = jb.W[(int64_t)u * n + r];
+      } else {
+        for (int64_t e = threadIdx.x; e < (int64_t)nb * n; e += kBjThreads) {
+          const int u = (int)(e % nb), r = (int)(e / nb);
+          jb.Q[(int64_t)(cb + u) * jb.vs + (int64_t)r * jb.es] = jb.W[(int64_t)u * n + r];
+        }
+      }
+      __syncthreads();
+      cnext = cb + nb;
+    }
+    const int nbn = min(kCB, jb.keep - cnext);
+    if (nbn > 0) rb_load(jb, cnext, nbn);
+  }
+}
+
+void reorth_basis(Context& c, std::vector<CompleteJob>& jobs) {
+  if (jobs.empty()) return;
+  std::vector<double*> scratch;
+  int nblk = 0;
+  for (CompleteJob& j : jobs) {
+    if (j.c0 < 1 || j.keep > j.n || j.c0 >= j.keep) throw Error(DMRJ_ERR_STATE, "re-orthonormalisation job out of range");
+    const int64_t sz = (int64_t)kCB * j.n + (int64_t)kCB * j.keep;
+    double* p = (double*)c.alloc.alloc(sz * (int64_t)sizeof(double));
+    scratch.push_back(p);
+    j.W = p;
+    j.coef = j.W + (int64_t)kCB * j.n;
+    nblk = std::max(nblk, (j.keep - j.c0 + kCB - 1) / kCB);
+  }
+  auto* dj = (CompleteJob*)c.upload(jobs.data(), jobs.size() * sizeof(CompleteJob));
+  const unsigned grid = (unsigned)std::min<size_t>(jobs.size(), 1024);
+  hipLaunchKernelGGL(k_reorth_block, dim3(grid), dim3(kBjThreads), 0, c.stream, dj, (int)jobs.size(), -1);
+  HIP_OK(hipGetLastError());
+  for (int b = 0; b < nblk; ++b) {
+    // W -= Q (Q^T W) twice over the vectors before the block
+    std::vector<GemmTask> t1, t2;
+    std::vector<GemmSeg> s1, s2;
+    for (const CompleteJob& j : jobs) {
+      const int cb = j.c0 + kCB * b;
+      if (cb >= j.keep) continue;
+      const int nb = std::min(kCB, j.keep - cb);
+      const bool cols = j.es == 1;
+      const int64_t ldq = cols ? j.vs : j.es;
+      t1.push_back({j.coef, j.keep, cb, nb, (int)s1.size(), 1, 0.0});
+      s1.push_back({j.Q, j.W, ldq, j.n, j.n, cols ? 1 : 0, 1.0});
+      t2.push_back({j.W, j.n, j.n, nb, (int)s2.size(), 1, 1.0});
+      s2.push_back({j.Q, j.coef, ldq, j.keep, cb, cols ? 0 : 1, -1.0});
+    }
+    if (!t1.empty())
+      for (int pass = 0; pass < 2; ++pass) {
+        launch_gemm(c, t1, s1, -1);
+        launch_gemm(c, t2, s2, -1);
+      }
+    hipLaunchKernelGGL(k_reorth_block, dim3(grid), dim3(kBjThreads), 0, c.stream, dj, (int)jobs.size(), b);
+    HIP_OK(hipGetLastError());
+  }
+  for (double* p : scratch) c.alloc.release(p);
+}
+
 size_t bj_lds_bytes(int rows, int bw) { return (size_t)2 * bw * rows * sizeof(double); }
+
+int bj_panel_width(int rows, size_t lds_max, size_t static_lds) {
+  if (static_lds >= lds_max) return 0;
+  const size_t dyn = lds_max - static_lds;
+  int bw = 16;
+  while (bw > 4 && bj_lds_bytes(rows, bw) > dyn) bw >>= 1;
+  return bj_lds_bytes(rows, bw) <= dyn ? bw : 0;
+}
+
+size_t bj_static_lds() {
+  // the compiled kernel's own static LDS, read once from its attributes
+  static size_t bytes = [] {
+    hipFuncAttributes a{};
+    if (hipFuncGetAttributes(&a, (const void*)k_bj_step) != hipSuccess) {
+      (void)hipGetLastError();
+      return (size_t)4096;  // a safe over-estimate if the attribute query fails
+    }
+    return (size_t)a.sharedSizeBytes;
+  }();
+  return bytes;
+}
 
 void launch_bj_sweep(Context& c, const BjJob* djobs, const std::vector<int2>& work, int nsteps, int inner_max,
                      size_t lds) {
   if (work.empty() || nsteps <= 0) return;
-  if (lds + kBjStaticLds > c.lds_max) throw Error(-2, "block Jacobi panel exceeds the work-group LDS");
+  const size_t static_lds = bj_static_lds();
+  if (lds + static_lds > c.lds_max) throw Error(-2, "block Jacobi panel exceeds the work-group LDS");
   static bool attr_set = false;
   if (!attr_set) {  // allow dynamic LDS up to the device limit less the kernel's static LDS
     (void)hipFuncSetAttribute((const void*)k_bj_step, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)(c.lds_max - kBjStaticLds));
+                              (int)(c.lds_max - static_lds));
     (void)hipGetLastError();
     attr_set = true;
   }

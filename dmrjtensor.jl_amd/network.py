@@ -368,6 +368,62 @@ def product_state(states, d, qphys=None, ctx=None):
     return MPS(A, oc=1, qphys=qphys, qbond=qbond, ctx=ctx, canonical=True)
 
 
+def _op_charge(op, qphys):
+    """Charge moved by a one-site operator: q(k) - q(j) for its non-zero
+    entries O[k, j] (one value, or the operator mixes charges)."""
+    nz = np.argwhere(np.asarray(op) != 0.0)
+    if len(nz) == 0:
+        return np.zeros(qphys.shape[1], dtype=np.int64)
+    dq = {tuple(int(v) for v in qphys[k] - qphys[j]) for k, j in nz}
+    if len(dq) != 1:
+        raise ValueError("applyOps!: the operator does not carry one definite charge")
+    return np.array(next(iter(dq)), dtype=np.int64)
+
+
+def _apply_site(psi, i, op, dqL, dqR):
+    op = np.asarray(op, dtype=np.float64)
+    d = psi.d[i - 1]
+    if op.shape != (d, d):
+        raise ValueError("applyOps!: operator must be %d x %d" % (d, d))
+    nq = psi.nq
+    h = C.c_void_p()
+    buf = np.ascontiguousarray(op.ravel(order="F"))
+    l = np.ascontiguousarray(dqL, dtype=np.int32) if nq else None
+    r = np.ascontiguousarray(dqR, dtype=np.int32) if nq else None
+    psi.ctx.call("dmrj_site_apply_op", psi[i].h, _lib.dptr(buf), _lib.iptr(l), _lib.iptr(r), C.byref(h))
+    psi[i].free()
+    psi[i] = DeviceTensor(psi.ctx, h)
+
+
+def applyOps(psi, sites, Op, trail=None):
+    """applyOps!(psi, sites, Op; trail) (lib/MPO_MPS/applyMPO.jl:21-33) in place
+    on the GPU: for every listed site p in order, psi[p] = Op psi[p] (Op[bra,
+    ket] acting on the physical index), then, when a trail is given, the trail
+    on every site left of p.  On a charged MPS a charged operator moves the
+    bond charges right of p by its charge (the tensors right of p are
+    relabelled, their data unchanged), so the state keeps the block rule
+    q(a) + q(s) = q(b).  Returns psi."""
+    Op = np.asarray(Op, dtype=np.float64)
+    tr = None if trail is None else np.asarray(trail, dtype=np.float64)
+    L = len(psi)
+    nq = psi.nq
+    zero = np.zeros(nq, dtype=np.int64)
+    for p in sites:
+        if not 1 <= p <= L:
+            raise ValueError("applyOps!: site %d outside 1..%d" % (p, L))
+        dq = _op_charge(Op, psi.qphys) if nq else zero
+        _apply_site(psi, p, Op, zero, dq)
+        if nq and np.any(dq != 0):
+            for i in range(p + 1, L + 1):
+                _apply_site(psi, i, np.eye(psi.d[i - 1]), dq, dq)
+        if tr is not None and not np.array_equal(tr, np.ones((1, 1))):
+            if nq and np.any(_op_charge(tr, psi.qphys) != 0):
+                raise ValueError("applyOps!: the trail must be neutral")
+            for i in range(1, p):
+                _apply_site(psi, i, tr, zero, zero)
+    return psi
+
+
 def randMPS_normal(L, d, m, seed=0, ctx=None):
     """Random dense MPS: i.i.d. N(0,1) entries from numpy default_rng(seed),
     chi_b = min(d^b, d^(L-b), m), filled site 1 -> L in column-major order,
@@ -384,27 +440,58 @@ def randMPS_normal(L, d, m, seed=0, ctx=None):
     return MPS(A, oc=1, ctx=ctx, canonical=True)
 
 
-def makeq(psi_tensors, Qlabels, oc=1, ctx=None):
-    """MPS(Qlabels, psi) for a dense host MPS whose bonds carry one charge per
-    basis state: the charge of each right-link state is the charge of the
-    largest-magnitude entry (assignflux!, lib/MPS/MPS.jl:206-219)."""
+def makeq_host(psi_tensors, Qlabels, seed=0, qleft=None):
+    """Host part of makeqMPS (lib/MPS/makeqMPS.jl:26-112) for a dense MPS:
+    the charge of every right-link state is that of the largest-magnitude
+    entry of its column (assignflux!, lib/MPS/MPS.jl:206-219: the first
+    maximum with the left index outer and the physical index inner; an
+    all-zero column takes the charge of its (1, 1) entry), entries off the charge-allowed blocks are dropped as the Qtens
+    construction drops them, and a tensor left without any non-zero allowed
+    entry is refilled with N(0,1) values (numpy default_rng(seed)) on its
+    allowed blocks, as randomize=true does (makeqMPS.jl:66-67).  qleft: the
+    charges of the first bond (default zero).  Returns (tensors, bond
+    charges)."""
     qphys = np.asarray(Qlabels, dtype=np.int64)
     nq = qphys.shape[1]
-    q = np.zeros((psi_tensors[0].shape[0], nq), dtype=np.int64)
+    rng = np.random.default_rng(seed)
+    q = np.zeros((psi_tensors[0].shape[0], nq), dtype=np.int64) if qleft is None else \
+        np.asarray(qleft, dtype=np.int64).reshape(psi_tensors[0].shape[0], nq)
     qbond = [q]
+    out = []
     for A in psi_tensors:
+        A = np.array(A, dtype=np.float64)
         a, d, b = A.shape
-        best = np.full(b, -1.0)
-        qr = np.zeros((b, nq), dtype=np.int64)
-        for ia in range(a):
-            for s in range(d):
-                for ib in range(b):
-                    v = abs(A[ia, s, ib])
-                    if v > best[ib]:
-                        best[ib] = v
-                        qr[ib] = qbond[-1][ia] + qphys[s]
+        mag = np.abs(A).reshape(a * d, b)                  # rows (a, s), s fastest: the loop order of assignflux!
+        rows = np.argmax(mag, axis=0)                      # first maximum in that order
+        qin = (qbond[-1][:, None, :] + qphys[None, :, :])  # [a, s, nq]
+        qr = qin.reshape(a * d, nq)[rows]
+        ok = np.all(qin[:, :, None, :] == qr[None, None, :, :], axis=3)   # [a, s, b]
+        A[~ok] = 0.0
+        if not np.any(A != 0.0):
+            A[ok] = rng.standard_normal(int(ok.sum()))
+        out.append(A)
         qbond.append(qr)
-    return MPS(psi_tensors, oc=oc, qphys=qphys, qbond=qbond, ctx=ctx, canonical=True)
+    return out, qbond
+
+
+def _dense_norm2(tensors):
+    E = np.ones((1, 1))
+    for A in tensors:
+        E = np.einsum("xy,xsb,ysc->bc", E, A, A)
+    return float(np.trace(E))
+
+
+def makeq(psi_tensors, Qlabels, oc=1, ctx=None, newnorm=True, seed=0):
+    """MPS(Qlabels, psi) (lib/MPS/MPS.jl:259-263, makeqMPS lib/MPS/makeqMPS.jl:
+    26-112) for a dense host MPS: charges and dropped entries by makeq_host,
+    then (newnorm) the tensor at the centre rescaled so that <psi|psi> is
+    that of the dense input (makeqMPS.jl:81-84)."""
+    tensors, qbond = makeq_host(psi_tensors, Qlabels, seed=seed)
+    if newnorm:
+        n0, n1 = _dense_norm2(psi_tensors), _dense_norm2(tensors)
+        if n1 > 0.0:
+            tensors[oc - 1] = tensors[oc - 1] * np.sqrt(n0 / n1)
+    return MPS(tensors, oc=oc, qphys=np.asarray(Qlabels, dtype=np.int64), qbond=qbond, ctx=ctx, canonical=True)
 
 
 # --------------------------------------------------------------- environment

@@ -2,10 +2,12 @@
 
 The library dispatches the empty ``k_region_marker`` kernel with as many
 work-groups as its id: 1 and 2 bracket the benchmark's timed sweeps, 3 and 4
-bracket each set of autotuner trial launches.  This script cuts kernel traces
-and counter collections to the dispatches between markers 1 and 2, less those
-between a 3 and the next 4, so that the figures describe the same launches as
-the benchmark's own HIP-event timing.
+bracket each set of autotuner trial launches, 5 and 6 the headline
+benchmark's profiled FP64 sweep, 7 and 8 its profiled mixed sweep.  This
+script cuts kernel traces and counter collections to the dispatches between
+two markers (--region, default 1 2), less those between a 3 and the next 4,
+so that the figures describe the same launches as the benchmark's own
+HIP-event timing.
 
     python rocprof_summary.py trace KERNEL_TRACE.csv [--kernel k_gemm_f64]
     python rocprof_summary.py traffic --fetch F.csv --write W.csv \
@@ -50,14 +52,17 @@ def _marker_id(r, hdr):
     return g // max(1, w)
 
 
+REGION = (1, 2)   # marker ids that open and close the region (--region)
+
+
 def _selector(markers):
     """markers: sorted (dispatch id, marker id).  Returns a predicate on
-    dispatch ids that is true inside the timed region and outside tuning
-    windows, or None when the region markers are missing."""
-    start = next((i for i, m in markers if m == 1), None)
+    dispatch ids that is true inside the region (markers REGION) and outside
+    tuning windows, or None when the region markers are missing."""
+    start = next((i for i, m in markers if m == REGION[0]), None)
     if start is None:
         return None
-    end = next((i for i, m in markers if m == 2 and i > start), None)
+    end = next((i for i, m in markers if m == REGION[1] and i > start), None)
     if end is None:
         return None
     windows, open_at = [], None
@@ -128,7 +133,7 @@ def _kernel_mean(path, counter, kernel, need_region=True):
     vals, names, markers = _per_dispatch(path, counter)
     inside = _selector(markers)
     if need_region and inside is None:
-        raise SystemExit("%s: no %s dispatches with ids 1 and 2" % (path, MARKER))
+        raise SystemExit("%s: no %s dispatches with ids %d and %d" % (path, MARKER, REGION[0], REGION[1]))
     sel = [v for i, v in vals.items() if kernel in names[i] and (not need_region or inside(i))]
     if not sel:
         raise SystemExit("%s: no %s dispatches with %s" % (path, kernel, counter))
@@ -211,7 +216,13 @@ def main(argv=None):
     mm.add_argument("csv")
     mm.add_argument("--kernel", default="k_gemm_f64")
     mm.add_argument("--json")
+    for sp in (t, f, mm):
+        sp.add_argument("--region", type=int, nargs=2, default=[1, 2], metavar=("OPEN", "CLOSE"),
+                        help="k_region_marker ids around the region (1 2: the timed sweeps; the headline "
+                             "benchmark's profiled FP64 sweep is 5 6, its profiled mixed sweep 7 8)")
     a = ap.parse_args(argv)
+    global REGION
+    REGION = tuple(a.region)
     if a.cmd == "trace":
         trace_summary(a.csv, a.kernel)
     elif a.cmd == "mfma":

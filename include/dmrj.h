@@ -70,8 +70,9 @@ int dmrj_ctx_memstats(dmrj_ctx* ctx, int64_t* out3);
  * largest size, where a cost model of the tridiagonalisation (fixed latency
  * per step against the padded elements read per step) says the saved padding
  * outweighs the extra steps; 0 = one batch padded to the largest sector;
- * 2 = one batch per sector.  Every mode gives a backward-stable result; the
- * bits differ between modes. */
+ * 2 = one batch per sector; 3 = batches of two sectors in size order (a test
+ * mode: every batch offset path is taken on small problems).  Every mode
+ * gives a backward-stable result; the bits differ between modes. */
 int dmrj_ctx_set_eig_batching(dmrj_ctx* ctx, int mode);
 /* force a GEMM tile shape (0: 32x32, 1: 64x64, 2: 128x128, 3: 64x128,
  * 4: 128x64, -1: automatic) */
@@ -80,7 +81,9 @@ int dmrj_ctx_set_gemm_cfg(dmrj_ctx* ctx, int cfg);
  * replaces the BLAS call's own blocking).  mode 1 (default) = time the five
  * tile shapes (32x32, 64x64, 128x128, 64x128, 128x64) once per
  * block-size-histogram key and reuse the fastest; 0 = fixed heuristic;
- * 2 = clear the cache and counters.  The shape never changes result bits.
+ * 2 = clear the cache and counters; 3 = frozen: keys already tuned keep
+ * their shape, new keys take the heuristic, no trial launches (a timed
+ * region free of tuning).  The shape never changes result bits.
  * A cfg set by dmrj_ctx_set_gemm_cfg takes precedence. */
 int dmrj_ctx_set_autotune(dmrj_ctx* ctx, int mode);
 /* Grouped-GEMM tile order (no counterpart in the reference): 1 (default) =
@@ -180,6 +183,19 @@ int dmrj_site_import(dmrj_ctx* ctx, int nq, int64_t mL, int64_t P, int64_t mR, c
                      const int32_t* qP, const int32_t* qR, const double* data, int64_t n, int cplx,
                      dmrj_tensor** out);
 
+/* One-site operator applied to an MPS tensor, the per-site step of
+ * applyOps! (lib/MPO_MPS/applyMPO.jl:21-33): out[a, k, b] = sum_j O[k, j]
+ * A[a, j, b] with op column-major d x d (op[k + d*j] = O[k, j], O acting on
+ * the ket as a matrix).  A charged operator moves the bond charges: the left
+ * bond of the result carries A's charges plus dqL, the right bond plus dqR
+ * (nq ints each; NULL when nq == 0), and every non-zero O[k, j] must satisfy
+ * q(k) - q(j) = dqR - dqL (else DMRJ_ERR_ARG).  applyOps! at site p with
+ * O of charge dq: dqL = 0, dqR = dq at p; dqL = dqR = dq with O = identity
+ * on the sites right of p; the trail on the sites left of p has dq = 0.
+ * Real or ComplexF64 tensors, real operators. */
+int dmrj_site_apply_op(dmrj_ctx* ctx, const dmrj_tensor* A, const double* op, const int32_t* dqL,
+                       const int32_t* dqR, dmrj_tensor** out);
+
 /* ---- ComplexF64 tensors (the TDVP path, algs/time.jl:288-362, evolves
  * ComplexF64 MPS).  Host arrays are Julia's ComplexF64 layout: interleaved
  * (re, im) pairs, column-major.  On the device a complex tensor keeps the
@@ -230,6 +246,17 @@ int dmrj_env_left(dmrj_ctx* ctx, const dmrj_tensor* Lin, const dmrj_tensor* A,
  * sum A[a,j,b] W[w,j,k,u] conj(A[a',k,b']) R[b,u,b']. */
 int dmrj_env_right(dmrj_ctx* ctx, const dmrj_tensor* Rin, const dmrj_tensor* A,
                    const dmrj_mpo* W, dmrj_tensor** Rout);
+
+/* Two-state forms: Lupdate(Lenv, dualpsi, psi, mpo) / Rupdate(Renv, dualpsi,
+ * psi, mpo) (lib/MPO_MPS/Lupdate.jl:17-26, Rupdate.jl:17-26) with the bra B
+ * (dualpsi, enters conjugated) and the ket A (psi) different tensors; Lin's
+ * bra / ket bonds are B's / A's left bonds (Rin: their right bonds), the
+ * result's bra / ket bonds their right (left) bonds.  The environments of
+ * correlation and of the two-state correlationmatrix. */
+int dmrj_env_left2(dmrj_ctx* ctx, const dmrj_tensor* Lin, const dmrj_tensor* B, const dmrj_tensor* A,
+                   const dmrj_mpo* W, dmrj_tensor** Lout);
+int dmrj_env_right2(dmrj_ctx* ctx, const dmrj_tensor* Rin, const dmrj_tensor* B, const dmrj_tensor* A,
+                    const dmrj_mpo* W, dmrj_tensor** Rout);
 
 /* ---- H_eff and Lanczos ----------------------------------------------- */
 /* psiL*psiR: contract(psiL,3,psiR,1) (algs/DMRG.jl:60). */
@@ -351,6 +378,12 @@ int dmrj_selftest_mfma_f64(dmrj_ctx* ctx, double* out768);
  * the calibration kernel of known traffic (8n bytes read, 8n written) for
  * the FETCH_SIZE / WRITE_SIZE hardware counters. */
 int dmrj_selftest_copy(dmrj_ctx* ctx, int64_t n);
+/* Column-block width of the block one-sided Jacobi panel (backend 4) for a
+ * sector side of `rows` rows: the largest of 16, 8, 4 whose rows x 2bw
+ * doubles fit in lds_max - static_lds bytes, 0 if none does (the sector
+ * then goes to the augmented eigensolve).  Host arithmetic only, no device
+ * call; the library passes the device's LDS limit and k_bj_step's static LDS. */
+int dmrj_bj_panel_width(int rows, int64_t lds_max, int64_t static_lds);
 /* Same three probes for v_mfma_f32_16x16x32_bf16 (768 floats). */
 int dmrj_selftest_mfma_bf16(dmrj_ctx* ctx, float* out768);
 

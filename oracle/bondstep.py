@@ -29,9 +29,14 @@ import numpy as np
 from oracle import dmrg as od
 from oracle.blocks import sectors, _add, _sub
 
+# algorithmic flops of the H_eff*psi and environment-update GEMMs done so far
+# (2 m n k per block product, the same count the GPU path reports)
+FLOPS = [0.0]
 
-def _key(row):
-    return tuple(int(x) for x in row)
+
+def _mm(a, b):
+    FLOPS[0] += 2.0 * a.shape[0] * a.shape[1] * b.shape[1]
+    return a @ b
 
 
 class Bond:
@@ -163,7 +168,7 @@ def heff2_blocks(Lb, Rb, W1, W2, th, dL, dR):
     for (a, j1, j2, b), blk in th.items():
         for w in range(W12.shape[0]):
             for (x, Lv) in Lby.get((w, a), ()):
-                T1.setdefault((w, j1, j2), []).append((x, b, Lv @ blk))
+                T1.setdefault((w, j1, j2), []).append((x, b, _mm(Lv, blk)))
     T3 = {}
     for (w, j1, k1, j2, k2, u) in nz:
         c = W12[w, j1, k1, j2, k2, u]
@@ -181,9 +186,9 @@ def heff2_blocks(Lb, Rb, W1, W2, th, dL, dR):
             continue
         key = (x, k1, k2, bp)
         if key in out:
-            out[key] += blk @ Rv
+            out[key] += _mm(blk, Rv)
         else:
-            out[key] = blk @ Rv
+            out[key] = _mm(blk, Rv)
     return out
 
 
@@ -193,12 +198,13 @@ def twosite_update_blocks(Lb, Rb, W1, W2, th, dL, dR):
     pair gets its own copy of the two-site data), then Lenv over (w, a), then
     Renv over (b, u)."""
     W12, nz = _w12(W1, W2)
+    byphys = {}
+    for (a, j1, j2, b), blk in th.items():
+        byphys.setdefault((j1, j2), []).append((a, b, blk))
     H = {}
     for (w, j1, k1, j2, k2, u) in nz:
         c = W12[w, j1, k1, j2, k2, u]
-        for (a, jj1, jj2, b), blk in th.items():
-            if jj1 != j1 or jj2 != j2:
-                continue
+        for (a, b, blk) in byphys.get((j1, j2), ()):
             key = (w, k1, k2, u, a, b)
             if key in H:
                 H[key] += c * blk
@@ -239,7 +245,7 @@ def lupdate_blocks(Lb, A, W, dL, dR):
     for (a, j, b), blk in A.items():
         for w in range(W.shape[0]):
             for (x, Lv) in Lby.get((w, a), ()):
-                T1.setdefault((w, j), []).append((x, b, Lv @ blk))
+                T1.setdefault((w, j), []).append((x, b, _mm(Lv, blk)))
     nz = [tuple(int(i) for i in ix) for ix in np.argwhere(W != 0.0)]
     T2 = {}
     for (w, j, k, u) in nz:
@@ -261,9 +267,9 @@ def lupdate_blocks(Lb, A, W, dL, dR):
         B, Ab = hit
         key = (B, u, b)
         if key in out:
-            out[key] += Ab.conj().T @ blk
+            out[key] += _mm(Ab.conj().T, blk)
         else:
-            out[key] = Ab.conj().T @ blk
+            out[key] = _mm(Ab.conj().T, blk)
     return out
 
 
@@ -277,7 +283,7 @@ def rupdate_blocks(Rb, A, W, dL, dR):
     for (a, j, b), blk in A.items():
         for u in range(W.shape[3]):
             for (B, Rv) in Rby.get((b, u), ()):
-                T1.setdefault((j, u), []).append((a, B, blk @ Rv))
+                T1.setdefault((j, u), []).append((a, B, _mm(blk, Rv)))
     nz = [tuple(int(i) for i in ix) for ix in np.argwhere(W != 0.0)]
     T2 = {}
     for (w, j, k, u) in nz:
@@ -299,9 +305,9 @@ def rupdate_blocks(Rb, A, W, dL, dR):
         x, Ab = hit
         key = (a, w, x)
         if key in out:
-            out[key] += blk @ Ab.conj().T
+            out[key] += _mm(blk, Ab.conj().T)
         else:
-            out[key] = blk @ Ab.conj().T
+            out[key] = _mm(blk, Ab.conj().T)
     return out
 
 

@@ -1,7 +1,8 @@
-"""Correlation matrix on a full state vector (oracle).
+"""Correlation matrix and general correlations on full state vectors (oracle).
 
-TEST INFRASTRUCTURE ONLY.  Restates lib/MPO_MPS/correlationmatrix.jl:21-83 by
-brute force on the 2^L (d^L) amplitude vector:
+TEST INFRASTRUCTURE ONLY.  Restates lib/MPO_MPS/correlationmatrix.jl:21-83 and
+lib/MPO_MPS/correlation.jl:74-237 by brute force on the 2^L (d^L) amplitude
+vector.  correlationmatrix:
     rho[i, j] = <psi| (Cc F)_i F_{i+1} ... F_{j-1} (Ca)_j |psi>   (i < j)
     rho[i, i] = <psi| (Cc Ca)_i |psi>,   rho[j, i] = conj(rho[i, j])
 with operators acting on the ket (the reference contracts Cc's second index
@@ -16,6 +17,21 @@ state rho(S+, S-) is diag(1, 0, 1, ...); for free fermions the matrix with
 the Jordan-Wigner trail is the projector on the occupied orbitals (without
 the trail it is not).  Vector index convention: site 1
 is the fastest index (Julia's column-major reshape of the MPS product).
+
+transition_matrix: the same sums between two vectors, <bra| ... |ket>, not
+divided by any norm (correlationmatrix(dualpsi, psi, Cc, Ca; trail),
+correlationmatrix.jl:21-62).
+
+correlation(bra, ket, ops, trail) (correlation.jl:92-237, default sites):
+    omega[p1, ..., pn] = <bra| X |ket>,  X = O1_{p1} T1_{<p1} On_{pn} Tn_{<pn}
+    ... O2_{p2} T2_{<p2} applied to the ket in the reference's order (operator
+    n and its trail first, operator 1 last), T_g the trail of operator g on
+    every site left of p_g (none when not given).  Pinned by: the i < j and
+    i = j entries equal correlationmatrix (one trail shared by both
+    operators); for free fermions with the Jordan-Wigner trail the whole
+    matrix, i > j included, is the occupied-orbital projector (a dropped
+    trail fails); on the Neel state a three-point Sz function is the product
+    of the local values.
 """
 import numpy as np
 
@@ -74,3 +90,54 @@ def correlationmatrix(vec, d, L, Cc, Ca, trail=None):
             rho[i - 1, j - 1] = v
             rho[j - 1, i - 1] = np.conj(v)
     return rho
+
+
+def transition_matrix(bra, ket, d, L, Cc, Ca, trail=None):
+    """<bra| (Cc F)_i F ... F (Ca)_j |ket> for i < j, <bra| (Cc Ca)_i |ket> on
+    the diagonal, and the lower triangle as the reference fills it:
+    rho[j, i] = conj(rho[i, j])."""
+    Cc = np.asarray(Cc)
+    Ca = np.asarray(Ca)
+    rho = np.zeros((L, L), dtype=np.result_type(bra, ket, Cc, Ca))
+    for i in range(1, L + 1):
+        rho[i - 1, i - 1] = np.vdot(bra, _apply(ket, Cc @ Ca, i, d, L))
+    FCc = Cc @ trail if trail is not None else Cc
+    for i in range(1, L):
+        for j in range(i + 1, L + 1):
+            w = _apply(ket, Ca, j, d, L)
+            if trail is not None:
+                for k in range(i + 1, j):
+                    w = _apply(w, trail, k, d, L)
+            w = _apply(w, FCc, i, d, L)
+            v = np.vdot(bra, w)
+            rho[i - 1, j - 1] = v
+            rho[j - 1, i - 1] = np.conj(v)
+    return rho
+
+
+def correlation(bra, ket, d, L, ops, trail=None):
+    """omega[p1-1, ..., pn-1] over all position tuples (see the module
+    docstring); trail: None, one matrix for every operator, or a tuple with
+    one entry per operator (None = no trail for that operator)."""
+    n = len(ops)
+    if trail is None or isinstance(trail, np.ndarray):
+        trails = [trail] * n
+    else:
+        trails = [trail[g % len(trail)] for g in range(n)]
+    omega = np.zeros((L,) * n, dtype=np.result_type(bra, ket, *ops))
+    for pos in np.ndindex(*(L,) * n):
+        w = ket
+        for g in list(range(1, n)) [::-1] + [0]:
+            p = pos[g] + 1
+            if g > 0:
+                w = _apply(w, ops[g], p, d, L)
+                if trails[g] is not None:
+                    for s in range(1, p):
+                        w = _apply(w, trails[g], s, d, L)
+            else:
+                if trails[0] is not None:
+                    for s in range(1, p):
+                        w = _apply(w, trails[0], s, d, L)
+                w = _apply(w, ops[0], p, d, L)
+        omega[pos] = np.vdot(bra, w)
+    return omega

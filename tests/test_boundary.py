@@ -82,3 +82,77 @@ def test_host_mpo_deltas(pkg):
     b = omps.mpo_deltas(om.makeMPO(om.hubbardMPO(), 4, 6), q)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
+
+
+def test_makeq_host_assigns_charges_drops_and_randomises(pkg):
+    """makeqMPS's charge assignment (assignflux!: the charge of the largest
+    entry of every right-link column, first maximum with the physical index
+    innermost), the dropping of entries off the charge blocks, and the
+    refilling of a tensor left empty, on hand-labelled MPS (CPU only)."""
+    from dmrjtensor_jl_amd.network import makeq_host
+    q = [[1], [-1]]
+    # Neel product state: bond charges are the running sums
+    A = [np.zeros((1, 2, 1)) for _ in range(4)]
+    for i in range(4):
+        A[i][0, i % 2, 0] = 1.0
+    t, qb = makeq_host(A, q)
+    assert [int(x[0, 0]) for x in qb] == [0, 1, 0, 1, 0]
+    assert all(np.array_equal(a, b) for a, b in zip(t, A))
+    # two right-link states; column 0 is dominated by (a=0, s=1), column 1 by
+    # (a=0, s=0); a small entry of column 0 at (a=0, s=0) breaks the charge
+    # rule and is dropped
+    B = np.zeros((1, 2, 2))
+    B[0, 1, 0] = 0.9
+    B[0, 0, 0] = 0.1
+    B[0, 0, 1] = 0.5
+    t, qb = makeq_host([B], q)
+    assert qb[1].ravel().tolist() == [-1, 1]
+    assert t[0][0, 0, 0] == 0.0 and t[0][0, 1, 0] == 0.9 and t[0][0, 0, 1] == 0.5
+    # ties go to the first entry with the left index outer and the physical
+    # index inner: (a=0, s=1) before (a=1, s=0), whose charges differ here
+    T = np.zeros((2, 2, 1))
+    T[0, 1, 0] = 1.0
+    T[1, 0, 0] = 1.0
+    t, qb = makeq_host([T], q, qleft=[[0], [0]])
+    assert int(qb[1][0, 0]) == -1
+    # an all-zero tensor is refilled on its allowed blocks only
+    Z = np.zeros((2, 2, 1))
+    t, qb = makeq_host([np.ones((1, 2, 2)) * [[[1.0, 0.0], [0.0, 1.0]]], Z], q, seed=3)
+    ok = (qb[1][:, None, None, 0] + np.array([1, -1])[None, :, None]) == qb[2][None, None, :, 0]
+    assert np.any(t[1][ok] != 0.0) and np.all(t[1][~ok] == 0.0)
+
+
+def test_load_mps_rejects_corrupted_archives(pkg, tmp_path):
+    """A truncated or inconsistent checkpoint raises ValueError before any
+    array reaches the library (CPU only: no device call happens)."""
+    import json as _json
+    FORMAT = pkg.checkpoint.FORMAT if hasattr(pkg, "checkpoint") else "dmrj-mps-1"
+
+    def write(path, qL_rows=2, qP_rows=2, data_n=8, info_n=8, nq_site=1, nq_head=1):
+        head = {"format": FORMAT, "L": 1, "oc": 1, "nq": nq_head, "d": [2], "qphys": [[1], [-1]], "extra": {}}
+        info = np.array([nq_site, 2, 2, 2, 8, 0, 0, 0][:info_n], dtype=np.int64)
+        arrays = {"header": np.frombuffer(_json.dumps(head).encode(), dtype=np.uint8), "A1_info": info,
+                  "A1_qL": np.zeros((qL_rows, 1), np.int32), "A1_qP": np.zeros((qP_rows, 1), np.int32),
+                  "A1_qR": np.zeros((2, 1), np.int32), "A1_data": np.zeros(data_n)}
+        with open(path, "wb") as f:
+            np.savez(f, **arrays)
+        return path
+
+    load = pkg.load_mps
+    for kw in ({"qL_rows": 1}, {"qP_rows": 3}, {"data_n": 7}, {"info_n": 4}, {"nq_site": 2}):
+        p = write(str(tmp_path / "bad.npz"), **kw)
+        with pytest.raises(ValueError):
+            load(p)
+
+
+def test_block_jacobi_panel_width(pkg):
+    """The block-Jacobi panel (rows x 2bw doubles of dynamic LDS) at the edge
+    of the 160 KiB work-group LDS with 256 bytes of static LDS: 639 rows fit
+    width 16, 640 rows need width 8, 1280 rows width 4 (host arithmetic)."""
+    lib = pkg._lib.load()
+    lds, st = 160 * 1024, 256
+    assert lib.dmrj_bj_panel_width(639, lds, st) == 16
+    assert lib.dmrj_bj_panel_width(640, lds, st) == 8
+    assert lib.dmrj_bj_panel_width(1280, lds, st) == 4
+    assert lib.dmrj_bj_panel_width(5200, lds, st) == 0
+    assert lib.dmrj_bj_panel_width(640, lds, 0) == 16

@@ -237,7 +237,10 @@ def test_dmrg3s_hubbard_u1xu1_matches_oracle(pkg, ctx):
     omps.apply_ops(o, list(range(1, 5)), Cup.T, trail=F)
     omps.apply_ops(o, list(range(1, 5)), Cdn.T, trail=F)
     omps.product_charges(o, qh)
-    psi = pkg.MPS([a.copy() for a in o.A], oc=1, qphys=qh, qbond=o.qbond, ctx=ctx, canonical=True)
+    psi = pkg.product_state([0] * L, 4, qphys=qh, ctx=ctx)          # the product's own applyOps!
+    pkg.applyOps(psi, list(range(1, 5)), Cup.T, trail=F)
+    pkg.applyOps(psi, list(range(1, 5)), Cdn.T, trail=F)
+    assert all(np.array_equal(a, b) for a, b in zip(psi.to_dense()[0], o.A))
     oE, ohist = o3.dmrg3S(o, hmpo, m=24, sweeps=4, cutoff=1e-12)
     params = pkg.algvars(psi)
     pkg.dmrg(psi, pkg.MPO(hmpo, qphys=qh), m=24, sweeps=4, cutoff=1e-12, method="3S", params=params)
@@ -305,13 +308,10 @@ def test_dmrg3s_dense_vs_qn_reference_criteria(pkg, ctx):
     hmpo = om.makeMPO(om.hubbardMPO(), 4, L)
     out = []
     for q in (None, qh):
-        o = omps.product_mps([0] * L, 4)
-        omps.apply_ops(o, list(range(1, 6)), Cup.T, trail=F)
-        omps.apply_ops(o, list(range(1, 6)), Cdn.T, trail=F)
-        if q is not None:
-            omps.product_charges(o, q)
-        hpsi = pkg.MPS([a.copy() for a in o.A], oc=1, qphys=q, qbond=o.qbond if q is not None else None, ctx=ctx,
-                       canonical=True)
+        # the Hubbard filling of test/dmrg_test.jl:67-75 by the product's applyOps!
+        hpsi = pkg.product_state([0] * L, 4, qphys=q, ctx=ctx)
+        pkg.applyOps(hpsi, list(range(1, 6)), Cup.T, trail=F)
+        pkg.applyOps(hpsi, list(range(1, 6)), Cdn.T, trail=F)
         out.append(pkg.dmrg(hpsi, pkg.MPO(hmpo, qphys=q) if q is not None else pkg.MPO(hmpo), m=45, sweeps=10,
                             cutoff=1e-12, method="3S"))
     say(" Hubbard L=10: dense", out[0], "U(1)xU(1)", out[1])

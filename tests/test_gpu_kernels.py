@@ -340,3 +340,34 @@ def test_grouped_gemm_bf16x3(pkg, ctx, cfg, tA, tB, M, N, K):
     err = np.max(np.abs(C - want) / scale)
     say(" bf16x3 cfg", cfg, tA, tB, M, N, K, "err/|A||B| %.2e" % err)
     assert err < 6e-5
+
+
+def test_block_jacobi_panel_boundary_640_rows(pkg, ctx):
+    """Backend 4 on a sector of exactly 640 rows: the 16-wide panel would fill
+    the whole 160 KiB of LDS, so the launcher narrows it (width 8) with the
+    kernel's static LDS read from its attributes; the spectrum of a graded
+    640 x 600 matrix matches LAPACK."""
+    say("case block Jacobi 640 rows")
+    R, C = 640, 600
+    rng = np.random.default_rng(640)
+    k = min(R, C)
+    U, _ = np.linalg.qr(rng.standard_normal((R, k)))
+    V, _ = np.linalg.qr(rng.standard_normal((C, k)))
+    s = np.logspace(0, -6, k)
+    M = (U * s) @ V.T
+    M /= np.linalg.norm(M)
+    AA = M.reshape(R // 2, 2, 2, C // 2, order="F")
+    t = pkg.site2_from_dense(ctx, AA)
+    ctx.set_svd_backend(4)
+    try:
+        l, r, te, sv = _split(pkg, ctx, t, 0, 1)
+    finally:
+        ctx.set_svd_backend(0)
+    want = np.linalg.svd(AA.reshape(R, C, order="F"), compute_uv=False)
+    want = want / np.linalg.norm(want)
+    n = min(len(sv), len(want))
+    err = np.max(np.abs(sv[:n] - want[:n]))
+    say(" kept", len(sv), "abs err %.2e" % err, "jacobi stats", ctx.svd_stats())
+    assert err < 1e-13
+    for x in (l, r, t):
+        x.free()

@@ -103,3 +103,78 @@ def test_correlationmatrix_complex_tdvp_state(pkg, ctx, qn):
         assert err < 1e-12 * max(1.0, np.max(np.abs(want)))
         assert np.allclose(M, M.conj().T, atol=1e-13)
     assert np.max(np.abs(pkg.correlationmatrix(psi, Sp, Sm).imag)) > 1e-6
+
+
+def test_correlationmatrix_two_state_golden(pkg, ctx):
+    """correlationmatrix(psi, psi, Sp, Sm) as the reference's test calls it
+    (test/MPmeas_test.jl:316-318): the golden checkM and the single-state
+    form."""
+    say("case correlationmatrix(psi, psi) L=10 golden")
+    with open(os.path.join(GOLD, "correlationmatrix_L10.json")) as f:
+        chk = np.array(json.load(f)["checkM"])
+    E, V = np.linalg.eigh(om.fullH(om.makeMPO(om.heisenbergMPO(), 2, 10)))
+    A = ome.vector_to_mps(V[:, 0], 2, 10)
+    psi = pkg.MPS(A, oc=1, ctx=ctx)
+    psi2 = psi.copy()
+    Sp, Sm, Sz, O, Id = pkg.spinOps()
+    M2 = pkg.correlationmatrix(psi2, psi, Sp, Sm)       # two handles, same state
+    M1 = pkg.correlationmatrix(psi, Sp, Sm)
+    want = ome.transition_matrix(V[:, 0], V[:, 0], 2, 10, Sp, Sm)
+    say(" two-state vs oracle %.2e, vs single-state %.2e" % (np.max(np.abs(M2 - want)), np.max(np.abs(M2 - M1))))
+    assert np.linalg.norm(chk - M2) / chk.size < 1e-5
+    assert np.max(np.abs(M2 - want)) < 1e-12
+    assert np.max(np.abs(M2 - M1)) < 1e-12
+
+
+def test_two_state_measurements_u1(pkg, ctx):
+    """<dualpsi| ... |psi> between two different U(1) states (GPU DMRG at two
+    bond dimensions) against the oracle's brute force on their vectors:
+    correlationmatrix(dualpsi, psi, S+, S-) and correlation(dualpsi, psi,
+    S+, S-), and correlation(psi, Sz, Sz, Sz) (three operators)."""
+    say("case two-state measurements U(1) L=8")
+    L = 8
+    qphys = np.array([[1], [-1]])
+    mpo = pkg.MPO(pkg.makeMPO(pkg.heisenbergMPO(), 2, L).H, qphys=qphys)
+    psi = pkg.product_state([i % 2 for i in range(L)], 2, qphys=qphys, ctx=ctx)
+    pkg.dmrg(psi, mpo, m=16, sweeps=3, cutoff=1e-12)
+    phi = pkg.product_state([(i // 2) % 2 for i in range(L)], 2, qphys=qphys, ctx=ctx)
+    pkg.dmrg(phi, mpo, m=4, sweeps=1, cutoff=1e-12)
+    vpsi = ome.mps_to_vector(psi.to_dense()[0])
+    vphi = ome.mps_to_vector(phi.to_dense()[0])
+    Sp, Sm, Sz, O, Id = pkg.spinOps()
+    M = pkg.correlationmatrix(phi, psi, Sp, Sm)
+    want = ome.transition_matrix(vphi, vpsi, 2, L, Sp, Sm)
+    err = np.max(np.abs(M - want))
+    say(" correlationmatrix(phi, psi) max err %.2e (scale %.2e)" % (err, np.max(np.abs(want))))
+    assert err < 1e-12
+    W = pkg.correlation(phi, psi, Sp, Sm)
+    want = ome.correlation(vphi, vpsi, 2, L, [Sp, Sm])
+    err = np.max(np.abs(W - want))
+    say(" correlation(phi, psi, S+, S-) max err %.2e" % err)
+    assert err < 1e-12
+    W3 = pkg.correlation(psi, Sz, Sz, Sz)
+    want = ome.correlation(vpsi, vpsi, 2, L, [Sz, Sz, Sz]) / np.vdot(vpsi, vpsi)
+    err = np.max(np.abs(W3 - want))
+    say(" correlation(psi, Sz, Sz, Sz) max err %.2e" % err)
+    assert err < 1e-12
+
+
+def test_correlation_fermions_with_trail(pkg, ctx):
+    """correlation(psi, c+, c; trail=F) on a U(1)xU(1) Hubbard DMRG state
+    against the oracle (both orderings of the two sites, the Jordan-Wigner
+    trail on the sites left of each operator)."""
+    say("case correlation Hubbard L=6 with trail")
+    L = 6
+    qphys = np.array([[0, 0], [1, 1], [1, -1], [2, 0]])
+    Cup, Cdn, F, Nup, Ndn, Ndens, O, Id = om.fermionOps()
+    psi = pkg.product_state([0] * L, 4, qphys=qphys, ctx=ctx)
+    pkg.applyOps(psi, list(range(1, 4)), Cup.T, trail=F)
+    pkg.applyOps(psi, list(range(1, 4)), Cdn.T, trail=F)
+    mpo = pkg.MPO(pkg.makeMPO(pkg.hubbardMPO(), 4, L).H, qphys=qphys)
+    pkg.dmrg(psi, mpo, m=32, sweeps=3, cutoff=1e-12)
+    v = ome.mps_to_vector(psi.to_dense()[0])
+    W = pkg.correlation(psi, Cup.T, Cup, trail=F)
+    want = ome.correlation(v, v, 4, L, [Cup.T, Cup], trail=F)
+    err = np.max(np.abs(W - want))
+    say(" max err %.2e" % err)
+    assert err < 1e-12

@@ -116,7 +116,13 @@ def test_hubbard_L10_u1xu1_reference_constant(pkg, ctx):
     omps.apply_ops(opsi, list(range(1, 6)), Cdn.T, trail=F)
     qphys = np.array([[0, 0], [1, 1], [1, -1], [2, 0]])
     omps.product_charges(opsi, qphys)
-    psi = pkg.MPS([a.copy() for a in opsi.A], oc=1, qphys=qphys, qbond=opsi.qbond, ctx=ctx, canonical=True)
+    # the GPU start state is built by the product's own applyOps! on the device
+    psi = pkg.product_state([0] * L, 4, qphys=qphys, ctx=ctx)
+    pkg.applyOps(psi, list(range(1, 6)), Cup.T, trail=F)
+    pkg.applyOps(psi, list(range(1, 6)), Cdn.T, trail=F)
+    tensors, qb = psi.to_dense()
+    assert all(np.array_equal(a, b) for a, b in zip(tensors, opsi.A))
+    assert all(np.array_equal(a, b) for a, b in zip(qb, opsi.qbond))
     mpo = pkg.MPO(pkg.makeMPO(pkg.hubbardMPO(), 4, L).H, qphys=qphys)
     params = pkg.algvars(psi)
     E = pkg.dmrg(psi, mpo, m=45, sweeps=10, cutoff=1e-12, params=params)
@@ -166,8 +172,10 @@ def test_autotune_is_result_neutral(pkg, ctx):
 @pytest.mark.parametrize("backend", [0, 5])
 def test_eig_batching_modes_agree(pkg, ctx, backend):
     """The SVD's batched eigensolves give the same sweeps whether the sectors
-    form one batch padded to the largest, one batch each, or the cost-model
-    size classes (which keep one batch at these sizes).  L=16 keeps every
+    form one batch padded to the largest, one batch each, batches of two
+    sectors (mode 3: with several classes of several sectors every per-class
+    and per-member buffer offset is used), or the cost-model size classes
+    (which keep one batch at these sizes).  L=16 keeps every
     bond below m, so the cutoff alone truncates and no rounding-level change
     can move a degenerate multiplet across the bond-dimension cut.  Per-sweep
     energies within 1e-12 relative for the augmented form (backend 0), 1e-9
@@ -179,7 +187,7 @@ def test_eig_batching_modes_agree(pkg, ctx, backend):
     states = [i % 2 for i in range(L)]
     qmpo = pkg.MPO(pkg.makeMPO(pkg.heisenbergMPO(), 2, L).H, qphys=qphys)
     runs = []
-    for mode in (0, 2, 1):
+    for mode in (0, 2, 3, 1):
         ctx.set_eig_batching(mode)
         ctx.set_svd_backend(backend)
         try:
@@ -277,3 +285,57 @@ def test_mps_copy_is_independent(pkg, ctx):
     assert pkg.expect(psi, mpo) == e0
     assert e1 < e0 + 1e-12
     cp.free()
+
+
+def test_eig_batching_pairs_against_oracle_3s_and_complex(pkg, ctx):
+    """Batches of two sectors (eigensolve batching mode 3) on the paths whose
+    batches the cost model rarely splits at test sizes: the two-site sweep
+    (augmented form) against the oracle per sweep, 3S (its enlarged sectors go
+    through the LQ/QR-reduced eigensolve) against mode 0, and real-time TDVP
+    (complex Hermitian eigensolves) against the oracle."""
+    from oracle import tdvp as ot
+    say("case eigensolve batches of two sectors")
+    L = 16
+    qphys = np.array([[1], [-1]])
+    states = [i % 2 for i in range(L)]
+    qmpo = pkg.MPO(pkg.makeMPO(pkg.heisenbergMPO(), 2, L).H, qphys=qphys)
+    ompo = om.makeMPO(om.heisenbergMPO(), 2, L)
+    ctx.set_eig_batching(3)
+    try:
+        psi = pkg.product_state(states, 2, qphys=qphys, ctx=ctx)
+        params = pkg.algvars(psi)
+        pkg.dmrg(psi, qmpo, m=512, sweeps=3, cutoff=1e-10, params=params)
+        opsi = omps.product_mps(states, 2, qphys=qphys)
+        oE, ohist = od.dmrg(opsi, ompo, m=512, sweeps=3, cutoff=1e-10)
+        for h, oh in zip(params.history, ohist):
+            rel = abs(h["energy"] - oh["energy"]) / abs(oh["energy"])
+            say(" two-site sweep %d rel %.2e" % (h["sweep"], rel))
+            assert rel <= 1e-12
+            assert np.max(np.abs(np.asarray(h["storeD"]) - np.asarray(oh["storeD"]))) <= 1e-10
+        runs = []
+        for mode in (3, 0):
+            ctx.set_eig_batching(mode)
+            psi = pkg.product_state(states, 2, qphys=qphys, ctx=ctx)
+            p3 = pkg.algvars(psi)
+            pkg.dmrg(psi, qmpo, m=160, sweeps=3, cutoff=1e-12, params=p3, method="3S")
+            runs.append(p3.history)
+            say(" 3S mode %d E %.15f" % (mode, p3.energy))
+        for a, b in zip(*runs):
+            assert abs(a["energy"] - b["energy"]) <= 1e-12 * abs(a["energy"])
+        ctx.set_eig_batching(3)
+        Lt = 10
+        tmpo = pkg.MPO(pkg.makeMPO(pkg.heisenbergMPO(), 2, Lt).H, qphys=qphys)
+        tpsi = pkg.product_state([i % 2 for i in range(Lt)], 2, qphys=qphys, ctx=ctx)
+        opsi = omps.product_mps([i % 2 for i in range(Lt)], 2, qphys=qphys)
+        otm = om.makeMPO(om.heisenbergMPO(), 2, Lt)
+        Sz = np.diag([0.5, -0.5])
+        gLe = gRe = oLe = oRe = None
+        for hs in range(4):
+            _, gLe, gRe = pkg.tdvp_twosite(tpsi, tmpo, m=16, cutoff=1e-12, r=12, prefactor=-0.05j, Lenv=gLe, Renv=gRe)
+            _, oLe, oRe = ot.tdvp_twosite(opsi, otm, m=16, cutoff=1e-12, r=12, prefactor=-0.05j, Lenv=oLe, Renv=oRe)
+        sz = np.array([pkg.local_expect(tpsi, Sz, i) for i in range(1, Lt + 1)])
+        osz = np.array([ot.local_expect(opsi, Sz, i) for i in range(1, Lt + 1)])
+        say(" TDVP max dSz %.2e" % np.max(np.abs(sz - osz)))
+        assert np.max(np.abs(sz - osz)) <= 1e-10
+    finally:
+        ctx.set_eig_batching(1)

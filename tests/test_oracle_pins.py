@@ -509,3 +509,40 @@ def test_block_bond_step_matches_dense_oracle(model):
     # spectrum per (w, u) channel is not needed: compare energies through it
     Lchk = od.Lupdate(Lenv, bs.mps_to_dense(nL, bonds[1], d, bnew), mpo[1])
     assert np.max(np.abs(got - Lchk)) < tol(Lchk)
+
+
+def test_correlation_oracle_pins():
+    """oracle/measure.py correlation (lib/MPO_MPS/correlation.jl:92-237) and
+    transition_matrix against correlationmatrix, the free-fermion projector
+    with the Jordan-Wigner trail (all i, j), and a Neel closed form."""
+    from oracle import measure as ome
+    E, V = ed_ground(om.makeMPO(om.heisenbergMPO(), 2, 8))
+    v = V[:, 0]
+    Sp, Sm, Sz, O, Id = om.spinOps()
+    M = ome.correlationmatrix(v, 2, 8, Sp, Sm)
+    W = ome.correlation(v, v, 2, 8, [Sp, Sm])
+    iu = np.triu_indices(8)
+    assert np.allclose(W[iu], M[iu], atol=1e-13)
+    assert np.allclose(ome.transition_matrix(v, v, 2, 8, Sp, Sm), M, atol=1e-13)
+    # transition element between two different states: linear in the ket
+    v2 = V[:, 1]
+    T1 = ome.transition_matrix(v, v + 2 * v2, 2, 8, Sp, Sm)
+    T2 = ome.transition_matrix(v, v, 2, 8, Sp, Sm) + 2 * ome.transition_matrix(v, v2, 2, 8, Sp, Sm)
+    assert np.allclose(np.triu(T1), np.triu(T2), atol=1e-12)
+    # free fermions: <c+_i c_j> with the trail on both operators is the occupied projector everywhere
+    Cup, Cdn, F, Nup, Ndn, Ndens, _, _ = om.fermionOps()
+    L = 4
+    E, V = ed_ground(om.makeMPO(om.hubbardMPO(t=1.0, mu=0.0, HubU=0.0), 4, L))
+    rho = ome.correlation(V[:, 0], V[:, 0], 4, L, [Cup.T, Cup], trail=F)
+    h = np.diag(np.ones(L - 1), 1) + np.diag(np.ones(L - 1), -1)
+    e, phi = np.linalg.eigh(h)
+    occ = phi[:, e < 0]
+    assert np.allclose(rho, occ @ occ.T, atol=1e-12)
+    assert not np.allclose(ome.correlation(V[:, 0], V[:, 0], 4, L, [Cup.T, Cup]), occ @ occ.T, atol=1e-3)
+    # Neel |up dn up dn ...>: <Sz_i Sz_j Sz_k> = product of +-1/2
+    L = 5
+    neel = np.zeros(2 ** L)
+    neel[sum(1 << i for i in range(1, L, 2))] = 1.0
+    sz = np.array([0.5 if i % 2 == 0 else -0.5 for i in range(L)])
+    W3 = ome.correlation(neel, neel, 2, L, [Sz, Sz, Sz])
+    assert np.allclose(W3, np.einsum("i,j,k->ijk", sz, sz, sz), atol=1e-14)
