@@ -1053,6 +1053,7 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
     double* X = nullptr;
     int bj = 0, rowsX = 0, ncolsX = 0;
     int qr = 0;  // one-factor sector solved through its LQ / QR triangle (X: K x K factor)
+    int gram = 0;  // orthonormal factor = eigenvectors of the Gram matrix (orthonormal to working precision)
     std::vector<int> perm;
     std::vector<double> hn;
     std::vector<const double*> colsrc;
@@ -1333,6 +1334,7 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
         else
           segs.push_back({s.M, s.M, s.R, s.R, (int)s.R, 1, 1.0});  // M^T M
         s.bj = 1;
+        s.gram = 1;
         s.rowsX = n;
         is_small[gramsec[b]] = 1;  // handled
         const double nn = (double)n, kk = (double)(dir > 0 ? s.C : s.R);
@@ -1776,15 +1778,17 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
       hoff += s.kmin;
     }
     launch_bj_gather(c, gat);
-    // canonical factor: kept vectors of small singular values (sigma <
-    // 1e-4 sigma_max, whose extracted vectors are orthogonal to the rest only
-    // to about eps / sigma) re-orthonormalised in place, then an orthonormal
+    // canonical factor: kept vectors of singular values below 1e-2 sigma_max
+    // (extracted vectors are orthogonal to the rest only to about
+    // eps sigma_max / sigma, so those above stay within ~1e-14; not in Gram
+    // sectors, whose factor is a set of eigenvectors) re-orthonormalised in
+    // place, then an orthonormal
     // completion for numerically zero singular values (sigma <= 1e-10
     // sigma_max); the latter carry no weight
     double smax = 0.0;
     for (int64_t i = 0; i < n; ++i) smax = std::max(smax, ents[i].v);
     const double szero = 1e-10 * smax;
-    const double ssmall = 1e-4 * smax;
+    const double ssmall = 1e-2 * smax;
     std::vector<CompleteJob> cj, rj;
     hoff = 0;
     for (int si : order) {
@@ -1793,7 +1797,8 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
         int c0 = s.keep;
         while (c0 > 0 && hs[hoff + c0 - 1] <= szero) --c0;
         int c1 = c0;
-        while (c1 > 1 && hs[hoff + c1 - 1] < ssmall) --c1;
+        if (!s.gram)  // Gram eigenvectors need no re-orthonormalisation
+          while (c1 > 1 && hs[hoff + c1 - 1] < ssmall) --c1;
         const int nv = (int)(dir > 0 ? s.R : s.C);
         if (c1 < c0 && c1 >= 1) {
           if (dir > 0)
@@ -1974,54 +1979,13 @@ dmrj_tensor* lanczos2(Context& c, const Tensor& L, const Tensor& R, const Tensor
 }
 
 // ============================================================ Krylov exponential
-// Symmetric eigendecomposition of a small tridiagonal matrix (cyclic Jacobi,
-// FP64, host): T = Q diag(lam) Q^T, Q column-major k x k.
-void tridiag_eig(int k, const double* a, const double* b, std::vector<double>& lam, std::vector<double>& Q) {
-  std::vector<double> A((size_t)k * k, 0.0);
-  for (int i = 0; i < k; ++i) {
-    A[i + (size_t)i * k] = a[i];
-    if (i + 1 < k) A[i + (size_t)(i + 1) * k] = A[(i + 1) + (size_t)i * k] = b[i];
-  }
-  Q.assign((size_t)k * k, 0.0);
-  for (int i = 0; i < k; ++i) Q[i + (size_t)i * k] = 1.0;
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0.0;
-    for (int p = 0; p < k; ++p)
-      for (int q = p + 1; q < k; ++q) off += A[p + (size_t)q * k] * A[p + (size_t)q * k];
-    if (off < 1e-300) break;
-    for (int p = 0; p < k; ++p)
-      for (int q = p + 1; q < k; ++q) {
-        const double apq = A[p + (size_t)q * k];
-        if (apq == 0.0) continue;
-        const double app = A[p + (size_t)p * k], aqq = A[q + (size_t)q * k];
-        const double theta = (aqq - app) / (2.0 * apq);
-        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
-        const double cs = 1.0 / std::sqrt(t * t + 1.0), sn = t * cs;
-        for (int r = 0; r < k; ++r) {  // columns p, q
-          const double x = A[r + (size_t)p * k], y = A[r + (size_t)q * k];
-          A[r + (size_t)p * k] = cs * x - sn * y;
-          A[r + (size_t)q * k] = sn * x + cs * y;
-        }
-        for (int r = 0; r < k; ++r) {  // rows p, q
-          const double x = A[p + (size_t)r * k], y = A[q + (size_t)r * k];
-          A[p + (size_t)r * k] = cs * x - sn * y;
-          A[q + (size_t)r * k] = sn * x + cs * y;
-        }
-        for (int r = 0; r < k; ++r) {
-          const double x = Q[r + (size_t)p * k], y = Q[r + (size_t)q * k];
-          Q[r + (size_t)p * k] = cs * x - sn * y;
-          Q[r + (size_t)q * k] = sn * x + cs * y;
-        }
-      }
-  }
-  lam.resize(k);
-  for (int i = 0; i < k; ++i) lam[i] = A[i + (size_t)i * k];
-}
-
 // Lanczos with r vectors and full reorthogonalisation (FP64 dots) on x,
 // then out = sum_j c_j v_j with c = exp(pf T) e1 (TensorPACK lanczos +
-// exp(M, prefactor) as used by krylovTimeEvol, algs/time.jl:182-202).
-// Returns the lowest Ritz value in *energy and the Krylov dimension used.
+// exp(M, prefactor) as used by krylovTimeEvol, algs/time.jl:182-202); the
+// tridiagonal eigensolve and the coefficients run on the device
+// (k_krylov_coef).  Returns the lowest Ritz value in *energy and the Krylov
+// dimension used when those pointers are given (one stream synchronisation);
+// with both NULL the call enqueues work only.
 // slot: graph slot of the iteration's kernel sequence (1: two-site, 2: one-site)
 // ritz: out is the lowest Ritz vector instead (c = eigenvector of T of the
 // lowest eigenvalue, signed so that c_1 = <v_1, out> >= 0; the prefactor is
@@ -2103,43 +2067,12 @@ dmrj_tensor* krylov_expm(Context& c, const Tensor& L, const Tensor& R, const Ten
     run_captured(c, iterate, r * hl.flops_alg, slot);
   else
     iterate();
-  std::vector<double> hs(128);
-  HIP_OK(hipMemcpyAsync(hs.data(), sc, 128 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-  HIP_OK(hipStreamSynchronize(c.stream));
-  c.release_transients();
-  c.stage_reset();
-  // effective dimension: stop at the first negligible beta
-  int k = r;
-  for (int j = 0; j + 1 < r; ++j) {
-    if (!std::isfinite(hs[64 + j]) || !std::isfinite(hs[j])) throw Error(DMRJ_ERR_NUMERIC, "Krylov: non-finite value");
-    if (hs[64 + j] <= 1e-10) {  // same threshold as the device-side breakdown flag
-      k = j + 1;
-      break;
-    }
-  }
-  std::vector<double> lam, Q;
-  tridiag_eig(k, hs.data(), hs.data() + 64, lam, Q);
-  // c = Q exp(pf Lambda) Q^T e1, pf complex
-  std::vector<double> cre(k, 0.0), cim(k, 0.0);
-  double emin = lam[0];
-  int imin = 0;
-  for (int i = 0; i < k; ++i)
-    if (lam[i] < emin) {
-      emin = lam[i];
-      imin = i;
-    }
-  for (int i = 0; i < k && !ritz; ++i) {
-    const double mag = std::exp(pf_re * lam[i]) * Q[0 + (size_t)i * k];
-    const double gr = mag * std::cos(pf_im * lam[i]), gi = mag * std::sin(pf_im * lam[i]);
-    for (int jj = 0; jj < k; ++jj) {
-      cre[jj] += Q[jj + (size_t)i * k] * gr;
-      cim[jj] += Q[jj + (size_t)i * k] * gi;
-    }
-  }
-  if (ritz) {
-    const double sg = Q[0 + (size_t)imin * k] >= 0.0 ? 1.0 : -1.0;
-    for (int jj = 0; jj < k; ++jj) cre[jj] = sg * Q[jj + (size_t)imin * k];
-  }
+  // the tridiagonal eigensolve and the exponential (or Ritz) coefficients on
+  // the device: no host round trip unless the caller asks for the energy or
+  // the Krylov dimension (sc[128..]: coefficients, sc[200..202]: outputs)
+  double* dcoef = (double*)c.alloc.alloc((size_t)4 * r * sizeof(double));
+  double* douts = sc + 200;
+  launch_krylov_coef(c, sc, r, pf_re, pf_im, ritz ? 1 : 0, zv ? 1 : 0, dcoef, douts);
   auto* out = new dmrj_tensor();
   out->kind = 0;
   out->nq = x.nq;
@@ -2147,25 +2080,20 @@ dmrj_tensor* krylov_expm(Context& c, const Tensor& L, const Tensor& R, const Ten
   out->cplx = zout ? 1 : 0;
   alloc_data(c, *out);
   if (!zv) {
-    double* dcr = (double*)c.upload(cre.data(), k * sizeof(double));
-    launch_combine(c, out->data, V, n, m.size, dcr, k);
-    if (zout) {
-      double* dci = (double*)c.upload(cim.data(), k * sizeof(double));
-      launch_combine(c, out->data + m.size, V, n, m.size, dci, k);
-    }
+    launch_combine(c, out->data, V, n, m.size, dcoef, r);
+    if (zout) launch_combine(c, out->data + m.size, V, n, m.size, dcoef + r, r);
   } else {
     // planes of the Krylov basis: plane 2j = Re v_j, plane 2j+1 = Im v_j, stride n
-    std::vector<double> a(2 * k), b(2 * k);
-    for (int jj = 0; jj < k; ++jj) {
-      a[2 * jj] = cre[jj];
-      a[2 * jj + 1] = -cim[jj];
-      b[2 * jj] = cim[jj];
-      b[2 * jj + 1] = cre[jj];
-    }
-    double* da = (double*)c.upload(a.data(), a.size() * sizeof(double));
-    double* db = (double*)c.upload(b.data(), b.size() * sizeof(double));
-    launch_combine(c, out->data, V, n, m.size, da, 2 * k);
-    launch_combine(c, out->data + m.size, V, n, m.size, db, 2 * k);
+    launch_combine(c, out->data, V, n, m.size, dcoef, 2 * r);
+    launch_combine(c, out->data + m.size, V, n, m.size, dcoef + 2 * r, 2 * r);
+  }
+  double hout[3] = {0.0, (double)r, 0.0};
+  const bool need = energy != nullptr || kdim != nullptr;
+  if (need) {
+    HIP_OK(hipMemcpyAsync(hout, douts, 3 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+    c.release_transients();
+    c.stage_reset();
   }
   c.alloc.release(T1);
   c.alloc.release(T3);
@@ -2174,8 +2102,15 @@ dmrj_tensor* krylov_expm(Context& c, const Tensor& L, const Tensor& R, const Ten
   c.alloc.release(xin);
   c.alloc.release(gs);
   c.alloc.release(sc);
-  if (energy) *energy = emin;
-  if (kdim) *kdim = k;
+  c.alloc.release(dcoef);
+  if (need && hout[2] != 0.0) {
+    dmrj_tensor* o = out;
+    c.alloc.release(o->data);
+    delete o;
+    throw Error(DMRJ_ERR_NUMERIC, "Krylov: non-finite value");
+  }
+  if (energy) *energy = hout[0];
+  if (kdim) *kdim = (int)hout[1];
   return out;
 }
 

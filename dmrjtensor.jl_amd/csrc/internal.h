@@ -389,6 +389,11 @@ void launch_comb_dev(Context& c, double* out, const double* a, const double* b, 
                      const double* ca, const double* cb);
 void launch_lz_scalar(Context& c, int stage);
 void launch_combine(Context& c, double* out, const double* V, int64_t ld, int64_t n, const double* coef, int k);
+// exp(pf T) e1 (or the lowest Ritz vector) coefficients of a Krylov space on
+// the device: sc = {alpha[0..64), beta[64..128)}; coef receives 2r (real
+// Krylov vectors) or 4r (complex) weights, outs {lowest Ritz value, k, flag}
+void launch_krylov_coef(Context& c, const double* sc, int r, double pf_re, double pf_im, int ritz, int zv,
+                        double* coef, double* outs);
 void launch_mfma_probe(Context& c, double* out);
 void launch_mfma_bf16_probe(Context& c, float* out);
 void launch_gemm_lp(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs, int cfg);

@@ -542,6 +542,109 @@ __global__ void k_combine(double* out, const double* V, int64_t ld, int64_t n, c
   }
 }
 
+// Coefficients of the Krylov exponential / Ritz vector on the device (one
+// thread; the Krylov dimension is at most 64, typically 2..12):
+//   k = the Krylov dimension used (first beta_j <= 1e-10 ends the space),
+//   T = Q diag(lam) Q^T by cyclic Jacobi (FP64, the tridiagonal of alpha in
+//   sc[0..], beta in sc[64..]),
+//   c = Q exp(pf lam) Q^T e1 with the complex prefactor pf = (pf_re, pf_im),
+//   or (ritz) the eigenvector of the lowest lam signed to c_1 >= 0.
+// coef: real Krylov vectors: c_re[0..r), c_im[r..2r); complex Krylov vectors
+// (split planes, basis planes 2j = Re v_j, 2j+1 = Im v_j): the plane weights
+// of Re out in [0..2r) and of Im out in [2r..4r).  Entries j >= k are zero.
+// outs: {lowest Ritz value, k, 1 if an alpha/beta was non-finite}.
+// Dynamic LDS: 2 r^2 doubles (the matrix and its eigenvectors).
+__global__ void k_krylov_coef(const double* __restrict__ sc, int r, double pf_re, double pf_im, int ritz, int zv,
+                              double* __restrict__ coef, double* __restrict__ outs) {
+  extern __shared__ double kc_lds[];
+  if (threadIdx.x != 0) return;
+  int k = r;
+  int bad = 0;
+  for (int j = 0; j + 1 < r; ++j) {
+    if (!isfinite(sc[64 + j]) || !isfinite(sc[j])) bad = 1;
+    if (sc[64 + j] <= 1e-10) {
+      k = j + 1;
+      break;
+    }
+  }
+  double* A = kc_lds;
+  double* Q = kc_lds + (int64_t)k * k;
+  for (int e = 0; e < k * k; ++e) {
+    A[e] = 0.0;
+    Q[e] = 0.0;
+  }
+  for (int i = 0; i < k; ++i) {
+    A[i + i * k] = sc[i];
+    if (i + 1 < k) A[i + (i + 1) * k] = A[(i + 1) + i * k] = sc[64 + i];
+    Q[i + i * k] = 1.0;
+  }
+  for (int sweep = 0; sweep < 100 && !bad; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < k; ++p)
+      for (int q = p + 1; q < k; ++q) off += A[p + q * k] * A[p + q * k];
+    if (off < 1e-300) break;
+    for (int p = 0; p < k; ++p)
+      for (int q = p + 1; q < k; ++q) {
+        const double apq = A[p + q * k];
+        if (apq == 0.0) continue;
+        const double app = A[p + p * k], aqq = A[q + q * k];
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        const double cs = 1.0 / sqrt(t * t + 1.0), sn = t * cs;
+        for (int rr = 0; rr < k; ++rr) {  // columns p, q
+          const double x = A[rr + p * k], y = A[rr + q * k];
+          A[rr + p * k] = cs * x - sn * y;
+          A[rr + q * k] = sn * x + cs * y;
+        }
+        for (int rr = 0; rr < k; ++rr) {  // rows p, q
+          const double x = A[p + rr * k], y = A[q + rr * k];
+          A[p + rr * k] = cs * x - sn * y;
+          A[q + rr * k] = sn * x + cs * y;
+        }
+        for (int rr = 0; rr < k; ++rr) {
+          const double x = Q[rr + p * k], y = Q[rr + q * k];
+          Q[rr + p * k] = cs * x - sn * y;
+          Q[rr + q * k] = sn * x + cs * y;
+        }
+      }
+  }
+  double emin = A[0];
+  int imin = 0;
+  for (int i = 1; i < k; ++i)
+    if (A[i + i * k] < emin) {
+      emin = A[i + i * k];
+      imin = i;
+    }
+  const int nc = zv ? 4 * r : 2 * r;
+  for (int e = 0; e < nc; ++e) coef[e] = 0.0;
+  for (int jj = 0; jj < k; ++jj) {
+    double cre = 0.0, cim = 0.0;
+    if (ritz) {
+      const double sg = Q[0 + imin * k] >= 0.0 ? 1.0 : -1.0;
+      cre = sg * Q[jj + imin * k];
+    } else {
+      for (int i = 0; i < k; ++i) {
+        const double lam = A[i + i * k];
+        const double mag = exp(pf_re * lam) * Q[0 + i * k];
+        cre += Q[jj + i * k] * (mag * cos(pf_im * lam));
+        cim += Q[jj + i * k] * (mag * sin(pf_im * lam));
+      }
+    }
+    if (!zv) {
+      coef[jj] = cre;
+      coef[r + jj] = cim;
+    } else {
+      coef[2 * jj] = cre;           // Re out += Re c Re v - Im c Im v
+      coef[2 * jj + 1] = -cim;
+      coef[2 * r + 2 * jj] = cim;   // Im out += Im c Re v + Re c Im v
+      coef[2 * r + 2 * jj + 1] = cre;
+    }
+  }
+  outs[0] = emin;
+  outs[1] = (double)k;
+  outs[2] = bad ? 1.0 : 0.0;
+}
+
 // ---------------------------------------------------------------------------
 static int grid_for(int64_t n, int per = kThreads * 4) {
   int64_t g = (n + per - 1) / per;
@@ -750,6 +853,14 @@ void launch_cgs_pass(Context& c, double* w, const double* V, int64_t nv, int64_t
   hipLaunchKernelGGL(k_vdots_final, dim3(k), dim3(kThreads), 0, c.stream, partial, nblk, coef);
   HIP_OK(hipGetLastError());
   hipLaunchKernelGGL(k_cgs_sub, dim3(grid_for(n)), dim3(kThreads), 0, c.stream, w, V, nv, n, k, cplx, coef);
+  HIP_OK(hipGetLastError());
+}
+
+void launch_krylov_coef(Context& c, const double* sc, int r, double pf_re, double pf_im, int ritz, int zv,
+                        double* coef, double* outs) {
+  if (r < 1 || r > 64) throw Error(-1, "launch_krylov_coef: Krylov dimension must be 1..64");
+  const size_t lds = (size_t)2 * r * r * sizeof(double);
+  hipLaunchKernelGGL(k_krylov_coef, dim3(1), dim3(64), lds, c.stream, sc, r, pf_re, pf_im, ritz, zv, coef, outs);
   HIP_OK(hipGetLastError());
 }
 

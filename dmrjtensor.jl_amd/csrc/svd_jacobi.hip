@@ -547,145 +547,123 @@ void complete_basis(Context& c, std::vector<CompleteJob>& jobs) {
 // Re-orthonormalisation of kept vectors c0 .. keep-1 of a canonical factor
 // whose singular values are small (their vectors, taken from an augmented
 // eigenvector or a Jacobi column, are orthogonal to the rest only to about
-// eps / sigma): blocks of kCB vectors, two classical Gram-Schmidt passes
-// against all vectors before the block on the grouped GEMM, then inside the
-// block two Cholesky-QR passes (G = W^T W = R^T R, W <- W R^-1; G is within
-// ~1e-6 of the identity here, so the factorisation is benign).  Each vector
-// moves by its own non-orthogonal part only.  b < 0: load block 0 into W;
-// b >= 0: finish block b (projected against vectors 0 .. c0 + kCB b - 1),
-// store it, load block b + 1.
-__device__ void rb_load(const CompleteJob& jb, int c, int s) {
-  const int n = jb.n;
-  if (jb.es == 1) {
-    for (int u = 0; u < s; ++u)
-      for (int r = threadIdx.x; r < n; r += kBjThreads)
-        jb.W[(int64_t)u * n + r] = jb.Q[(int64_t)(c + u) * jb.vs + r];
-  } else {
-    for (int64_t e = threadIdx.x; e < (int64_t)s * n; e += kBjThreads) {
-      const int u = (int)(e % s), r = (int)(e / s);
-      jb.W[(int64_t)u * n + r] = jb.Q[(int64_t)(c + u) * jb.vs + (int64_t)r * jb.es];
+// eps / sigma, at most ~1e-6 above the completion threshold).  All k of them
+// at once, per sector:
+//   W = Q[:, c0:keep];  twice  W -= Q0 (Q0^T W)   (Q0 = the first c0 vectors)
+//   three times  G = W^T W,  W <- W (I - F),  F = strict_upper(G) + diag(G - I) / 2
+// The second step is the first-order form of the Cholesky-QR update
+// W <- W R^-1 (G = R^T R, R = I + F + O((G-I)^2)): F + F^T = G - I, so
+// (I - F)^T G (I - F) = I + O((G-I)^2) and each pass squares the deviation
+// from orthonormality.  I - F is upper triangular, so vector j moves only
+// along vectors before it and by its own norm, as in Gram-Schmidt.  Every
+// product is one grouped-GEMM launch over all sectors.
+struct ReorthJob {
+  double* Q;
+  int64_t vs, es;
+  int n, c0, k;
+  double* W;  // n x k (column-major)
+  double* V;  // n x k, the other buffer of the W <- W (I - F) passes
+  double* C;  // c0 x k
+  double* G;  // k x k
+  double* T;  // k x k
+};
+constexpr int kReorthPasses = 3;
+
+// to_q = 0: W = Q[:, c0:keep]; to_q = 1: Q[:, c0:keep] = W.  Job blockIdx.y.
+__global__ __launch_bounds__(256) void k_reorth_move(const ReorthJob* __restrict__ jobs, int to_q) {
+  const ReorthJob jb = jobs[blockIdx.y];
+  const int64_t tot = (int64_t)jb.n * jb.k;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < tot; e += (int64_t)gridDim.x * 256) {
+    int u, r;
+    if (jb.es == 1) {  // vectors are columns: walk rows fastest
+      u = (int)(e / jb.n);
+      r = (int)(e % jb.n);
+    } else {  // vectors are rows (stride es between elements): walk vectors fastest
+      r = (int)(e / jb.k);
+      u = (int)(e % jb.k);
     }
+    double* q = jb.Q + (int64_t)(jb.c0 + u) * jb.vs + (int64_t)r * jb.es;
+    double* w = jb.W + (int64_t)u * jb.n + r;
+    if (to_q)
+      *q = *w;
+    else
+      *w = *q;
   }
-  __syncthreads();
 }
 
-// One Cholesky-QR pass on the nb columns of W (n rows each): G = W^T W
-// (pairs spread over the threads), G = R^T R (one thread: a 32 x 32
-// factorisation), then every row w of W is replaced by w R^-1 (a forward
-// substitution per row, one row per thread).
-__device__ void rb_cholqr(const CompleteJob& jb, int nb, double (*G)[kCB + 1]) {
-  const int n = jb.n;
-  const int npairs = nb * (nb + 1) / 2;
-  for (int pi = threadIdx.x; pi < npairs; pi += kBjThreads) {
-    int t = 0;
-    while ((t + 1) * (t + 2) / 2 <= pi) ++t;
-    const int u = pi - t * (t + 1) / 2;  // u <= t
-    const double* wu = jb.W + (int64_t)u * n;
-    const double* wt = jb.W + (int64_t)t * n;
-    double a = 0.0;
-    for (int r = 0; r < n; ++r) a += wu[r] * wt[r];
-    G[u][t] = a;
-    G[t][u] = a;
+// T = I - F from G (both k x k column-major): T[u][t] = -G[u][t] (u < t),
+// 1.5 - G[t][t] / 2 (u = t), 0 (u > t).  Job blockIdx.y.
+__global__ __launch_bounds__(256) void k_reorth_tri(const ReorthJob* __restrict__ jobs) {
+  const ReorthJob jb = jobs[blockIdx.y];
+  const int64_t tot = (int64_t)jb.k * jb.k;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < tot; e += (int64_t)gridDim.x * 256) {
+    const int u = (int)(e % jb.k), t = (int)(e / jb.k);
+    const double g = jb.G[e];
+    jb.T[e] = u < t ? -g : (u == t ? 1.5 - 0.5 * g : 0.0);
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    // upper Cholesky factor in place: G[u][t] (u <= t) becomes R[u][t]
-    for (int j = 0; j < nb; ++j) {
-      double d = G[j][j];
-      for (int u = 0; u < j; ++u) d -= G[u][j] * G[u][j];
-      d = sqrt(fmax(d, 1e-300));
-      G[j][j] = d;
-      for (int t = j + 1; t < nb; ++t) {
-        double a = G[j][t];
-        for (int u = 0; u < j; ++u) a -= G[u][j] * G[u][t];
-        G[j][t] = a / d;
-      }
-    }
-  }
-  __syncthreads();
-  for (int r = threadIdx.x; r < n; r += kBjThreads) {
-    double w[kCB];
-    for (int t = 0; t < nb; ++t) w[t] = jb.W[(int64_t)t * n + r];
-    for (int t = 0; t < nb; ++t) {
-      double a = w[t];
-      for (int u = 0; u < t; ++u) a -= w[u] * G[u][t];
-      w[t] = a / G[t][t];
-    }
-    for (int t = 0; t < nb; ++t) jb.W[(int64_t)t * n + r] = w[t];
-  }
-  __syncthreads();
 }
 
-__global__ __launch_bounds__(kBjThreads) void k_reorth_block(const CompleteJob* __restrict__ jobs, int njobs, int b) {
-  __shared__ double G[kCB][kCB + 1];
-  for (int ji = blockIdx.x; ji < njobs; ji += gridDim.x) {
-    const CompleteJob jb = jobs[ji];
-    const int n = jb.n;
-    int cnext = jb.c0;
-    if (b >= 0) {
-      const int cb = jb.c0 + kCB * b;
-      if (cb >= jb.keep) continue;
-      const int nb = min(kCB, jb.keep - cb);
-      rb_cholqr(jb, nb, G);
-      rb_cholqr(jb, nb, G);
-      if (jb.es == 1) {
-        for (int u = 0; u < nb; ++u)
-          for (int r = threadIdx.x; r < n; r += kBjThreads)
-            jb.Q[(int64_t)(cb + u) * jb.vs + r] = jb.W[(int64_t)u * n + r];
-      } else {
-        for (int64_t e = threadIdx.x; e < (int64_t)nb * n; e += kBjThreads) {
-          const int u = (int)(e % nb), r = (int)(e / nb);
-          jb.Q[(int64_t)(cb + u) * jb.vs + (int64_t)r * jb.es] = jb.W[(int64_t)u * n + r];
-        }
-      }
-      __syncthreads();
-      cnext = cb + nb;
-    }
-    const int nbn = min(kCB, jb.keep - cnext);
-    if (nbn > 0) rb_load(jb, cnext, nbn);
-  }
+static unsigned reorth_chunks(const std::vector<ReorthJob>& rj, bool square) {
+  int64_t mx = 1;
+  for (const ReorthJob& j : rj) mx = std::max<int64_t>(mx, square ? (int64_t)j.k * j.k : (int64_t)j.n * j.k);
+  return (unsigned)std::min<int64_t>((mx + 255) / 256, 256);
 }
 
 void reorth_basis(Context& c, std::vector<CompleteJob>& jobs) {
   if (jobs.empty()) return;
+  if (jobs.size() > 65535) throw Error(DMRJ_ERR_STATE, "too many sectors for re-orthonormalisation");
+  std::vector<ReorthJob> rj;
   std::vector<double*> scratch;
-  int nblk = 0;
-  for (CompleteJob& j : jobs) {
+  for (const CompleteJob& j : jobs) {
     if (j.c0 < 1 || j.keep > j.n || j.c0 >= j.keep) throw Error(DMRJ_ERR_STATE, "re-orthonormalisation job out of range");
-    const int64_t sz = (int64_t)kCB * j.n + (int64_t)kCB * j.keep;
-    double* p = (double*)c.alloc.alloc(sz * (int64_t)sizeof(double));
+    const int k = j.keep - j.c0;
+    const int64_t nk = (int64_t)j.n * k, kk = (int64_t)k * k;
+    double* p = (double*)c.alloc.alloc((2 * nk + (int64_t)j.c0 * k + 2 * kk) * (int64_t)sizeof(double));
     scratch.push_back(p);
-    j.W = p;
-    j.coef = j.W + (int64_t)kCB * j.n;
-    nblk = std::max(nblk, (j.keep - j.c0 + kCB - 1) / kCB);
+    rj.push_back({j.Q, j.vs, j.es, j.n, j.c0, k, p, p + nk, p + 2 * nk, p + 2 * nk + (int64_t)j.c0 * k,
+                  p + 2 * nk + (int64_t)j.c0 * k + kk});
   }
-  auto* dj = (CompleteJob*)c.upload(jobs.data(), jobs.size() * sizeof(CompleteJob));
-  const unsigned grid = (unsigned)std::min<size_t>(jobs.size(), 1024);
-  hipLaunchKernelGGL(k_reorth_block, dim3(grid), dim3(kBjThreads), 0, c.stream, dj, (int)jobs.size(), -1);
+  const unsigned nj = (unsigned)rj.size();
+  auto* dj = (ReorthJob*)c.upload(rj.data(), rj.size() * sizeof(ReorthJob));
+  hipLaunchKernelGGL(k_reorth_move, dim3(reorth_chunks(rj, false), nj), dim3(256), 0, c.stream, dj, 0);
   HIP_OK(hipGetLastError());
-  for (int b = 0; b < nblk; ++b) {
-    // W -= Q (Q^T W) twice over the vectors before the block
+  {
+    // W -= Q0 (Q0^T W), twice
     std::vector<GemmTask> t1, t2;
     std::vector<GemmSeg> s1, s2;
-    for (const CompleteJob& j : jobs) {
-      const int cb = j.c0 + kCB * b;
-      if (cb >= j.keep) continue;
-      const int nb = std::min(kCB, j.keep - cb);
+    for (const ReorthJob& j : rj) {
       const bool cols = j.es == 1;
       const int64_t ldq = cols ? j.vs : j.es;
-      t1.push_back({j.coef, j.keep, cb, nb, (int)s1.size(), 1, 0.0});
+      t1.push_back({j.C, j.c0, j.c0, j.k, (int)s1.size(), 1, 0.0});
       s1.push_back({j.Q, j.W, ldq, j.n, j.n, cols ? 1 : 0, 1.0});
-      t2.push_back({j.W, j.n, j.n, nb, (int)s2.size(), 1, 1.0});
-      s2.push_back({j.Q, j.coef, ldq, j.keep, cb, cols ? 0 : 1, -1.0});
+      t2.push_back({j.W, j.n, j.n, j.k, (int)s2.size(), 1, 1.0});
+      s2.push_back({j.Q, j.C, ldq, j.c0, j.c0, cols ? 0 : 1, -1.0});
     }
-    if (!t1.empty())
-      for (int pass = 0; pass < 2; ++pass) {
-        launch_gemm(c, t1, s1, -1);
-        launch_gemm(c, t2, s2, -1);
-      }
-    hipLaunchKernelGGL(k_reorth_block, dim3(grid), dim3(kBjThreads), 0, c.stream, dj, (int)jobs.size(), b);
-    HIP_OK(hipGetLastError());
+    for (int pass = 0; pass < 2; ++pass) {
+      launch_gemm(c, t1, s1, -1);
+      launch_gemm(c, t2, s2, -1);
+    }
   }
+  const unsigned tri_chunks = reorth_chunks(rj, true);
+  for (int pass = 0; pass < kReorthPasses; ++pass) {
+    // G = W^T W;  T = I - F;  V = W T;  then V and W trade places
+    std::vector<GemmTask> tg, tv;
+    std::vector<GemmSeg> sg, sv;
+    for (const ReorthJob& j : rj) {
+      tg.push_back({j.G, j.k, j.k, j.k, (int)sg.size(), 1, 0.0});
+      sg.push_back({j.W, j.W, j.n, j.n, j.n, 1, 1.0});
+      tv.push_back({j.V, j.n, j.n, j.k, (int)sv.size(), 1, 0.0});
+      sv.push_back({j.W, j.T, j.n, j.k, j.k, 0, 1.0});
+    }
+    launch_gemm(c, tg, sg, -1);
+    hipLaunchKernelGGL(k_reorth_tri, dim3(tri_chunks, nj), dim3(256), 0, c.stream, dj);
+    HIP_OK(hipGetLastError());
+    launch_gemm(c, tv, sv, -1);
+    for (ReorthJob& j : rj) std::swap(j.W, j.V);
+    dj = (ReorthJob*)c.upload(rj.data(), rj.size() * sizeof(ReorthJob));
+  }
+  hipLaunchKernelGGL(k_reorth_move, dim3(reorth_chunks(rj, false), nj), dim3(256), 0, c.stream, dj, 1);
+  HIP_OK(hipGetLastError());
   for (double* p : scratch) c.alloc.release(p);
 }
 

@@ -24,29 +24,33 @@ def _theta(ctx, a, b):
     return DeviceTensor(ctx, h)
 
 
-def krylovTimeEvol2(Lenv, Renv, AA, mpo, iL, r, prefactor, precision="f64"):
+def krylovTimeEvol2(Lenv, Renv, AA, mpo, iL, r, prefactor, precision="f64", energy=True):
     """krylovTimeEvol with joinedtwosite_update (algs/time.jl:169-202);
-    complex prefactor (-i dt: real time)."""
+    complex prefactor (-i dt: real time).  energy=False leaves the lowest
+    Ritz value on the device (None is returned): the call then has no host
+    round trip."""
     ctx = AA.ctx
     h = C.c_void_p()
     e = C.c_double()
     k = C.c_int()
     pf = complex(prefactor)
     ctx.call("dmrj_krylov_expm2", Lenv.h, Renv.h, AA.h, mpo.device(ctx, iL), mpo.device(ctx, iL + 1), int(r),
-             pf.real, pf.imag, _lib.PREC[precision], C.byref(h), C.byref(e), C.byref(k))
-    return DeviceTensor(ctx, h), e.value
+             pf.real, pf.imag, _lib.PREC[precision], C.byref(h), C.byref(e) if energy else None,
+             C.byref(k) if energy else None)
+    return DeviceTensor(ctx, h), (e.value if energy else None)
 
 
-def krylovTimeEvol1(Lenv, Renv, A, mpo, i, r, prefactor, precision="f64"):
-    """krylovTimeEvol with singlesite_update (algs/time.jl:182-202)."""
+def krylovTimeEvol1(Lenv, Renv, A, mpo, i, r, prefactor, precision="f64", energy=True):
+    """krylovTimeEvol with singlesite_update (algs/time.jl:182-202);
+    energy=False as for krylovTimeEvol2."""
     ctx = A.ctx
     h = C.c_void_p()
     e = C.c_double()
     k = C.c_int()
     pf = complex(prefactor)
     ctx.call("dmrj_krylov_expm1", Lenv.h, Renv.h, A.h, mpo.device(ctx, i), int(r), pf.real, pf.imag,
-             _lib.PREC[precision], C.byref(h), C.byref(e), C.byref(k))
-    return DeviceTensor(ctx, h), e.value
+             _lib.PREC[precision], C.byref(h), C.byref(e) if energy else None, C.byref(k) if energy else None)
+    return DeviceTensor(ctx, h), (e.value if energy else None)
 
 
 def _normalize(t):
@@ -78,11 +82,12 @@ def tdvp_twosite(psi, mpo, m=0, minm=2, cutoff=0.0, r=2, prefactor=1.0, Lenv=Non
     j = 1 if psi.oc == 1 else (-1 if psi.oc == Ns else 1)
     En = 0.0
     ctx = psi.ctx
-    for _ in range(Ns - 1):
+    for step in range(Ns - 1):
+        last = step == Ns - 2     # the energy the half-sweep returns is the last one's
         i = psi.oc
         iL, iR = (i, i + 1) if j > 0 else (i - 1, i)
         AA = _theta(ctx, psi[iL], psi[iR])
-        newpsi, En = krylovTimeEvol2(Lenv[iL], Renv[iR], AA, mpo, iL, r, timestep, precision)
+        newpsi, _ = krylovTimeEvol2(Lenv[iL], Renv[iR], AA, mpo, iL, r, timestep, precision, energy=False)
         AA.free()
         _normalize(newpsi)
         left, right, _ = _svd(newpsi, m, minm, cutoff, 1 if j > 0 else -1)
@@ -93,14 +98,16 @@ def tdvp_twosite(psi, mpo, m=0, minm=2, cutoff=0.0, r=2, prefactor=1.0, Lenv=Non
             psi[iL] = left
             Lenv[iR] = Lupdate(Lenv[iL], psi[iL], mpo, iL)
             _normalize(right)
-            psi[iR], En = krylovTimeEvol1(Lenv[iR], Renv[iR], right, mpo, iR, r, timestep, precision)
+            psi[iR], e1 = krylovTimeEvol1(Lenv[iR], Renv[iR], right, mpo, iR, r, timestep, precision, energy=last)
             right.free()
         else:
             psi[iR] = right
             Renv[iL] = Rupdate(Renv[iR], psi[iR], mpo, iR)
             _normalize(left)
-            psi[iL], En = krylovTimeEvol1(Lenv[iL], Renv[iL], left, mpo, iL, r, timestep, precision)
+            psi[iL], e1 = krylovTimeEvol1(Lenv[iL], Renv[iL], left, mpo, iL, r, timestep, precision, energy=last)
             left.free()
+        if last:
+            En = e1
         psi.oc += j
         _normalize(psi[psi.oc])
         if psi.oc == Ns and j > 0:

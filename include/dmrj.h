@@ -282,7 +282,11 @@ int dmrj_lanczos2(dmrj_ctx* ctx, const dmrj_tensor* L, const dmrj_tensor* R,
  * v_j with v_1 = AA/|AA|; energy = lowest Ritz value, kdim = Krylov
  * dimension used (smaller when the space is exhausted).  Complex prefactor
  * prefactor_re + i prefactor_im: -i dt evolves in real time (complex output),
- * a real prefactor in imaginary time. */
+ * a real prefactor in imaginary time.  The tridiagonal eigensolve and the
+ * exponential run on the device; energy and kdim may be NULL, and then the
+ * call synchronises nothing (non-finite Krylov scalars are reported only
+ * when they are requested; the truncated SVD that follows rejects a
+ * non-finite tensor). */
 int dmrj_krylov_expm2(dmrj_ctx* ctx, const dmrj_tensor* L, const dmrj_tensor* R, const dmrj_tensor* AA,
                       const dmrj_mpo* W1, const dmrj_mpo* W2, int krylov, double prefactor_re,
                       double prefactor_im, int precision, dmrj_tensor** out, double* energy, int* kdim);
