@@ -116,6 +116,9 @@ def main():
     ap.add_argument("--converge", type=int, default=0,
                     help="with --compare: first sweep at m (up to this many sweeps) until the energy changes by "
                          "less than 1e-11 relative, so that the variants start from a converged state")
+    ap.add_argument("--step-errors", type=int, default=1,
+                    help="headline: 1 = one more mixed sweep whose every bond step is also solved in FP64 and in the "
+                         "plain split form from the same inputs (per-bond energy deviations); 0 = skip it")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run one block-copy kernel over this many doubles (counter calibration)")
     args = ap.parse_args()
@@ -490,6 +493,14 @@ def bench_headline(pkg, ctx, args):
     log("FP64 drop-in %.3f s/sweep, mixed %.3f s/sweep: %.2fx; |dE/E| per sweep %s; centre spectrum |dD| %.2e"
         % (t_fp64, t_mx, t_fp64 / t_mx, ["%.2e" % r for r in rel], dspec))
 
+    steperr = None
+    if args.step_errors:
+        ctx.set_svd_backend(args.mixed_svd)
+        st2 = kept[0].copy()
+        steperr = bond_step_errors(pkg, ctx, st2, mpo, m, args.cutoff, mixed)
+        st2.free()
+        ctx.set_svd_backend(0)
+
     cpu = None
     if not args.no_cpu:
         st, p2 = kept
@@ -530,6 +541,7 @@ def bench_headline(pkg, ctx, args):
         "rel_dE_per_sweep": rel,
         "max_rel_dE": max(rel) if rel else None,
         "centre_spectrum_max_abs_dD": dspec,
+        "bond_step_rel_dE_vs_fp64": steperr,
         "comparison": rows,
         "roofline": roof_mx,
         "roofline_fp64_dropin": roof_64,
@@ -539,6 +551,48 @@ def bench_headline(pkg, ctx, args):
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def bond_step_errors(pkg, ctx, st, mpo, m, cutoff, mixed):
+    """One mixed sweep from the state st in which every bond step first runs
+    the Lanczos step (dmrj_lanczos2) on the same two-site tensor and
+    environments in FP64, in the split form, and in the split form with the
+    FP64 Rayleigh-quotient correction, then takes the mixed step itself
+    (dmrj_twostep, the sweep's own arithmetic).  Returns the per-bond
+    relative energy deviations of the two split forms from the FP64 step:
+    the precision form's error on identical inputs, apart from the
+    trajectory differences that whole sweeps from an unconverged state
+    accumulate."""
+    import ctypes as C
+    name = mixed["precision"]
+    rec = []
+
+    def step(n, j, i, iL, iR, psi, mpo_, Lenv, Renv, params):
+        out = {}
+        for key in ("f64", name, name + "c"):
+            h, e = C.c_void_p(), C.c_double()
+            ctx.call("dmrj_lanczos2", Lenv[iL].h, Renv[iR].h, psi[iL].h, psi[iR].h, mpo_.device(ctx, iL),
+                     mpo_.device(ctx, iR), pkg._lib.PREC[key], C.byref(h), C.byref(e))
+            pkg.DeviceTensor(ctx, h).free()
+            out[key] = e.value
+        rec.append(out)
+        pkg.twostep(n, j, i, iL, iR, psi, mpo_, Lenv, Renv, params)
+
+    p = pkg.algvars(st)
+    p.maxm, p.minm, p.sweeps, p.cutoff = m, 2, 1, cutoff
+    p.precision, p.env_precision, p.correction, p.method = name, "f64", mixed["correction"], "twosite"
+    pkg.optmps(st, mpo, p, stepfct=step)
+    _free_envs(p)
+    res = {"bonds": len(rec)}
+    for key in (name, name + "c"):
+        r = np.array([abs(x[key] - x["f64"]) / abs(x["f64"]) for x in rec])
+        res[key] = {"max": float(r.max()), "median": float(np.median(r)),
+                    "bonds_le_1e-10": int(np.sum(r <= 1e-10))}
+    log("bond steps of one mixed sweep vs FP64 on identical inputs: %s max %.2e median %.2e; with FP64 "
+        "correction max %.2e median %.2e (%d of %d bonds <= 1e-10)"
+        % (name, res[name]["max"], res[name]["median"], res[name + "c"]["max"], res[name + "c"]["median"],
+           res[name + "c"]["bonds_le_1e-10"], len(rec)))
+    return res
 
 
 def cpu_sweep_baseline(pkg, ctx, psi, mpo, params, budget_s, m, cutoff):
