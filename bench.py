@@ -116,12 +116,16 @@ def main():
     ap.add_argument("--converge", type=int, default=0,
                     help="with --compare: first sweep at m (up to this many sweeps) until the energy changes by "
                          "less than 1e-11 relative, so that the variants start from a converged state")
+    ap.add_argument("--progress", action="store_true",
+                    help="headline: a progress line every 16 bond steps (runs under a counter-collecting profiler)")
     ap.add_argument("--step-errors", type=int, default=1,
                     help="headline: 1 = one more mixed sweep whose every bond step is also solved in FP64 and in the "
                          "plain split form from the same inputs (per-bond energy deviations); 0 = skip it")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run one block-copy kernel over this many doubles (counter calibration)")
     args = ap.parse_args()
+    if args.cmp_sweeps < 1:
+        ap.error("--cmp-sweeps must be at least 1")
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (one GPU per process; the "
               "sharding of the bond step across GPUs is a design, not code); run with --gpus 1",
@@ -369,8 +373,9 @@ def _traffic(kernel, model, L, m, precision, sweep):
 
 GEMM_KINDS = {"f64": ("gemm_f64", "k_gemm_f64", "FP64 grouped GEMM, MFMA 16x16x4 f64", PEAK_FP64),
               "bf16x3": ("gemm_lp", "k_gemm_bf16x3", "BF16x3 split grouped GEMM, MFMA 16x16x32 bf16", PEAK_BF16),
-              "f16x3": ("gemm_f16", "k_gemm_f16x3", "per-block scaled FP16x3 split grouped GEMM, MFMA 16x16x32 "
-                        "f16", PEAK_BF16)}
+              "f16x3": ("gemm_f16", "k_gemm_f16p", "per-block scaled FP16x3 grouped GEMM on pre-split operand planes, "
+                        "MFMA 16x16x32 f16; its pre-passes k_region_absmax and k_f16_presplit are timed apart "
+                        "(profiled sweep f16_prep_s)", PEAK_BF16)}
 
 
 def _roofline(stats, prec, model, L, m, sweep):
@@ -401,7 +406,25 @@ def bench_headline(pkg, ctx, args):
     ctx.set_svd_backend(args.mixed_svd)
 
     def sweeps(st, p, mm, n, cfg):
-        pkg.dmrg(st, mpo, m=mm, sweeps=n, cutoff=args.cutoff, params=p, **cfg)
+        if not args.progress:
+            pkg.dmrg(st, mpo, m=mm, sweeps=n, cutoff=args.cutoff, params=p, **cfg)
+            return
+        # the same sweeps with a line every 16 bond steps (for runs under a
+        # counter-collecting profiler, where one sweep takes minutes)
+        count = [0]
+
+        def step(nn, j, i, iL, iR, psi_, mpo_, Lenv, Renv, params):
+            pkg.twostep(nn, j, i, iL, iR, psi_, mpo_, Lenv, Renv, params)
+            count[0] += 1
+            if count[0] % 16 == 0:
+                log("  bond steps done: %d" % count[0])
+
+        p.method = "twosite"
+        p.maxm, p.minm, p.sweeps, p.cutoff = mm, 2, n, args.cutoff
+        p.silent, p.goal, p.precision = True, 0.0, cfg["precision"]
+        p.env_precision = cfg["env_precision"]
+        p.correction = cfg["correction"] or ("none" if cfg["precision"] == "f64" else "sweep")
+        pkg.optmps(st, mpo, p, stepfct=step)
 
     t_ramp = time.perf_counter()
     for mr in [int(x) for x in args.ramp.split(",") if x]:
@@ -452,14 +475,14 @@ def bench_headline(pkg, ctx, args):
         hist = list(p2.history[-args.cmp_sweeps:])
         # one more sweep with HIP events on every grouped-GEMM launch
         ctx.profile(True)
-        for k in ("gemm_f64", "mix", "copy", "svd", "gemm_lp", "gemm_f16"):
+        for k in ("gemm_f64", "mix", "copy", "svd", "gemm_lp", "gemm_f16", "f16_prep"):
             ctx.kernel_stats(k, reset=True)
         ctx.marker(marks[name][0])
         sweeps(st, p2, m, 1, cfg)
         ctx.sync()
         ctx.marker(marks[name][1])
         ctx.sync()
-        stats = {k: ctx.kernel_stats(k) for k in ("gemm_f64", "mix", "copy", "svd", "gemm_lp", "gemm_f16")}
+        stats = {k: ctx.kernel_stats(k) for k in ("gemm_f64", "mix", "copy", "svd", "gemm_lp", "gemm_f16", "f16_prep")}
         ctx.profile(False)
         prof = p2.history[-1]
         rows[name] = {"svd_backend": backend, "precision": cfg["precision"],
@@ -472,6 +495,7 @@ def bench_headline(pkg, ctx, args):
                                          "gemm_f64_s": round(stats["gemm_f64"]["ms"] / 1e3, 4),
                                          "gemm_split_s": round((stats["gemm_lp"]["ms"] + stats["gemm_f16"]["ms"])
                                                                / 1e3, 4),
+                                         "f16_prep_s": round(stats["f16_prep"]["ms"] / 1e3, 4),
                                          "mix_s": round(stats["mix"]["ms"] / 1e3, 4)},
                       "_stats": stats}
         log("%s: sweeps %s s, E %s; profiled sweep %.3f s (svd %.3f s, gemm f64 %.3f s, split gemm %.3f s)"

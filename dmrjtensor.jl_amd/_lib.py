@@ -33,6 +33,7 @@ SIGNATURES = {
     "dmrj_ctx_set_eig_batching": [_p, C.c_int],
     "dmrj_ctx_set_autotune": [_p, C.c_int],
     "dmrj_ctx_set_tile_order": [_p, C.c_int],
+    "dmrj_ctx_set_f16_presplit": [_p, C.c_int],
     "dmrj_ctx_svd_stats": [_p, _dp, C.c_int],
     "dmrj_ctx_autotune_stats": [_p, _dp],
     "dmrj_ctx_set_graph": [_p, C.c_int],
@@ -176,7 +177,7 @@ class Context:
     def profile(self, enable=True):
         self.call("dmrj_ctx_profile", 1 if enable else 0)
 
-    KINDS = {"gemm_f64": 0, "mix": 1, "copy": 2, "svd": 3, "gemm_lp": 4, "graph": 5, "gemm_f16": 6}
+    KINDS = {"gemm_f64": 0, "mix": 1, "copy": 2, "svd": 3, "gemm_lp": 4, "graph": 5, "gemm_f16": 6, "f16_prep": 7}
 
     def kernel_stats(self, kind, reset=False):
         out = (C.c_double * 4)()
@@ -197,6 +198,9 @@ class Context:
         out = np.zeros(2)
         self.call("dmrj_ctx_svd_stats", dptr(out), 1 if reset else 0)
         return dict(calls=int(out[0]), sweeps=int(out[1]))
+
+    def set_f16_presplit(self, on):
+        self.call("dmrj_ctx_set_f16_presplit", 1 if on else 0)
 
     def set_tile_order(self, mode):
         """1 = XCD-aware grouped-GEMM tile order (default), 0 = descending work only."""

@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
 
 #include "dmrj.h"
 #include "internal.h"
@@ -29,9 +30,99 @@ namespace dmrj {
 const TileShape kGemmShapes[kNumGemmShapes] = {{32, 32}, {64, 64}, {128, 128}, {64, 128}, {128, 64}};
 
 static int occupancy(Context& c, int cfg, int arith) {
-  int& o = c.occ_cache[arith][cfg];
-  if (o == 0) o = arith == 1 ? gemm_occupancy_lp(cfg) : (arith == 2 ? gemm_occupancy_f16(cfg) : gemm_occupancy_f64(cfg));
+  const bool planes = arith == 2 && c.f16_presplit;
+  int& o = c.occ_cache[planes ? 3 : arith][cfg];
+  if (o == 0)
+    o = arith == 1 ? gemm_occupancy_lp(cfg)
+                   : (arith == 2 ? (planes ? gemm_occupancy_f16p(cfg) : gemm_occupancy_f16(cfg)) : gemm_occupancy_f64(cfg));
   return o;
+}
+
+// FP16x3 operand planes of a launch: one region per distinct operand block
+// (the same pointer, leading dimension, shape, storage order and alpha), its
+// planes at 256-byte aligned offsets of one transient buffer, the split work
+// items and every segment's plane pointers.
+static void prepare_f16_planes(Context& c, GemmLaunch& g, const std::vector<GemmTask>& tasks,
+                               const std::vector<GemmSeg>& segs) {
+  struct Key {
+    const double* p;
+    int64_t ld;
+    int rows, K, tr, op;
+    double alpha;
+    bool operator<(const Key& o) const {
+      if (p != o.p) return p < o.p;
+      if (ld != o.ld) return ld < o.ld;
+      if (rows != o.rows) return rows < o.rows;
+      if (K != o.K) return K < o.K;
+      if (tr != o.tr) return tr < o.tr;
+      if (op != o.op) return op < o.op;
+      return alpha < o.alpha;
+    }
+  };
+  std::map<Key, int> idx;
+  std::vector<F16Region> regs;
+  std::vector<int64_t> off;
+  std::vector<int2> rsel(segs.size(), make_int2(-1, -1));
+  int64_t halves = 0;
+  auto region = [&](const Key& k) {
+    auto it = idx.find(k);
+    if (it != idx.end()) return it->second;
+    const int kpad = (k.K + 31) / 32 * 32;
+    F16Region r{k.p, k.ld, k.rows, k.K, kpad, k.tr, k.op, k.alpha, nullptr, nullptr};
+    regs.push_back(r);
+    off.push_back(halves);
+    halves += ((int64_t)2 * k.rows * kpad + 127) / 128 * 128;  // hi then lo, 256-byte aligned
+    idx[k] = (int)regs.size() - 1;
+    return (int)regs.size() - 1;
+  };
+  for (size_t ti = 0; ti < tasks.size(); ++ti) {
+    const GemmTask& t = tasks[ti];
+    for (int q = 0; q < t.nseg; ++q) {
+      const int si = t.seg0 + q;
+      const GemmSeg& sg = segs[si];
+      if (sg.K <= 0 || t.M <= 0 || t.N <= 0) continue;
+      const bool tA = sg.flags & 1, tB = sg.flags & 2;
+      // A element (m, k): tA ? A[k + m lda] : A[m + k lda]; B element (k, n) as row n:
+      // tB ? B[n + k ldb] : B[k + n ldb]
+      rsel[si].x = region(Key{sg.A, sg.lda, t.M, sg.K, tA ? 1 : 0, 0, sg.alpha});
+      rsel[si].y = region(Key{sg.B, sg.ldb, t.N, sg.K, tB ? 0 : 1, 1, 1.0});
+    }
+  }
+  g.presplit = true;
+  uint16_t* planes = (uint16_t*)c.alloc.alloc(std::max<int64_t>(halves, 128) * sizeof(uint16_t));
+  c.transient.push_back(planes);  // lives until the next synchronisation point
+  std::vector<int4> work, mwork;
+  for (size_t r = 0; r < regs.size(); ++r) {
+    regs[r].hi = planes + off[r];
+    regs[r].lo = regs[r].hi + (int64_t)regs[r].rows * regs[r].kpad;
+    for (int r0 = 0; r0 < regs[r].rows; r0 += 64)
+      for (int k0 = 0; k0 < regs[r].kpad; k0 += 32) work.push_back(make_int4((int)r, r0, k0, 0));
+    // the maximum pass walks the operand as stored: columns of srows elements
+    const int64_t srows = regs[r].tr ? regs[r].K : regs[r].rows;
+    const int64_t scols = regs[r].tr ? regs[r].rows : regs[r].K;
+    const int64_t step = std::max<int64_t>(1, 16384 / std::max<int64_t>(srows, 1));
+    for (int64_t c0 = 0; c0 < scols; c0 += step)
+      mwork.push_back(make_int4((int)r, (int)c0, (int)std::min(scols, c0 + step), 0));
+  }
+  std::vector<F16Seg> fs(std::max<size_t>(1, segs.size()));
+  for (size_t si = 0; si < segs.size(); ++si) {
+    if (rsel[si].x < 0) {
+      fs[si] = F16Seg{nullptr, nullptr, nullptr, nullptr, 0, 0, 0, 0};
+      continue;
+    }
+    const F16Region& a = regs[rsel[si].x];
+    const F16Region& b = regs[rsel[si].y];
+    fs[si] = F16Seg{a.hi, a.lo, b.hi, b.lo, a.kpad, rsel[si].x, rsel[si].y, 0};
+  }
+  g.nregions = (int)regs.size();
+  g.regionmax = (double*)c.alloc.alloc(std::max<size_t>(1, regs.size()) * sizeof(double));
+  c.transient.push_back(g.regionmax);
+  g.regionmaxwork = mwork.empty() ? nullptr : (const int4*)c.upload(mwork.data(), mwork.size() * sizeof(int4));
+  g.nregionmaxwork = (int)mwork.size();
+  g.regions = regs.empty() ? nullptr : (const F16Region*)c.upload(regs.data(), regs.size() * sizeof(F16Region));
+  g.splitwork = work.empty() ? nullptr : (const int4*)c.upload(work.data(), work.size() * sizeof(int4));
+  g.nsplitwork = (int)work.size();
+  g.f16segs = (const F16Seg*)c.upload(fs.data(), fs.size() * sizeof(F16Seg));
 }
 
 // XCD-aware order.  Work-group b of a launch runs on XCD b mod 8 (the
@@ -136,8 +227,12 @@ GemmLaunch prepare_gemm(Context& c, const std::vector<GemmTask>& tasks, const st
     g.segtask = (const int*)c.upload(st.data(), st.size() * sizeof(int));
     g.segmax = (double*)c.alloc.alloc(std::max<size_t>(2, 2 * segs.size()) * sizeof(double));
     c.transient.push_back(g.segmax);  // lives until the next synchronisation point
-    if (!mw.empty()) g.maxwork = (const int4*)c.upload(mw.data(), mw.size() * sizeof(int4));
-    g.nmaxwork = (int)mw.size();
+    if (c.f16_presplit) {
+      prepare_f16_planes(c, g, tasks, segs);  // maxima per distinct operand block instead
+    } else {
+      if (!mw.empty()) g.maxwork = (const int4*)c.upload(mw.data(), mw.size() * sizeof(int4));
+      g.nmaxwork = (int)mw.size();
+    }
   }
   return g;
 }

@@ -224,7 +224,8 @@ struct Context {
   hipEvent_t graph_done[kGraphSlots] = {};
   double graph_updates = 0, graph_instantiations = 0;
   std::unordered_map<uint64_t, int> tune_cache;
-  int occ_cache[3][8] = {{0}};
+  int occ_cache[4][8] = {{0}};
+  int f16_presplit = 1;  // FP16x3 GEMMs on pre-split operand planes (dmrj_ctx_set_f16_presplit)
   double tune_hits = 0, tune_misses = 0, tune_ms = 0, tune_skipped = 0;
   // statistics
   double flops_heff = 0.0, flops_env = 0.0;
@@ -408,6 +409,29 @@ extern const TileShape kGemmShapes[kNumGemmShapes];
 // Descriptor tables of one grouped-GEMM launch, already on the device.
 // arith: 0 = FP64 MFMA, 1 = BF16x3 split, 2 = FP16x3 split with per-segment
 // power-of-two scaling (the values of DMRJ_PREC_F64 / _BF16X3 / _F16X3).
+// FP16x3 operand planes (Context::f16_presplit): every distinct operand
+// region of a launch is scaled by its segment's power of two and split once
+// into FP16 hi / lo planes, rows x kpad each, k contiguous (kpad = K rounded
+// up to 32, zero-filled), so the GEMM reads MFMA fragments directly.
+struct F16Region {
+  const double* src;
+  int64_t ld;
+  int rows, K, kpad;
+  int tr;        // 1: element (r, k) at src[k + r ld]; 0: at src[r + k ld]
+  int operand;   // 0 A, 1 B
+  double alpha;  // folded into A (1 for B)
+  uint16_t* hi;  // FP16 bit patterns
+  uint16_t* lo;
+};
+struct F16Seg {
+  const uint16_t* ah;
+  const uint16_t* al;
+  const uint16_t* bh;
+  const uint16_t* bl;
+  int kpad;
+  int ra, rb;  // regions of A and B (their maxima scale the segment)
+  int pad;
+};
 struct GemmLaunch {
   const GemmTile* tiles = nullptr;
   const GemmTask* tasks = nullptr;
@@ -417,16 +441,29 @@ struct GemmLaunch {
   const int4* maxwork = nullptr;     // FP16x3: work items of the max pre-pass
   const int* segtask = nullptr;      // FP16x3: task of every segment
   int nmaxwork = 0;
+  bool presplit = false;             // FP16x3 with operand planes
+  const F16Region* regions = nullptr;
+  const int4* splitwork = nullptr;   // (region, first row, first k, 0)
+  int nsplitwork = 0;
+  const F16Seg* f16segs = nullptr;
+  double* regionmax = nullptr;       // max |alpha A| or max |B| of every region
+  const int4* regionmaxwork = nullptr;  // (region, first stored column, end column, 0)
+  int nregionmaxwork = 0, nregions = 0;
 };
 GemmLaunch prepare_gemm(Context& c, const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs,
                         int cfg, int arith);
 void issue_gemm_f64(Context& c, const GemmLaunch& g);
 void issue_gemm_lp(Context& c, const GemmLaunch& g);
 void issue_gemm_f16(Context& c, const GemmLaunch& g);
+// FP16x3 in two parts (plane path): the operand maxima and the plane split,
+// then the GEMM kernel alone (timed by its own dispatch when ext events are set)
+void issue_gemm_f16_prep(Context& c, const GemmLaunch& g);
+void issue_gemm_f16_main(Context& c, const GemmLaunch& g);
 void issue_gemm(Context& c, const GemmLaunch& g);
 int gemm_occupancy_f64(int cfg);
 int gemm_occupancy_lp(int cfg);
 int gemm_occupancy_f16(int cfg);
+int gemm_occupancy_f16p(int cfg);
 double gemm_flops(const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs);
 double gemm_bytes(const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs);
 // Tile shape for a task list: cached per block-size-histogram key; on a
@@ -472,7 +509,7 @@ double gemm_flops(const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>
 
 // kernel kinds for Context profiling
 enum KernelKind { KIND_GEMM_F64 = 0, KIND_MIX = 1, KIND_COPY = 2, KIND_SVD = 3, KIND_GEMM_LP = 4, KIND_GRAPH = 5,
-                  KIND_GEMM_F16 = 6 };
+                  KIND_GEMM_F16 = 6, KIND_F16_PREP = 7 };
 
 // Lanczos scalar slots in Context::scal
 enum Slot {

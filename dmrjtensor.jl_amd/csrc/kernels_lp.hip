@@ -354,6 +354,229 @@ __global__ __launch_bounds__(kLpThreads) void k_gemm_f16x3(const GemmTile* __res
   }
 }
 
+// Operand planes.  Work item (region, r0, k0): the 64 x 32 block of rows
+// r0.. and k0.. of one region, read along the source's contiguous index into
+// LDS, scaled by the region's power of two (alpha folded into A), split into
+// FP16 hi / lo exactly as the on-load split of k_gemm_f16x3 does, and written
+// k-contiguous (pairs of halves per 32-bit store).  k >= K is written as 0.
+// Maximum of |alpha A| or |B| over every region: work item (region, first
+// stored column, end column) over the operand as stored (columns of K
+// elements when tr, else of rows elements).
+__global__ __launch_bounds__(256) void k_region_absmax(const F16Region* __restrict__ regions,
+                                                      const int4* __restrict__ work, int nwork,
+                                                      double* __restrict__ regionmax) {
+  __shared__ double red[4];
+  for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+    const int4 w = work[wi];
+    const F16Region rg = regions[w.x];
+    const int srows = rg.tr ? rg.K : rg.rows;
+    const int ncol = w.z - w.y;
+    double m = 0.0;
+    for (int64_t e = threadIdx.x; e < (int64_t)srows * ncol; e += 256) {
+      const int r = (int)(e % srows), cc = w.y + (int)(e / srows);
+      m = fmax(m, fabs(rg.src[(int64_t)r + (int64_t)cc * rg.ld]));
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double v = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+      if (rg.operand == 0) v *= fabs(rg.alpha);
+      // non-negative doubles order like their bit patterns
+      atomicMax(reinterpret_cast<unsigned long long*>(regionmax + w.x), (unsigned long long)__double_as_longlong(v));
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_f16_presplit(const F16Region* __restrict__ regions,
+                                                     const int4* __restrict__ work, int nwork,
+                                                     const double* __restrict__ regionmax) {
+  __shared__ double tile[64][33];
+  for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+    const int4 w = work[wi];
+    const F16Region rg = regions[w.x];
+    const int r0 = w.y, k0 = w.z;
+    const double f = ldexp(rg.alpha, f16_scale_exp(regionmax[w.x]));
+    for (int q = threadIdx.x; q < 64 * 32; q += 256) {
+      int r, k;
+      if (rg.tr) {
+        k = q % 32;
+        r = q / 32;
+      } else {
+        r = q % 64;
+        k = q / 64;
+      }
+      const int gr = r0 + r, gk = k0 + k;
+      double v = 0.0;
+      if (gr < rg.rows && gk < rg.K)
+        v = rg.tr ? rg.src[(int64_t)gk + (int64_t)gr * rg.ld] : rg.src[(int64_t)gr + (int64_t)gk * rg.ld];
+      tile[r][k] = v * f;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < 64 * 16; q += 256) {
+      const int kp = q % 16, r = q / 16;
+      const int gr = r0 + r;
+      if (gr < rg.rows) {
+        _Float16 h0, l0, h1, l1;
+        split_f16(tile[r][2 * kp], h0, l0);
+        split_f16(tile[r][2 * kp + 1], h1, l1);
+        const int64_t o = (int64_t)gr * rg.kpad + k0 + 2 * kp;
+        const uint32_t hp = (uint32_t)__builtin_bit_cast(uint16_t, h0) | ((uint32_t)__builtin_bit_cast(uint16_t, h1) << 16);
+        const uint32_t lp = (uint32_t)__builtin_bit_cast(uint16_t, l0) | ((uint32_t)__builtin_bit_cast(uint16_t, l1) << 16);
+        *reinterpret_cast<uint32_t*>(rg.hi + o) = hp;
+        *reinterpret_cast<uint32_t*>(rg.lo + o) = lp;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// FP16x3 grouped GEMM on operand planes: the same tiles, K steps and MFMA
+// sequence as k_gemm_f16x3 (so the same bits), each lane reading its 16-byte
+// fragments straight from the planes; the next K step's fragments are loaded
+// before the current step's products.
+template <int BM, int BN>
+__global__ __launch_bounds__(kLpThreads) void k_gemm_f16p(const GemmTile* __restrict__ tiles, int ntiles,
+                                                           const GemmTask* __restrict__ tasks,
+                                                           const GemmSeg* __restrict__ segs,
+                                                           const F16Seg* __restrict__ fsegs,
+                                                           const double* __restrict__ regionmax) {
+  constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int lr = lane & 15, lk = lane >> 4;
+  const half8_t zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const GemmTile tl = tiles[t];
+    const GemmTask tk = tasks[tl.task];
+    const int m0 = tl.m0, n0 = tl.n0;
+    f32x4_t tot[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) tot[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    for (int sidx = 0; sidx < tk.nseg; ++sidx) {
+      const int si = tk.seg0 + sidx;
+      const int K = segs[si].K;
+      if (K <= 0) continue;
+      const F16Seg fs = fsegs[si];
+      const int ea = f16_scale_exp(regionmax[fs.ra]), eb = f16_scale_exp(regionmax[fs.rb]);
+      int64_t aoff[FM], boff[FN];
+      bool aok[FM], bok[FN];
+      // rows / columns beyond the task read row 0 of the plane (always
+      // present) and are replaced by zeros, so every load stays in bounds
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        const int row = m0 + wm * WM + i * 16 + lr;
+        aok[i] = row < tk.M;
+        aoff[i] = (int64_t)(aok[i] ? row : 0) * fs.kpad + lk * 8;
+      }
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int col = n0 + wn * WN + j * 16 + lr;
+        bok[j] = col < tk.N;
+        boff[j] = (int64_t)(bok[j] ? col : 0) * fs.kpad + lk * 8;
+      }
+      half8_t ah[FM], al[FM], bh[FN], bl[FN];
+      auto fetch = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+          const half8_t h = *reinterpret_cast<const half8_t*>(fs.ah + aoff[i] + k0);
+          const half8_t l = *reinterpret_cast<const half8_t*>(fs.al + aoff[i] + k0);
+          ah[i] = aok[i] ? h : zero8;
+          al[i] = aok[i] ? l : zero8;
+        }
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          const half8_t h = *reinterpret_cast<const half8_t*>(fs.bh + boff[j] + k0);
+          const half8_t l = *reinterpret_cast<const half8_t*>(fs.bl + boff[j] + k0);
+          bh[j] = bok[j] ? h : zero8;
+          bl[j] = bok[j] ? l : zero8;
+        }
+      };
+      f32x4_t acc[FM][FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+      fetch(0);
+      for (int k0 = 0; k0 < K; k0 += kLpBK) {
+        half8_t cah[FM], cal[FM], cbh[FN], cbl[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+          cah[i] = ah[i];
+          cal[i] = al[i];
+        }
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          cbh[j] = bh[j];
+          cbl[j] = bl[j];
+        }
+        if (k0 + kLpBK < K) fetch(k0 + kLpBK);
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) {
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(cal[i], cbh[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(cah[i], cbl[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(cah[i], cbh[j], acc[i][j], 0, 0, 0);
+          }
+      }
+      const float unscale = ldexpf(1.0f, -(ea + eb));  // exact power of two
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) tot[i][j] += acc[i][j] * unscale;
+    }
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int col = n0 + wn * WN + j * 16 + lr;
+        if (col >= tk.N) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm * WM + i * 16 + lk * 4 + r;
+          if (row < tk.M) {
+            double* p = tk.C + (int64_t)row + (int64_t)col * tk.ldc;
+            double v = (double)tot[i][j][r];
+            if (tk.beta != 0.0) v += tk.beta * *p;
+            *p = v;
+          }
+        }
+      }
+  }
+}
+
+template <int BM, int BN>
+static void gemm_f16p_kernel_launch(Context& c, const GemmLaunch& g) {
+  if (c.ext_a)
+    hipExtLaunchKernelGGL((k_gemm_f16p<BM, BN>), dim3(g.grid), dim3(kLpThreads), 0, c.stream, c.ext_a, c.ext_b, 0,
+                          g.tiles, g.ntiles, g.tasks, g.segs, g.f16segs, g.regionmax);
+  else
+    hipLaunchKernelGGL((k_gemm_f16p<BM, BN>), dim3(g.grid), dim3(kLpThreads), 0, c.stream, g.tiles, g.ntiles,
+                       g.tasks, g.segs, g.f16segs, g.regionmax);
+}
+
+static const void* gemm_f16p_kernel_ptr(int cfg) {
+  switch (cfg) {
+    case 0: return (const void*)k_gemm_f16p<32, 32>;
+    case 1: return (const void*)k_gemm_f16p<64, 64>;
+    case 2: return (const void*)k_gemm_f16p<128, 128>;
+    case 3: return (const void*)k_gemm_f16p<64, 128>;
+    default: return (const void*)k_gemm_f16p<128, 64>;
+  }
+}
+
+int gemm_occupancy_f16p(int cfg) {
+  int nb = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, gemm_f16p_kernel_ptr(cfg), kLpThreads, 0);
+  if (e != hipSuccess) throw Error(-3, std::string("HIP: ") + hipGetErrorString(e));
+  return std::max(nb, 1);
+}
+
 template <int BM, int BN>
 static void gemm_f16_kernel_launch(Context& c, const GemmLaunch& g) {
   hipLaunchKernelGGL((k_gemm_f16x3<BM, BN>), dim3(g.grid), dim3(kLpThreads), 0, c.stream, g.tiles, g.ntiles,
@@ -377,25 +600,62 @@ int gemm_occupancy_f16(int cfg) {
   return std::max(nb, 1);
 }
 
-void issue_gemm_f16(Context& c, const GemmLaunch& g) {
+void issue_gemm_f16_prep(Context& c, const GemmLaunch& g) {
   if (g.ntiles <= 0) return;
+  auto check = [] {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw Error(-3, std::string("HIP: ") + hipGetErrorString(e));
+  };
+  if (g.presplit) {
+    // maxima of every distinct operand block, then its planes
+    launch_fill(c, g.regionmax, std::max(1, g.nregions), 0.0);
+    if (g.nregionmaxwork > 0) {
+      hipLaunchKernelGGL(k_region_absmax, dim3(std::min(g.nregionmaxwork, c.num_cu * 8)), dim3(256), 0, c.stream,
+                         g.regions, g.regionmaxwork, g.nregionmaxwork, g.regionmax);
+      check();
+    }
+    if (g.nsplitwork > 0) {
+      hipLaunchKernelGGL(k_f16_presplit, dim3(std::min(g.nsplitwork, c.num_cu * 8)), dim3(256), 0, c.stream,
+                         g.regions, g.splitwork, g.nsplitwork, g.regionmax);
+      check();
+    }
+    return;
+  }
   // operand maxima of every segment, measured right before the product
   launch_fill(c, g.segmax, std::max(2, 2 * g.nseg), 0.0);
   if (g.nmaxwork > 0) {
     hipLaunchKernelGGL(k_seg_absmax, dim3(std::min(g.nmaxwork, c.num_cu * 8)), dim3(256), 0, c.stream, g.tasks,
                        g.segs, g.segtask, g.maxwork, g.nmaxwork, g.segmax);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) throw Error(-3, std::string("HIP: ") + hipGetErrorString(e));
+    check();
   }
-  switch (g.cfg) {
-    case 0: gemm_f16_kernel_launch<32, 32>(c, g); break;
-    case 1: gemm_f16_kernel_launch<64, 64>(c, g); break;
-    case 2: gemm_f16_kernel_launch<128, 128>(c, g); break;
-    case 3: gemm_f16_kernel_launch<64, 128>(c, g); break;
-    default: gemm_f16_kernel_launch<128, 64>(c, g); break;
+}
+
+void issue_gemm_f16_main(Context& c, const GemmLaunch& g) {
+  if (g.ntiles <= 0) return;
+  if (g.presplit) {
+    switch (g.cfg) {
+      case 0: gemm_f16p_kernel_launch<32, 32>(c, g); break;
+      case 1: gemm_f16p_kernel_launch<64, 64>(c, g); break;
+      case 2: gemm_f16p_kernel_launch<128, 128>(c, g); break;
+      case 3: gemm_f16p_kernel_launch<64, 128>(c, g); break;
+      default: gemm_f16p_kernel_launch<128, 64>(c, g); break;
+    }
+  } else {
+    switch (g.cfg) {
+      case 0: gemm_f16_kernel_launch<32, 32>(c, g); break;
+      case 1: gemm_f16_kernel_launch<64, 64>(c, g); break;
+      case 2: gemm_f16_kernel_launch<128, 128>(c, g); break;
+      case 3: gemm_f16_kernel_launch<64, 128>(c, g); break;
+      default: gemm_f16_kernel_launch<128, 64>(c, g); break;
+    }
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) throw Error(-3, std::string("HIP: ") + hipGetErrorString(e));
+}
+
+void issue_gemm_f16(Context& c, const GemmLaunch& g) {
+  issue_gemm_f16_prep(c, g);
+  issue_gemm_f16_main(c, g);
 }
 
 // Lane-layout probe of v_mfma_f32_16x16x32_bf16 (one wave, in-bounds):

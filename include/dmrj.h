@@ -91,6 +91,12 @@ int dmrj_ctx_set_autotune(dmrj_ctx* ctx, int mode);
  * b mod 8, each XCD has its own L2); 0 = plain descending-work order.  The
  * order never changes result bits. */
 int dmrj_ctx_set_tile_order(dmrj_ctx* ctx, int mode);
+/* FP16x3 grouped GEMMs (DMRJ_PREC_F16X3; no counterpart in the reference):
+ * 1 (default) = every distinct operand block of a launch is scaled and split
+ * into FP16 hi / lo planes once (k_f16_presplit), and the GEMM reads its
+ * MFMA fragments from the planes; 0 = the split is done by every tile on
+ * load.  Both give the same bits. */
+int dmrj_ctx_set_f16_presplit(dmrj_ctx* ctx, int on);
 /* out5 = {cached keys, cache hits, tuned misses, launches not tunable
  *         (beta != 0 or aliasing), total tuning time in ms} */
 int dmrj_ctx_autotune_stats(dmrj_ctx* ctx, double* out5);
@@ -140,8 +146,10 @@ int dmrj_ctx_last_residual(dmrj_ctx* ctx, double* out);
 /* per-kernel HIP-event timing on the context stream (enable = 1 on) */
 int dmrj_ctx_profile(dmrj_ctx* ctx, int enable);
 /* kind 0: FP64 grouped GEMM, 1: MPO mix, 2: block copy, 3: SVD solver,
- * 4: low-precision grouped GEMM.  out4 = [launches, total ms, algorithmic
- * flops, compulsory HBM bytes]; reset = 1 clears the counters. */
+ * 4: BF16x3 grouped GEMM, 5: captured graph, 6: FP16x3 grouped GEMM (on
+ * planes: the GEMM kernel alone), 7: FP16x3 pre-passes (operand maxima and
+ * plane split).  out4 = [launches, total ms, algorithmic flops, compulsory
+ * HBM bytes]; reset = 1 clears the counters. */
 int dmrj_ctx_kernel_stats(dmrj_ctx* ctx, int kind, double* out4, int reset);
 /* accumulated algorithmic flops of H_eff applications and env updates */
 int dmrj_ctx_flops(dmrj_ctx* ctx, double* out2, int reset);

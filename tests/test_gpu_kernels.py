@@ -191,6 +191,33 @@ def test_svd_block_jacobi_split(pkg, ctx, direction, nq):
         assert err <= 1e-11
 
 
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("tA,tB", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("M,N,K", [(16, 16, 32), (33, 70, 17), (130, 129, 200), (1, 300, 65)])
+def test_grouped_gemm_f16x3_planes_same_bits(pkg, ctx, cfg, tA, tB, M, N, K):
+    """FP16x3 on pre-split operand planes (default) and with the split done by
+    every tile on load: the same scaled halves, K steps and MFMA sequence, so
+    the same bits, for every tile shape, transpose and ragged edge."""
+    rng = np.random.default_rng(M + 3 * N + 7 * K + 11 * tA + 13 * tB)
+    A = rng.standard_normal((M, K)) * 1e3
+    B = rng.standard_normal((K, N)) * 1e-4
+    As = np.asfortranarray(A.T if tA else A)
+    Bs = np.asfortranarray(B.T if tB else B)
+    out = []
+    for planes in (1, 0):
+        ctx.set_f16_presplit(planes)
+        try:
+            C = np.zeros((M, N), order="F")
+            ctx.call("dmrj_selftest_gemm", cfg | (2 << 4), M, N, K, tA, tB, pkg._lib.dptr(As.ravel(order="F")),
+                     pkg._lib.dptr(Bs.ravel(order="F")), C.ctypes.data_as(pkg._lib._dp))
+        finally:
+            ctx.set_f16_presplit(1)
+        out.append(C)
+    assert np.array_equal(out[0], out[1])
+    err = np.max(np.abs(out[0] - A @ B) / (np.abs(A) @ np.abs(B)))
+    assert err < 5e-6
+
+
 @pytest.mark.parametrize("cfg", [0, 2, 3])
 @pytest.mark.parametrize("tA,tB", [(0, 0), (1, 1)])
 @pytest.mark.parametrize("M,N,K,scaleA,scaleB", [(33, 70, 17, 1.0, 1.0), (130, 129, 200, 3e-9, 2e6),

@@ -99,3 +99,34 @@ def test_mixed_heff_fp64_env_energy_without_polish(pkg, ctx, precision, tol):
             "rel %.2e" % (abs(h64["energy"] - hmx["energy"]) / abs(h64["energy"])))
     e64, emx = a64.history[-1]["energy"], amx.history[-1]["energy"]
     assert abs(e64 - emx) / abs(e64) <= tol
+
+
+def test_f16x3_planes_sweeps_same_bits(pkg, ctx):
+    """Mixed sweeps (FP16x3 H_eff*psi, FP64 environments, Gram SVD) with the
+    FP16x3 GEMMs on pre-split operand planes and with the on-load split:
+    identical energies, truncation errors and spectra, sweep by sweep (the
+    planes are shared by the segments that use one operand block, whose
+    scaling is the same for all of them)."""
+    L = 24
+    qphys = np.array([[1], [-1]])
+    states = [i % 2 for i in range(L)]
+    mpo = pkg.MPO(pkg.makeMPO(pkg.heisenbergMPO(), 2, L).H, qphys=qphys)
+    say("case FP16x3 planes vs on-load split, U(1) L=24 m=128")
+    runs = []
+    for planes in (1, 0):
+        ctx.set_f16_presplit(planes)
+        ctx.set_svd_backend(5)
+        try:
+            psi = pkg.product_state(states, 2, qphys=qphys, ctx=ctx)
+            params = pkg.algvars(psi)
+            pkg.dmrg(psi, mpo, m=128, sweeps=3, cutoff=1e-12, params=params, precision="f16x3",
+                     correction="bond", env_precision="f64")
+        finally:
+            ctx.set_f16_presplit(1)
+            ctx.set_svd_backend(0)
+        say(" planes", planes, "E %.15f" % params.energy)
+        runs.append(params.history)
+    for a, b in zip(runs[0], runs[1]):
+        assert a["energy"] == b["energy"]
+        assert a["maxtrunc"] == b["maxtrunc"]
+        assert list(a["storeD"]) == list(b["storeD"])
