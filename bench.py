@@ -117,7 +117,8 @@ def main():
                     help="with --compare: first sweep at m (up to this many sweeps) until the energy changes by "
                          "less than 1e-11 relative, so that the variants start from a converged state")
     ap.add_argument("--progress", action="store_true",
-                    help="headline: a progress line every 16 bond steps (runs under a counter-collecting profiler)")
+                    help="a progress line every 16 bond steps of two-site sweeps (runs under a counter-collecting "
+                         "profiler)")
     ap.add_argument("--step-errors", type=int, default=1,
                     help="headline: 1 = one more mixed sweep whose every bond step is also solved in FP64 and in the "
                          "plain split form from the same inputs (per-bond energy deviations); 0 = skip it")
@@ -162,8 +163,8 @@ def main():
     params = pkg.algvars(psi)
 
     def sweeps(mm, n, method="twosite"):
-        pkg.dmrg(psi, mpo, m=mm, sweeps=n, cutoff=args.cutoff, params=params, precision=args.precision,
-                 env_precision=args.env_precision, method=method)
+        run_sweeps(pkg, psi, mpo, params, mm, n, args.cutoff, precision=args.precision,
+                   env_precision=args.env_precision, method=method, progress=args.progress)
 
     t_ramp = time.perf_counter()
     for mr in [int(x) for x in args.ramp.split(",") if x]:
@@ -229,8 +230,8 @@ def main():
     gkind, gname, gpeak = {"f64": ("gemm_f64", "k_gemm_f64 (FP64 grouped GEMM, MFMA 16x16x4)", PEAK_FP64),
                            "bf16x3": ("gemm_lp", "k_gemm_bf16x3 (BF16x3 split grouped GEMM, MFMA 16x16x32)",
                                       PEAK_BF16),
-                           "f16x3": ("gemm_f16", "k_gemm_f16x3 (scaled FP16x3 split grouped GEMM, MFMA "
-                                     "16x16x32)", PEAK_BF16)}[args.precision]
+                           "f16x3": ("gemm_f16", "k_gemm_f16p (scaled FP16x3 grouped GEMM on pre-split "
+                                     "operand planes, MFMA 16x16x32)", PEAK_BF16)}[args.precision]
     g = stats[gkind]
     g_tflops = g["flops"] / (g["ms"] * 1e-3) / 1e12 if g["ms"] > 0 else 0.0
     g_avg_ms = g["ms"] / max(1, g["launches"])
@@ -346,6 +347,31 @@ def make_workload(pkg, ctx, args):
     return psi, mpo, desc, L, m
 
 
+def run_sweeps(pkg, psi, mpo, params, m, n, cutoff, precision="f64", env_precision=None, correction=None,
+               method="twosite", progress=False):
+    """pkg.dmrg(...) with the same settings; with progress, the same sweeps
+    through optmps with a line every 16 bond steps (for runs under a
+    counter-collecting profiler, where one sweep takes minutes)."""
+    if not progress or method != "twosite":
+        pkg.dmrg(psi, mpo, m=m, sweeps=n, cutoff=cutoff, params=params, precision=precision,
+                 env_precision=env_precision, correction=correction, method=method)
+        return
+    count = [0]
+
+    def step(nn, j, i, iL, iR, psi_, mpo_, Lenv, Renv, p):
+        pkg.twostep(nn, j, i, iL, iR, psi_, mpo_, Lenv, Renv, p)
+        count[0] += 1
+        if count[0] % 16 == 0:
+            log("  bond steps done: %d" % count[0])
+
+    params.method = "twosite"
+    params.maxm, params.minm, params.sweeps, params.cutoff = m, 2, n, cutoff
+    params.silent, params.goal, params.precision = True, 0.0, precision
+    params.env_precision = env_precision
+    params.correction = correction or ("none" if precision == "f64" else "sweep")
+    pkg.optmps(psi, mpo, params, stepfct=step)
+
+
 def _free_envs(params):
     for env in (params.Lenv, params.Renv):
         for e in (env.V if env is not None else []):
@@ -406,25 +432,7 @@ def bench_headline(pkg, ctx, args):
     ctx.set_svd_backend(args.mixed_svd)
 
     def sweeps(st, p, mm, n, cfg):
-        if not args.progress:
-            pkg.dmrg(st, mpo, m=mm, sweeps=n, cutoff=args.cutoff, params=p, **cfg)
-            return
-        # the same sweeps with a line every 16 bond steps (for runs under a
-        # counter-collecting profiler, where one sweep takes minutes)
-        count = [0]
-
-        def step(nn, j, i, iL, iR, psi_, mpo_, Lenv, Renv, params):
-            pkg.twostep(nn, j, i, iL, iR, psi_, mpo_, Lenv, Renv, params)
-            count[0] += 1
-            if count[0] % 16 == 0:
-                log("  bond steps done: %d" % count[0])
-
-        p.method = "twosite"
-        p.maxm, p.minm, p.sweeps, p.cutoff = mm, 2, n, args.cutoff
-        p.silent, p.goal, p.precision = True, 0.0, cfg["precision"]
-        p.env_precision = cfg["env_precision"]
-        p.correction = cfg["correction"] or ("none" if cfg["precision"] == "f64" else "sweep")
-        pkg.optmps(st, mpo, p, stepfct=step)
+        run_sweeps(pkg, st, mpo, p, mm, n, args.cutoff, progress=args.progress, **cfg)
 
     t_ramp = time.perf_counter()
     for mr in [int(x) for x in args.ramp.split(",") if x]:

@@ -579,10 +579,15 @@ __global__ void k_krylov_coef(const double* __restrict__ sc, int r, double pf_re
     Q[i + i * k] = 1.0;
   }
   for (int sweep = 0; sweep < 100 && !bad; ++sweep) {
-    double off = 0.0;
-    for (int p = 0; p < k; ++p)
+    // converged when the off-diagonal part is at the rounding level of the
+    // matrix (|off| <= 1e-15 |A|_F, the norm rotations keep); Jacobi converges
+    // quadratically, so a handful of sweeps
+    double off = 0.0, dia = 0.0;
+    for (int p = 0; p < k; ++p) {
+      dia += A[p + p * k] * A[p + p * k];
       for (int q = p + 1; q < k; ++q) off += A[p + q * k] * A[p + q * k];
-    if (off < 1e-300) break;
+    }
+    if (off <= 1e-30 * (dia + 2.0 * off) || off < 1e-300) break;
     for (int p = 0; p < k; ++p)
       for (int q = p + 1; q < k; ++q) {
         const double apq = A[p + q * k];
