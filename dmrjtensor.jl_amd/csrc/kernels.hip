@@ -543,10 +543,12 @@ __global__ void k_combine(double* out, const double* V, int64_t ld, int64_t n, c
 }
 
 // Coefficients of the Krylov exponential / Ritz vector on the device (one
-// thread; the Krylov dimension is at most 64, typically 2..12):
+// wave; the Krylov dimension is at most 64, typically 2..12):
 //   k = the Krylov dimension used (first beta_j <= 1e-10 ends the space),
-//   T = Q diag(lam) Q^T by cyclic Jacobi (FP64, the tridiagonal of alpha in
-//   sc[0..], beta in sc[64..]),
+//   T = Q diag(lam) Q^T by parallel-order Jacobi (FP64, the tridiagonal of
+//   alpha in sc[0..], beta in sc[64..]): each step rotates k/2 disjoint index
+//   pairs of a round-robin schedule at once (columns, then rows), k-1 steps
+//   per sweep, until the off-diagonal part is at the matrix's rounding level,
 //   c = Q exp(pf lam) Q^T e1 with the complex prefactor pf = (pf_re, pf_im),
 //   or (ritz) the eigenvector of the lowest lam signed to c_1 >= 0.
 // coef: real Krylov vectors: c_re[0..r), c_im[r..2r); complex Krylov vectors
@@ -554,75 +556,142 @@ __global__ void k_combine(double* out, const double* V, int64_t ld, int64_t n, c
 // of Re out in [0..2r) and of Im out in [2r..4r).  Entries j >= k are zero.
 // outs: {lowest Ritz value, k, 1 if an alpha/beta was non-finite}.
 // Dynamic LDS: 2 r^2 doubles (the matrix and its eigenvectors).
-__global__ void k_krylov_coef(const double* __restrict__ sc, int r, double pf_re, double pf_im, int ritz, int zv,
-                              double* __restrict__ coef, double* __restrict__ outs) {
-  extern __shared__ double kc_lds[];
-  if (threadIdx.x != 0) return;
-  int k = r;
-  int bad = 0;
-  for (int j = 0; j + 1 < r; ++j) {
-    if (!isfinite(sc[64 + j]) || !isfinite(sc[j])) bad = 1;
-    if (sc[64 + j] <= 1e-10) {
-      k = j + 1;
-      break;
-    }
+__device__ inline void kc_pair(int n, int s, int i, int& p, int& q) {  // round robin of n (even) players
+  if (i == 0) {
+    p = s;
+    q = n - 1;
+  } else {
+    p = (s + i) % (n - 1);
+    q = (s - i + (n - 1)) % (n - 1);
   }
+  if (p > q) {
+    const int t = p;
+    p = q;
+    q = t;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_krylov_coef(const double* __restrict__ sc, int r, double pf_re,
+                                                     double pf_im, int ritz, int zv, double* __restrict__ coef,
+                                                     double* __restrict__ outs) {
+  extern __shared__ double kc_lds[];
+  __shared__ int sk, sbad, simin;
+  __shared__ double scs[32], ssn[32];
+  __shared__ int sp[32], sq[32];
+  const int t = threadIdx.x;
+  if (t == 0) {
+    int k = r, bad = 0;
+    for (int j = 0; j + 1 < r; ++j) {
+      if (!isfinite(sc[64 + j]) || !isfinite(sc[j])) bad = 1;
+      if (sc[64 + j] <= 1e-10) {
+        k = j + 1;
+        break;
+      }
+    }
+    sk = k;
+    sbad = bad;
+  }
+  __syncthreads();
+  const int k = sk, bad = sbad;
   double* A = kc_lds;
   double* Q = kc_lds + (int64_t)k * k;
-  for (int e = 0; e < k * k; ++e) {
+  for (int e = t; e < k * k; e += 64) {
     A[e] = 0.0;
     Q[e] = 0.0;
   }
-  for (int i = 0; i < k; ++i) {
+  __syncthreads();
+  for (int i = t; i < k; i += 64) {
     A[i + i * k] = sc[i];
     if (i + 1 < k) A[i + (i + 1) * k] = A[(i + 1) + i * k] = sc[64 + i];
     Q[i + i * k] = 1.0;
   }
-  for (int sweep = 0; sweep < 100 && !bad; ++sweep) {
-    // converged when the off-diagonal part is at the rounding level of the
-    // matrix (|off| <= 1e-15 |A|_F, the norm rotations keep); Jacobi converges
-    // quadratically, so a handful of sweeps
-    double off = 0.0, dia = 0.0;
-    for (int p = 0; p < k; ++p) {
-      dia += A[p + p * k] * A[p + p * k];
-      for (int q = p + 1; q < k; ++q) off += A[p + q * k] * A[p + q * k];
+  __syncthreads();
+  const int K = (k + 1) & ~1;  // even number of players; player k of an odd k is idle
+  const int npair = K / 2;
+  for (int sweep = 0; sweep < 100 && !bad && k > 1; ++sweep) {
+    double off = 0.0, dia = 0.0;  // off: both triangles
+    for (int e = t; e < k * k; e += 64) {
+      const double v = A[e];
+      if (e % k == e / k)
+        dia += v * v;
+      else
+        off += v * v;
     }
-    if (off <= 1e-30 * (dia + 2.0 * off) || off < 1e-300) break;
-    for (int p = 0; p < k; ++p)
-      for (int q = p + 1; q < k; ++q) {
-        const double apq = A[p + q * k];
-        if (apq == 0.0) continue;
-        const double app = A[p + p * k], aqq = A[q + q * k];
-        const double theta = (aqq - app) / (2.0 * apq);
-        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-        const double cs = 1.0 / sqrt(t * t + 1.0), sn = t * cs;
-        for (int rr = 0; rr < k; ++rr) {  // columns p, q
-          const double x = A[rr + p * k], y = A[rr + q * k];
-          A[rr + p * k] = cs * x - sn * y;
-          A[rr + q * k] = sn * x + cs * y;
+    for (int o = 32; o > 0; o >>= 1) {
+      off += __shfl_xor(off, o, 64);
+      dia += __shfl_xor(dia, o, 64);
+    }
+    // lane 0's sums decide for all lanes (the butterfly may round differently
+    // per lane), so the loop and its barriers stay uniform
+    off = __shfl(off, 0, 64);
+    dia = __shfl(dia, 0, 64);
+    if (off <= 2e-30 * (dia + off) || off < 1e-300) break;
+    for (int st = 0; st < K - 1; ++st) {
+      if (t < npair) {
+        int p, q;
+        kc_pair(K, st, t, p, q);
+        double cs = 1.0, sn = 0.0;
+        if (q < k) {
+          const double apq = A[p + q * k];
+          if (apq != 0.0) {
+            const double app = A[p + p * k], aqq = A[q + q * k];
+            const double theta = (aqq - app) / (2.0 * apq);
+            const double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+            cs = 1.0 / sqrt(tt * tt + 1.0);
+            sn = tt * cs;
+          }
         }
-        for (int rr = 0; rr < k; ++rr) {  // rows p, q
-          const double x = A[p + rr * k], y = A[q + rr * k];
-          A[p + rr * k] = cs * x - sn * y;
-          A[q + rr * k] = sn * x + cs * y;
-        }
-        for (int rr = 0; rr < k; ++rr) {
-          const double x = Q[rr + p * k], y = Q[rr + q * k];
-          Q[rr + p * k] = cs * x - sn * y;
-          Q[rr + q * k] = sn * x + cs * y;
-        }
+        sp[t] = p;
+        sq[t] = q;
+        scs[t] = cs;
+        ssn[t] = sn;
       }
-  }
-  double emin = A[0];
-  int imin = 0;
-  for (int i = 1; i < k; ++i)
-    if (A[i + i * k] < emin) {
-      emin = A[i + i * k];
-      imin = i;
+      __syncthreads();
+      for (int e = t; e < npair * k; e += 64) {  // columns p, q of A and of Q
+        const int pi = e / k, rr = e % k;
+        const int p = sp[pi], q = sq[pi];
+        if (q >= k) continue;
+        const double cs = scs[pi], sn = ssn[pi];
+        double x = A[rr + p * k], y = A[rr + q * k];
+        A[rr + p * k] = cs * x - sn * y;
+        A[rr + q * k] = sn * x + cs * y;
+        x = Q[rr + p * k];
+        y = Q[rr + q * k];
+        Q[rr + p * k] = cs * x - sn * y;
+        Q[rr + q * k] = sn * x + cs * y;
+      }
+      __syncthreads();
+      for (int e = t; e < npair * k; e += 64) {  // rows p, q of A
+        const int pi = e / k, rr = e % k;
+        const int p = sp[pi], q = sq[pi];
+        if (q >= k) continue;
+        const double cs = scs[pi], sn = ssn[pi];
+        const double x = A[p + rr * k], y = A[q + rr * k];
+        A[p + rr * k] = cs * x - sn * y;
+        A[q + rr * k] = sn * x + cs * y;
+      }
+      __syncthreads();
     }
+  }
+  if (t == 0) {
+    double emin = A[0];
+    int imin = 0;
+    for (int i = 1; i < k; ++i)
+      if (A[i + i * k] < emin) {
+        emin = A[i + i * k];
+        imin = i;
+      }
+    simin = imin;
+    outs[0] = emin;
+    outs[1] = (double)k;
+    outs[2] = bad ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  const int imin = simin;
   const int nc = zv ? 4 * r : 2 * r;
-  for (int e = 0; e < nc; ++e) coef[e] = 0.0;
-  for (int jj = 0; jj < k; ++jj) {
+  for (int e = t; e < nc; e += 64) coef[e] = 0.0;
+  __syncthreads();
+  for (int jj = t; jj < k; jj += 64) {
     double cre = 0.0, cim = 0.0;
     if (ritz) {
       const double sg = Q[0 + imin * k] >= 0.0 ? 1.0 : -1.0;
@@ -645,9 +714,6 @@ __global__ void k_krylov_coef(const double* __restrict__ sc, int r, double pf_re
       coef[2 * r + 2 * jj + 1] = cre;
     }
   }
-  outs[0] = emin;
-  outs[1] = (double)k;
-  outs[2] = bad ? 1.0 : 0.0;
 }
 
 // ---------------------------------------------------------------------------
