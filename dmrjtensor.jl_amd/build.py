@@ -8,7 +8,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRC = [os.path.join(HERE, "csrc", f) for f in ("api.hip", "kernels.hip", "svd_small.hip", "kernels_lp.hip", "autotune.hip", "complex.hip", "svd_jacobi.hip")]
+SRC = [os.path.join(HERE, "csrc", f) for f in ("api.hip", "kernels.hip", "svd_small.hip", "kernels_lp.hip", "autotune.hip", "complex.hip", "svd_jacobi.hip", "svd.hip")]
 OUT = os.path.join(HERE, "lib", "libdmrjhip.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 

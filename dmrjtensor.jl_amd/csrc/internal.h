@@ -25,6 +25,8 @@
 #include <string>
 #include <vector>
 
+struct dmrj_tensor;  // the C ABI's tensor handle (a dmrj::Tensor)
+
 namespace dmrj {
 
 constexpr int kMaxQ = 2;
@@ -353,6 +355,19 @@ struct CompleteJob {
   double* seld = nullptr;  // 32
   int* cand = nullptr;     // 32
 };
+// A new tensor handle of the given kind (0 site, 1 left env, 2 right env) and
+// its device storage for the current block table (api.hip).
+::dmrj_tensor* new_tensor(Context& c, int kind);
+void alloc_data(Context& c, Tensor& t);
+// Truncated two-site SVD (svd.hip): left / right tensors of the kept
+// singular vectors (the centre's weights on the side dir points to).
+struct SvdResult {
+  ::dmrj_tensor* L = nullptr;
+  ::dmrj_tensor* R = nullptr;
+  double truncerr = 0.0;
+  std::vector<double> svals;  // kept, descending
+};
+SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, double cutoff, int dir);
 // Orthonormal completion (svd_jacobi.hip); allocates and releases its scratch.
 void complete_basis(Context& c, std::vector<CompleteJob>& jobs);
 // Re-orthonormalisation of the kept vectors c0 .. keep-1 against the vectors
@@ -521,3 +536,8 @@ enum Slot {
 };
 
 }  // namespace dmrj
+
+// The C ABI's opaque handles are the library's own objects.
+struct dmrj_ctx : dmrj::Context {};
+struct dmrj_tensor : dmrj::Tensor {};
+struct dmrj_mpo : dmrj::MpoSite {};
