@@ -404,20 +404,37 @@ GEMM_KINDS = {"f64": ("gemm_f64", "k_gemm_f64", "FP64 grouped GEMM, MFMA 16x16x4
                         "(profiled sweep f16_prep_s)", PEAK_BF16)}
 
 
+def _bound(flops, nbytes, ms, peak_tflops):
+    """Roofline placement of a kernel from its algorithmic flops and bytes
+    per launch: below the ridge (peak flops / HBM bandwidth) it is HBM-bound
+    and measured in GB/s against the HBM peak, above it against the matrix
+    peak in TFLOP/s.  Returns (bound, achieved, peak, unit, frac)."""
+    ridge = peak_tflops * 1e12 / (PEAK_HBM * 1e9)
+    if nbytes > 0 and flops / nbytes < ridge:
+        ach = nbytes / (ms * 1e-3) / 1e9
+        return "hbm", round(ach, 1), PEAK_HBM, "GB/s", round(ach / PEAK_HBM, 5)
+    ach = flops / (ms * 1e-3) / 1e12
+    return "mfma", round(ach, 3), peak_tflops, "TFLOP/s", round(ach / peak_tflops, 5)
+
+
 def _roofline(stats, prec, model, L, m, sweep):
     kind, kname, desc, peak = GEMM_KINDS[prec]
     g = stats[kind]
     if g["launches"] == 0 or g["ms"] <= 0:
         return None
-    ach = g["flops"] / (g["ms"] * 1e-3) / 1e12
+    n = g["launches"]
+    bound, ach, pk, unit, frac = _bound(g["flops"] / n, g["bytes"] / n, g["ms"] / n, peak)
     traffic, note = _traffic(kname, model, L, m, prec, sweep)
-    return {"bound": "mfma", "kernel": "%s (%s)" % (kname, desc), "achieved": round(ach, 3), "peak": peak,
-            "unit": "TFLOP/s", "frac": round(ach / peak, 5), "traffic": traffic,
-            "flops_basis": "algorithmic FP64-equivalent flops per launch (split forms execute 3 MFMA products "
-                           "each; the peak is the dense BF16/FP16 matrix peak for them)",
-            "traffic_note": note, "avg_launch_ms": round(g["ms"] / g["launches"], 5), "launches": g["launches"],
-            "flops_per_launch": g["flops"] / g["launches"],
-            "compulsory_bytes_per_launch": g["bytes"] / g["launches"]}
+    return {"bound": bound, "kernel": "%s (%s)" % (kname, desc), "achieved": ach, "peak": pk,
+            "unit": unit, "frac": frac, "traffic": traffic,
+            "achieved_tflops": round(g["flops"] / (g["ms"] * 1e-3) / 1e12, 3), "matrix_peak_tflops": peak,
+            "basis": "algorithmic FP64-equivalent flops and bytes per launch (the FP64 operands and result of "
+                     "the contraction; split forms execute 3 MFMA products per FP64-equivalent product, against "
+                     "the dense FP16/BF16 matrix peak); bound = hbm when flops/bytes is below the ridge "
+                     "peak / 8 TB/s",
+            "traffic_note": note, "avg_launch_ms": round(g["ms"] / n, 5), "launches": n,
+            "flops_per_launch": g["flops"] / n,
+            "compulsory_bytes_per_launch": g["bytes"] / n}
 
 
 def bench_headline(pkg, ctx, args):

@@ -432,10 +432,12 @@ __global__ __launch_bounds__(256) void k_f16_presplit(const F16Region* __restric
   }
 }
 
-// FP16x3 grouped GEMM on operand planes: the same tiles, K steps and MFMA
-// sequence as k_gemm_f16x3 (so the same bits), each lane reading its 16-byte
-// fragments straight from the planes; the next K step's fragments are loaded
-// before the current step's products.
+// FP16x3 grouped GEMM on operand planes: the same tiles, K steps, LDS layout
+// and MFMA sequence as k_gemm_f16x3 (so the same bits).  Each K step the
+// work-group copies the 16-byte chunks of its A and B plane rows into LDS
+// (one copy per element and work-group instead of a load per wave that uses
+// it), the next step's chunks are loaded into registers while the current
+// step's products run.
 template <int BM, int BN>
 __global__ __launch_bounds__(kLpThreads) void k_gemm_f16p(const GemmTile* __restrict__ tiles, int ntiles,
                                                            const GemmTask* __restrict__ tasks,
@@ -443,11 +445,18 @@ __global__ __launch_bounds__(kLpThreads) void k_gemm_f16p(const GemmTile* __rest
                                                            const F16Seg* __restrict__ fsegs,
                                                            const double* __restrict__ regionmax) {
   constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
+  constexpr int ACH = (BM * 4 + kLpThreads - 1) / kLpThreads;  // 16-byte chunks per thread and plane
+  constexpr int BCH = (BN * 4 + kLpThreads - 1) / kLpThreads;
+  __shared__ __attribute__((aligned(16))) _Float16 sAh[2][BM * kLpPitch];
+  __shared__ __attribute__((aligned(16))) _Float16 sAl[2][BM * kLpPitch];
+  __shared__ __attribute__((aligned(16))) _Float16 sBh[2][BN * kLpPitch];
+  __shared__ __attribute__((aligned(16))) _Float16 sBl[2][BN * kLpPitch];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
   const int lr = lane & 15, lk = lane >> 4;
   const half8_t zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
+  int buf = 0;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const GemmTile tl = tiles[t];
     const GemmTask tk = tasks[tl.task];
@@ -463,37 +472,49 @@ __global__ __launch_bounds__(kLpThreads) void k_gemm_f16p(const GemmTile* __rest
       if (K <= 0) continue;
       const F16Seg fs = fsegs[si];
       const int ea = f16_scale_exp(regionmax[fs.ra]), eb = f16_scale_exp(regionmax[fs.rb]);
-      int64_t aoff[FM], boff[FN];
-      bool aok[FM], bok[FN];
-      // rows / columns beyond the task read row 0 of the plane (always
-      // present) and are replaced by zeros, so every load stays in bounds
+      // chunk c of a plane tile: row c / 4, halves 8 (c % 4) .. +7 of the K step;
+      // rows beyond the task read row 0 (always present) and become zeros
+      half8_t rah[ACH], ral[ACH], rbh[BCH], rbl[BCH];
+      auto load = [&](int k0) {
 #pragma unroll
-      for (int i = 0; i < FM; ++i) {
-        const int row = m0 + wm * WM + i * 16 + lr;
-        aok[i] = row < tk.M;
-        aoff[i] = (int64_t)(aok[i] ? row : 0) * fs.kpad + lk * 8;
-      }
-#pragma unroll
-      for (int j = 0; j < FN; ++j) {
-        const int col = n0 + wn * WN + j * 16 + lr;
-        bok[j] = col < tk.N;
-        boff[j] = (int64_t)(bok[j] ? col : 0) * fs.kpad + lk * 8;
-      }
-      half8_t ah[FM], al[FM], bh[FN], bl[FN];
-      auto fetch = [&](int k0) {
-#pragma unroll
-        for (int i = 0; i < FM; ++i) {
-          const half8_t h = *reinterpret_cast<const half8_t*>(fs.ah + aoff[i] + k0);
-          const half8_t l = *reinterpret_cast<const half8_t*>(fs.al + aoff[i] + k0);
-          ah[i] = aok[i] ? h : zero8;
-          al[i] = aok[i] ? l : zero8;
+        for (int q = 0; q < ACH; ++q) {
+          const int c = tid + q * kLpThreads;
+          const int row = m0 + c / 4;
+          const bool ok = c < BM * 4 && row < tk.M;
+          const int64_t o = (int64_t)(ok ? row : 0) * fs.kpad + k0 + (c % 4) * 8;
+          const half8_t h = *reinterpret_cast<const half8_t*>(fs.ah + o);
+          const half8_t l = *reinterpret_cast<const half8_t*>(fs.al + o);
+          rah[q] = ok ? h : zero8;
+          ral[q] = ok ? l : zero8;
         }
 #pragma unroll
-        for (int j = 0; j < FN; ++j) {
-          const half8_t h = *reinterpret_cast<const half8_t*>(fs.bh + boff[j] + k0);
-          const half8_t l = *reinterpret_cast<const half8_t*>(fs.bl + boff[j] + k0);
-          bh[j] = bok[j] ? h : zero8;
-          bl[j] = bok[j] ? l : zero8;
+        for (int q = 0; q < BCH; ++q) {
+          const int c = tid + q * kLpThreads;
+          const int col = n0 + c / 4;
+          const bool ok = c < BN * 4 && col < tk.N;
+          const int64_t o = (int64_t)(ok ? col : 0) * fs.kpad + k0 + (c % 4) * 8;
+          const half8_t h = *reinterpret_cast<const half8_t*>(fs.bh + o);
+          const half8_t l = *reinterpret_cast<const half8_t*>(fs.bl + o);
+          rbh[q] = ok ? h : zero8;
+          rbl[q] = ok ? l : zero8;
+        }
+      };
+      auto store = [&](int b) {
+#pragma unroll
+        for (int q = 0; q < ACH; ++q) {
+          const int c = tid + q * kLpThreads;
+          if (c < BM * 4) {
+            *reinterpret_cast<half8_t*>(&sAh[b][(c / 4) * kLpPitch + (c % 4) * 8]) = rah[q];
+            *reinterpret_cast<half8_t*>(&sAl[b][(c / 4) * kLpPitch + (c % 4) * 8]) = ral[q];
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < BCH; ++q) {
+          const int c = tid + q * kLpThreads;
+          if (c < BN * 4) {
+            *reinterpret_cast<half8_t*>(&sBh[b][(c / 4) * kLpPitch + (c % 4) * 8]) = rbh[q];
+            *reinterpret_cast<half8_t*>(&sBl[b][(c / 4) * kLpPitch + (c % 4) * 8]) = rbl[q];
+          }
         }
       };
       f32x4_t acc[FM][FN];
@@ -501,28 +522,33 @@ __global__ __launch_bounds__(kLpThreads) void k_gemm_f16p(const GemmTile* __rest
       for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-      fetch(0);
+      load(0);
       for (int k0 = 0; k0 < K; k0 += kLpBK) {
-        half8_t cah[FM], cal[FM], cbh[FN], cbl[FN];
+        store(buf);
+        __syncthreads();
+        if (k0 + kLpBK < K) load(k0 + kLpBK);
+        half8_t ah[FM], al[FM], bh[FN], bl[FN];
 #pragma unroll
         for (int i = 0; i < FM; ++i) {
-          cah[i] = ah[i];
-          cal[i] = al[i];
+          const int row = wm * WM + i * 16 + lr;
+          ah[i] = *reinterpret_cast<const half8_t*>(&sAh[buf][row * kLpPitch + lk * 8]);
+          al[i] = *reinterpret_cast<const half8_t*>(&sAl[buf][row * kLpPitch + lk * 8]);
         }
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
-          cbh[j] = bh[j];
-          cbl[j] = bl[j];
+          const int col = wn * WN + j * 16 + lr;
+          bh[j] = *reinterpret_cast<const half8_t*>(&sBh[buf][col * kLpPitch + lk * 8]);
+          bl[j] = *reinterpret_cast<const half8_t*>(&sBl[buf][col * kLpPitch + lk * 8]);
         }
-        if (k0 + kLpBK < K) fetch(k0 + kLpBK);
 #pragma unroll
         for (int i = 0; i < FM; ++i)
 #pragma unroll
           for (int j = 0; j < FN; ++j) {
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(cal[i], cbh[j], acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(cah[i], cbl[j], acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(cah[i], cbh[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
           }
+        buf ^= 1;
       }
       const float unscale = ldexpf(1.0f, -(ea + eb));  // exact power of two
 #pragma unroll
