@@ -1691,6 +1691,7 @@ int dmrj_ctx_destroy(dmrj_ctx* c) {
   if (c->scal) (void)hipFree(c->scal);
   if (c->partial) (void)hipFree(c->partial);
   if (c->hscal) (void)hipHostFree(c->hscal);
+  if (c->hsig) (void)hipHostFree(c->hsig);
   if (c->stage.base) (void)hipHostFree(c->stage.base);
   c->alloc.trim();
   if (c->stream) (void)hipStreamDestroy(c->stream);

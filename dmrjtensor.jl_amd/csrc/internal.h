@@ -200,6 +200,24 @@ struct Context {
   std::string err;
   double* scal = nullptr;       // device scalars for Lanczos
   double* hscal = nullptr;      // pinned mirror
+  double* hsig = nullptr;       // pinned buffer for singular values coming to the host
+  int64_t hsig_cap = 0;
+  // the pinned singular-value buffer with room for n doubles (grown when
+  // needed; only after a synchronisation of the stream that used it)
+  double* host_sigma(int64_t n) {
+    if (n > hsig_cap) {
+      if (hsig) {
+        if (hipStreamSynchronize(stream) != hipSuccess || hipHostFree(hsig) != hipSuccess)
+          throw Error(-3, "HIP: pinned singular-value buffer");
+        hsig = nullptr;
+      }
+      const int64_t cap = std::max<int64_t>(n, 2 * hsig_cap);
+      if (hipHostMalloc((void**)&hsig, (size_t)cap * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        throw Error(-3, "HIP: pinned singular-value buffer");
+      hsig_cap = cap;
+    }
+    return hsig;
+  }
   double* partial = nullptr;    // reduction partials
   double* noalloc = nullptr;    // device address of dmrj_dots_noalloc's scratch on this device
   int num_cu = 256;

@@ -127,17 +127,26 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
   std::vector<int> order;
   for (auto& kv : qidx) order.push_back(kv.second);
   std::vector<CopyOp> gather;
+  // the singular values of all sectors in one array, in sector order, so that
+  // they come to the host in one copy
   int64_t total_s = 0;
   for (int si : order) {
     Sec& s = secs[si];
     if (s.R == 0 || s.C == 0) continue;
     s.kmin = (int)std::min(s.R, s.C);
+    total_s += s.kmin;
+  }
+  double* sall = (double*)c.alloc.alloc(std::max<int64_t>(total_s, 1) * sizeof(double));
+  int64_t soff_all = 0;
+  for (int si : order) {
+    Sec& s = secs[si];
+    if (s.R == 0 || s.C == 0) continue;
     s.M = (double*)c.alloc.alloc(s.R * s.C * zf * sizeof(double));
     s.U = (double*)c.alloc.alloc(s.R * s.kmin * zf * sizeof(double));
     s.Vt = (double*)c.alloc.alloc((int64_t)s.kmin * s.C * zf * sizeof(double));
-    s.s = (double*)c.alloc.alloc(s.kmin * sizeof(double));
+    s.s = sall + soff_all;
+    soff_all += s.kmin;
     s.E = (double*)c.alloc.alloc(std::max(1, s.kmin) * sizeof(double));
-    total_s += s.kmin;
     for (size_t r = 0; r < s.rows.size(); ++r)
       for (size_t q = 0; q < s.cols.size(); ++q) {
         const int a = s.rows[r].first, j1 = s.rows[r].second;
@@ -167,9 +176,9 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
         c.alloc.release(s.M);
         c.alloc.release(s.U);
         c.alloc.release(s.Vt);
-        c.alloc.release(s.s);
         c.alloc.release(s.E);
       }
+      c.alloc.release(sall);
       throw Error(DMRJ_ERR_NUMERIC, "two-site tensor holds NaN/Inf; SVD not attempted");
     }
   }
@@ -638,21 +647,12 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
   }
   if (small_on_side) HIP_OK(hipStreamWaitEvent(c.stream, c.side_out, 0));  // small sectors done
   c.prof_end(psvd, KIND_SVD, svd_flops, 0.0);
-  // singular values to host
+  // singular values to host: one copy into the context's pinned buffer (the
+  // entries of one-factor sectors are filled from their own results below)
   std::vector<double> hs(std::max<int64_t>(total_s, 1));
-  {
-    int64_t o = 0;
-    for (int si : order) {
-      Sec& s = secs[si];
-      if (s.kmin == 0) continue;
-      if (s.bj && !s.qr) {
-        // column norms already on the host (block-Jacobi loop)
-      } else {
-        HIP_OK(hipMemcpyAsync(hs.data() + o, s.s, s.kmin * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-      }
-      o += s.kmin;
-    }
-  }
+  double* hsig = c.host_sigma(std::max<int64_t>(total_s, 1));
+  if (total_s > 0)
+    HIP_OK(hipMemcpyAsync(hsig, sall, (size_t)total_s * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   std::vector<double> hgram((size_t)std::max<int64_t>(1, gramWtot));
   std::vector<rocblas_int> hginfo(std::max(1, ngram));
   if (ngram) {
@@ -669,6 +669,7 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
   if (nqr)
     HIP_OK(hipMemcpyAsync(hqinfo.data(), qinfo, nqr * sizeof(rocblas_int), hipMemcpyDeviceToHost, c.stream));
   HIP_OK(hipStreamSynchronize(c.stream));
+  if (total_s > 0) std::memcpy(hs.data(), hsig, (size_t)total_s * sizeof(double));
   c.alloc.release(aug);
   c.alloc.release(augW);
   c.alloc.release(augE);
@@ -916,10 +917,10 @@ SvdResult svd_trunc(Context& c, const Tensor& AA, int64_t m, int64_t minm, doubl
     c.alloc.release(s.M);
     c.alloc.release(s.U);
     c.alloc.release(s.Vt);
-    c.alloc.release(s.s);
     c.alloc.release(s.E);
     c.alloc.release(s.X);
   }
+  c.alloc.release(sall);
   c.alloc.release(bjflag);
   c.alloc.release(bjnorm);
   c.alloc.release(gramZ);
