@@ -38,6 +38,14 @@ constexpr int kBK = 16;
 // each touch 16 distinct bank pairs.
 constexpr int kPitch = kBK + 1;
 
+// Load through a global-address-space pointer: a global_load counts only
+// against vmcnt, so the prefetch of the next K step stays in flight across the
+// LDS reads and MFMAs of the current one (a generic flat_load would also hold
+// every lgkmcnt wait of those LDS reads).
+__device__ __forceinline__ double ld_global(const double* p) {
+  return *(const __attribute__((address_space(1))) double*)p;
+}
+
 template <int BM, int BN>
 __global__ __launch_bounds__(kThreads) void k_gemm_f64(const GemmTile* __restrict__ tiles,
                                                        int ntiles,
@@ -78,8 +86,8 @@ __global__ __launch_bounds__(kThreads) void k_gemm_f64(const GemmTile* __restric
           const int gm = m0 + m, gk = k0 + k;
           double v = 0.0;
           if (gm < tk.M && gk < K)
-            v = tA ? sg.A[(int64_t)gk + (int64_t)gm * sg.lda] : sg.A[(int64_t)gm + (int64_t)gk * sg.lda];
-          ra[r] = v * sg.alpha;
+            v = ld_global(tA ? sg.A + ((int64_t)gk + (int64_t)gm * sg.lda) : sg.A + ((int64_t)gm + (int64_t)gk * sg.lda));
+          ra[r] = v;
         }
 #pragma unroll
         for (int r = 0; r < BE; ++r) {
@@ -89,17 +97,19 @@ __global__ __launch_bounds__(kThreads) void k_gemm_f64(const GemmTile* __restric
           const int gn = n0 + n, gk = k0 + k;
           double v = 0.0;
           if (gn < tk.N && gk < K)
-            v = tB ? sg.B[(int64_t)gn + (int64_t)gk * sg.ldb] : sg.B[(int64_t)gk + (int64_t)gn * sg.ldb];
+            v = ld_global(tB ? sg.B + ((int64_t)gn + (int64_t)gk * sg.ldb) : sg.B + ((int64_t)gk + (int64_t)gn * sg.ldb));
           rb[r] = v;
         }
       };
+      // alpha is applied when the prefetched values are written to LDS, after
+      // the products of the current step, so nothing waits on the loads early
       auto store = [&](int b) {
 #pragma unroll
         for (int r = 0; r < AE; ++r) {
           const int idx = tid + r * kThreads;
           int m, k;
           if (!tA) { m = idx % BM; k = idx / BM; } else { k = idx % kBK; m = idx / kBK; }
-          sA[b][m * kPitch + k] = ra[r];
+          sA[b][m * kPitch + k] = ra[r] * sg.alpha;
         }
 #pragma unroll
         for (int r = 0; r < BE; ++r) {

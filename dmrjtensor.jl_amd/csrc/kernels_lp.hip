@@ -473,7 +473,11 @@ __global__ __launch_bounds__(kLpThreads) void k_gemm_f16p(const GemmTile* __rest
       const F16Seg fs = fsegs[si];
       const int ea = f16_scale_exp(regionmax[fs.ra]), eb = f16_scale_exp(regionmax[fs.rb]);
       // chunk c of a plane tile: row c / 4, halves 8 (c % 4) .. +7 of the K step;
-      // rows beyond the task read row 0 (always present) and become zeros
+      // rows beyond the task read row 0 (always present) and are written to
+      // LDS as zeros.  The loads go through global-address-space pointers and
+      // the zero mask is applied at the LDS store, so the prefetch of the next
+      // K step stays in flight across the current step's products.
+      typedef __attribute__((address_space(1))) const half8_t ghalf8_t;
       half8_t rah[ACH], ral[ACH], rbh[BCH], rbl[BCH];
       auto load = [&](int k0) {
 #pragma unroll
@@ -482,10 +486,8 @@ __global__ __launch_bounds__(kLpThreads) void k_gemm_f16p(const GemmTile* __rest
           const int row = m0 + c / 4;
           const bool ok = c < BM * 4 && row < tk.M;
           const int64_t o = (int64_t)(ok ? row : 0) * fs.kpad + k0 + (c % 4) * 8;
-          const half8_t h = *reinterpret_cast<const half8_t*>(fs.ah + o);
-          const half8_t l = *reinterpret_cast<const half8_t*>(fs.al + o);
-          rah[q] = ok ? h : zero8;
-          ral[q] = ok ? l : zero8;
+          rah[q] = *(ghalf8_t*)(fs.ah + o);
+          ral[q] = *(ghalf8_t*)(fs.al + o);
         }
 #pragma unroll
         for (int q = 0; q < BCH; ++q) {
@@ -493,10 +495,8 @@ __global__ __launch_bounds__(kLpThreads) void k_gemm_f16p(const GemmTile* __rest
           const int col = n0 + c / 4;
           const bool ok = c < BN * 4 && col < tk.N;
           const int64_t o = (int64_t)(ok ? col : 0) * fs.kpad + k0 + (c % 4) * 8;
-          const half8_t h = *reinterpret_cast<const half8_t*>(fs.bh + o);
-          const half8_t l = *reinterpret_cast<const half8_t*>(fs.bl + o);
-          rbh[q] = ok ? h : zero8;
-          rbl[q] = ok ? l : zero8;
+          rbh[q] = *(ghalf8_t*)(fs.bh + o);
+          rbl[q] = *(ghalf8_t*)(fs.bl + o);
         }
       };
       auto store = [&](int b) {
@@ -504,16 +504,18 @@ __global__ __launch_bounds__(kLpThreads) void k_gemm_f16p(const GemmTile* __rest
         for (int q = 0; q < ACH; ++q) {
           const int c = tid + q * kLpThreads;
           if (c < BM * 4) {
-            *reinterpret_cast<half8_t*>(&sAh[b][(c / 4) * kLpPitch + (c % 4) * 8]) = rah[q];
-            *reinterpret_cast<half8_t*>(&sAl[b][(c / 4) * kLpPitch + (c % 4) * 8]) = ral[q];
+            const bool ok = m0 + c / 4 < tk.M;
+            *reinterpret_cast<half8_t*>(&sAh[b][(c / 4) * kLpPitch + (c % 4) * 8]) = ok ? rah[q] : zero8;
+            *reinterpret_cast<half8_t*>(&sAl[b][(c / 4) * kLpPitch + (c % 4) * 8]) = ok ? ral[q] : zero8;
           }
         }
 #pragma unroll
         for (int q = 0; q < BCH; ++q) {
           const int c = tid + q * kLpThreads;
           if (c < BN * 4) {
-            *reinterpret_cast<half8_t*>(&sBh[b][(c / 4) * kLpPitch + (c % 4) * 8]) = rbh[q];
-            *reinterpret_cast<half8_t*>(&sBl[b][(c / 4) * kLpPitch + (c % 4) * 8]) = rbl[q];
+            const bool ok = n0 + c / 4 < tk.N;
+            *reinterpret_cast<half8_t*>(&sBh[b][(c / 4) * kLpPitch + (c % 4) * 8]) = ok ? rbh[q] : zero8;
+            *reinterpret_cast<half8_t*>(&sBl[b][(c / 4) * kLpPitch + (c % 4) * 8]) = ok ? rbl[q] : zero8;
           }
         }
       };
