@@ -92,7 +92,9 @@ def main():
                          "inside graphs are not timed individually, so the roofline then covers the "
                          "environment-update GEMMs only)")
     ap.add_argument("--no-autotune", action="store_true", help="fixed tile-shape heuristic instead of the autotuner")
-    ap.add_argument("--no-xcd-order", action="store_true", help="plain descending-work tile order")
+    ap.add_argument("--tile-order", type=int, default=None, choices=[0, 1],
+                    help="grouped-GEMM tile order (dmrj_ctx_set_tile_order; default 1 = tiles sharing a panel "
+                         "dealt to one XCD, 0 = plain descending work)")
     ap.add_argument("--eig-batching", type=int, default=None, choices=[0, 1, 2],
                     help="SVD eigensolve batching (dmrj_ctx_set_eig_batching; default 1 = size classes)")
     ap.add_argument("--gemm-cfg", type=int, default=-1,
@@ -143,8 +145,8 @@ def main():
         return
     if args.no_autotune:
         ctx.set_autotune(0)
-    if args.no_xcd_order:
-        ctx.set_tile_order(0)
+    if args.tile_order is not None:
+        ctx.set_tile_order(args.tile_order)
     if args.gemm_cfg >= 0:
         ctx.set_gemm_cfg(args.gemm_cfg)
     if args.eig_batching is not None:

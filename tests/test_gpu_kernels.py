@@ -76,6 +76,34 @@ def test_gemm_shapes_same_bits(pkg, ctx, tA, tB, M, N, K):
         assert np.array_equal(out[0], out[cfg]), cfg
 
 
+@pytest.mark.parametrize("cfg,M,N,K,prec", [(0, 1600, 1600, 40, 0), (1, 2100, 2100, 24, 0), (0, 1600, 1600, 40, 2),
+                                            (1, 2100, 2100, 24, 1)])
+def test_gemm_tile_orders_more_tiles_than_workgroups(pkg, ctx, cfg, M, N, K, prec):
+    """More tiles than resident work-groups (several tiles per work-group of
+    the persistent grid): both tile orders (0 descending work, 1 XCD streams)
+    run every tile exactly once and give the same bits, and the result
+    matches numpy."""
+    say("tile orders cfg", cfg, "MNK", M, N, K, "precision", prec)
+    rng = np.random.default_rng(M + K)
+    A = np.asfortranarray(rng.standard_normal((M, K)))
+    B = np.asfortranarray(rng.standard_normal((K, N)))
+    out = []
+    try:
+        for order in (0, 1):
+            ctx.set_tile_order(order)
+            C = np.zeros((M, N), order="F")
+            ctx.call("dmrj_selftest_gemm", cfg | (prec << 4), M, N, K, 0, 0, pkg._lib.dptr(A.ravel(order="F")),
+                     pkg._lib.dptr(B.ravel(order="F")), C.ctypes.data_as(pkg._lib._dp))
+            out.append(C)
+    finally:
+        ctx.set_tile_order(1)
+    want = A @ B
+    # FP64: rounding level; split forms: their error bounds relative to |A||B|
+    tol = {0: 1e-13, 1: 6e-5, 2: 5e-6}[prec]
+    assert np.max(np.abs(out[0] - want) / (np.abs(A) @ np.abs(B))) <= tol
+    assert np.array_equal(out[0], out[1])
+
+
 @pytest.mark.parametrize("backend", [0, pytest.param(1, marks=pytest.mark.xfail(
     reason="rocSOLVER gesdd loses the small singular values of graded spectra; kept only as a "
            "selectable backend, never the default", strict=False)), 2, 3, 4, 5])
