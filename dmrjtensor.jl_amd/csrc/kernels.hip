@@ -46,8 +46,11 @@ __device__ __forceinline__ double ld_global(const double* p) {
   return *(const __attribute__((address_space(1))) double*)p;
 }
 
+// The 32x32 shape is held to 64 registers per lane (eight waves per SIMD,
+// eight work-groups per CU as its LDS allows) without scratch; the larger
+// shapes keep the compiler's allocation.
 template <int BM, int BN>
-__global__ __launch_bounds__(kThreads) void k_gemm_f64(const GemmTile* __restrict__ tiles,
+__global__ __launch_bounds__(kThreads, (BM * BN <= 1024 ? 8 : 1)) void k_gemm_f64(const GemmTile* __restrict__ tiles,
                                                        int ntiles,
                                                        const GemmTask* __restrict__ tasks,
                                                        const GemmSeg* __restrict__ segs) {
@@ -77,28 +80,47 @@ __global__ __launch_bounds__(kThreads) void k_gemm_f64(const GemmTile* __restric
       const bool tA = sg.flags & 1, tB = sg.flags & 2;
       const int K = sg.K;
       double ra[AE], rb[BE];
+      // Per-thread element offsets of the first K step (k0 = 0); a K step adds
+      // k0 * (stride along k).  A row (column) outside the task reads the
+      // task's first row (column) instead and is written to LDS as zero, so
+      // a K step that lies wholly inside K loads without per-element checks.
+      const int64_t ska = tA ? 1 : sg.lda, skb = tB ? sg.ldb : 1;
+      int64_t oa[AE], ob[BE];
+      int ka[AE], kb[BE];
+      bool va[AE], vb[BE];
+#pragma unroll
+      for (int r = 0; r < AE; ++r) {
+        const int idx = tid + r * kThreads;
+        int m, k;
+        if (!tA) { m = idx % BM; k = idx / BM; } else { k = idx % kBK; m = idx / kBK; }
+        va[r] = m0 + m < tk.M;
+        const int gm = va[r] ? m0 + m : m0;
+        ka[r] = k;
+        oa[r] = tA ? (int64_t)k + (int64_t)gm * sg.lda : (int64_t)gm + (int64_t)k * sg.lda;
+      }
+#pragma unroll
+      for (int r = 0; r < BE; ++r) {
+        const int idx = tid + r * kThreads;
+        int n, k;
+        if (!tB) { k = idx % kBK; n = idx / kBK; } else { n = idx % BN; k = idx / BN; }
+        vb[r] = n0 + n < tk.N;
+        const int gn = vb[r] ? n0 + n : n0;
+        kb[r] = k;
+        ob[r] = tB ? (int64_t)gn + (int64_t)k * sg.ldb : (int64_t)k + (int64_t)gn * sg.ldb;
+      }
       auto load = [&](int k0) {
+        const double* pa = sg.A + (int64_t)k0 * ska;
+        const double* pb = sg.B + (int64_t)k0 * skb;
+        if (k0 + kBK <= K) {
 #pragma unroll
-        for (int r = 0; r < AE; ++r) {
-          const int idx = tid + r * kThreads;
-          int m, k;
-          if (!tA) { m = idx % BM; k = idx / BM; } else { k = idx % kBK; m = idx / kBK; }
-          const int gm = m0 + m, gk = k0 + k;
-          double v = 0.0;
-          if (gm < tk.M && gk < K)
-            v = ld_global(tA ? sg.A + ((int64_t)gk + (int64_t)gm * sg.lda) : sg.A + ((int64_t)gm + (int64_t)gk * sg.lda));
-          ra[r] = v;
-        }
+          for (int r = 0; r < AE; ++r) ra[r] = ld_global(pa + oa[r]);
 #pragma unroll
-        for (int r = 0; r < BE; ++r) {
-          const int idx = tid + r * kThreads;
-          int n, k;
-          if (!tB) { k = idx % kBK; n = idx / kBK; } else { n = idx % BN; k = idx / BN; }
-          const int gn = n0 + n, gk = k0 + k;
-          double v = 0.0;
-          if (gn < tk.N && gk < K)
-            v = ld_global(tB ? sg.B + ((int64_t)gn + (int64_t)gk * sg.ldb) : sg.B + ((int64_t)gk + (int64_t)gn * sg.ldb));
-          rb[r] = v;
+          for (int r = 0; r < BE; ++r) rb[r] = ld_global(pb + ob[r]);
+        } else {
+#pragma unroll
+          for (int r = 0; r < AE; ++r) ra[r] = k0 + ka[r] < K ? ld_global(pa + oa[r]) : 0.0;
+#pragma unroll
+          for (int r = 0; r < BE; ++r) rb[r] = k0 + kb[r] < K ? ld_global(pb + ob[r]) : 0.0;
         }
       };
       // alpha is applied when the prefetched values are written to LDS, after
@@ -109,14 +131,14 @@ __global__ __launch_bounds__(kThreads) void k_gemm_f64(const GemmTile* __restric
           const int idx = tid + r * kThreads;
           int m, k;
           if (!tA) { m = idx % BM; k = idx / BM; } else { k = idx % kBK; m = idx / kBK; }
-          sA[b][m * kPitch + k] = ra[r] * sg.alpha;
+          sA[b][m * kPitch + k] = va[r] ? ra[r] * sg.alpha : 0.0;
         }
 #pragma unroll
         for (int r = 0; r < BE; ++r) {
           const int idx = tid + r * kThreads;
           int n, k;
           if (!tB) { k = idx % kBK; n = idx / kBK; } else { n = idx % BN; k = idx / BN; }
-          sB[b][n * kPitch + k] = rb[r];
+          sB[b][n * kPitch + k] = vb[r] ? rb[r] : 0.0;
         }
       };
       if (K <= 0) continue;

@@ -479,24 +479,32 @@ __global__ __launch_bounds__(kLpThreads) void k_gemm_f16p(const GemmTile* __rest
       // K step stays in flight across the current step's products.
       typedef __attribute__((address_space(1))) const half8_t ghalf8_t;
       half8_t rah[ACH], ral[ACH], rbh[BCH], rbl[BCH];
+      // chunk offsets of the first K step; a K step adds k0
+      int64_t oa[ACH], ob[BCH];
+#pragma unroll
+      for (int q = 0; q < ACH; ++q) {
+        const int c = tid + q * kLpThreads;
+        const int row = m0 + c / 4;
+        const bool ok = c < BM * 4 && row < tk.M;
+        oa[q] = (int64_t)(ok ? row : 0) * fs.kpad + (c % 4) * 8;
+      }
+#pragma unroll
+      for (int q = 0; q < BCH; ++q) {
+        const int c = tid + q * kLpThreads;
+        const int col = n0 + c / 4;
+        const bool ok = c < BN * 4 && col < tk.N;
+        ob[q] = (int64_t)(ok ? col : 0) * fs.kpad + (c % 4) * 8;
+      }
       auto load = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < ACH; ++q) {
-          const int c = tid + q * kLpThreads;
-          const int row = m0 + c / 4;
-          const bool ok = c < BM * 4 && row < tk.M;
-          const int64_t o = (int64_t)(ok ? row : 0) * fs.kpad + k0 + (c % 4) * 8;
-          rah[q] = *(ghalf8_t*)(fs.ah + o);
-          ral[q] = *(ghalf8_t*)(fs.al + o);
+          rah[q] = *(ghalf8_t*)(fs.ah + k0 + oa[q]);
+          ral[q] = *(ghalf8_t*)(fs.al + k0 + oa[q]);
         }
 #pragma unroll
         for (int q = 0; q < BCH; ++q) {
-          const int c = tid + q * kLpThreads;
-          const int col = n0 + c / 4;
-          const bool ok = c < BN * 4 && col < tk.N;
-          const int64_t o = (int64_t)(ok ? col : 0) * fs.kpad + k0 + (c % 4) * 8;
-          rbh[q] = *(ghalf8_t*)(fs.bh + o);
-          rbl[q] = *(ghalf8_t*)(fs.bl + o);
+          rbh[q] = *(ghalf8_t*)(fs.bh + k0 + ob[q]);
+          rbl[q] = *(ghalf8_t*)(fs.bl + k0 + ob[q]);
         }
       };
       auto store = [&](int b) {
