@@ -15,10 +15,10 @@ HIP-event timing.
     python rocprof_summary.py mfma COUNTERS.csv [--kernel k_gemm_f64] [--json OUT]
 
 ``trace`` prints per-kernel launch counts and durations (whole run and the
-marked region).  ``mfma`` reads a run with ``--pmc MfmaUtil MfmaFlopsF64``
+marked region).  ``mfma`` reads a run with ``--pmc MfmaUtil MfmaFlopsF64 [MfmaFlopsF16]``
 (rocprofv3's derived metrics: MFMA-busy cycles over GUI-active cycles of all
-SIMDs, and executed FP64 MFMA flops) and prints the duration-weighted MFMA
-utilisation and the executed MFMA flops per launch of one kernel.  ``traffic`` prints the HBM bytes per launch of one kernel in
+SIMDs, and executed FP64 / FP16 MFMA flops) and prints the duration-weighted
+MFMA utilisation and the executed MFMA flops per launch of one kernel.  ``traffic`` prints the HBM bytes per launch of one kernel in
 the marked region from FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each
 scaled by what the same counter reads for the copy kernel of the calibration
 run, which moves exactly 8*N bytes each way.
@@ -168,7 +168,8 @@ def mfma_summary(path, kernel, json_out=None):
     cn, cd = _col(hdr, "Kernel_Name"), _col(hdr, "Dispatch_Id", "Correlation_Id")
     cc, cv = _col(hdr, "Counter_Name"), _col(hdr, "Counter_Value")
     cs, ce = _col(hdr, "Start_Timestamp"), _col(hdr, "End_Timestamp")
-    util, flops, dur, names, markers = {}, {}, {}, {}, {}
+    util, dur, names, markers = {}, {}, {}, {}
+    flops = {"MfmaFlopsF64": {}, "MfmaFlopsF16": {}}
     for r in rows:
         i = int(r[cd])
         names[i] = r[cn]
@@ -177,24 +178,28 @@ def mfma_summary(path, kernel, json_out=None):
             markers[i] = _marker_id(r, hdr)
         if r[cc] == "MfmaUtil":
             util[i] = util.get(i, 0.0) + float(r[cv])
-        elif r[cc] == "MfmaFlopsF64":
-            flops[i] = flops.get(i, 0.0) + float(r[cv])
+        elif r[cc] in flops:
+            flops[r[cc]][i] = flops[r[cc]].get(i, 0.0) + float(r[cv])
     inside = _selector(sorted(markers.items()))
     if inside is None:
-        raise SystemExit("%s: no %s dispatches with ids 1 and 2" % (path, MARKER))
+        raise SystemExit("%s: no %s dispatches with ids %d and %d" % (path, MARKER, REGION[0], REGION[1]))
     sel = [i for i in util if kernel in names[i] and inside(i)]
     if not sel:
         raise SystemExit("%s: no %s dispatches with MfmaUtil" % (path, kernel))
     tdur = sum(dur[i] for i in sel)
     wutil = sum(util[i] * dur[i] for i in sel) / max(1, tdur)
-    fl = sum(flops.get(i, 0.0) for i in sel)
+    per = {k: sum(v.get(i, 0.0) for i in sel) for k, v in flops.items()}
     print("%s in the marked region: %d launches, MFMA busy %.1f %% of all SIMD cycles (duration-weighted), "
-          "executed FP64 MFMA flops %.4g per launch, %.2f TFLOP/s over the counted kernel time"
-          % (kernel, len(sel), wutil, fl / len(sel), fl / max(1, tdur) / 1e3))
+          "executed FP64 MFMA flops %.4g and FP16 MFMA flops %.4g per launch, %.2f / %.2f TFLOP/s over the "
+          "counted kernel time"
+          % (kernel, len(sel), wutil, per["MfmaFlopsF64"] / len(sel), per["MfmaFlopsF16"] / len(sel),
+             per["MfmaFlopsF64"] / max(1, tdur) / 1e3, per["MfmaFlopsF16"] / max(1, tdur) / 1e3))
     if json_out:
         with open(json_out, "w") as fo:
             json.dump({"kernel": kernel, "launches": len(sel), "mfma_util_pct": wutil,
-                       "mfma_flops_f64_per_launch": fl / len(sel), "kernel_ns_total": tdur}, fo, indent=1)
+                       "mfma_flops_f64_per_launch": per["MfmaFlopsF64"] / len(sel),
+                       "mfma_flops_f16_per_launch": per["MfmaFlopsF16"] / len(sel),
+                       "kernel_ns_total": tdur}, fo, indent=1)
 
 
 def main(argv=None):

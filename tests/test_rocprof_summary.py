@@ -111,6 +111,7 @@ def test_mfma_duration_weighted(tmp_path):
             elif "k_gemm_f64" in n:
                 w.writerow([i, 64 * wg, n, 64, "MfmaUtil", util.get(i, 99.0), t, t + d])
                 w.writerow([i, 64 * wg, n, 64, "MfmaFlopsF64", 1e9, t, t + d])
+                w.writerow([i, 64 * wg, n, 64, "MfmaFlopsF16", 3e9 if i == 3 else 0.0, t, t + d])
             t += d + 10
     buf = io.StringIO()
     with redirect_stdout(buf):
@@ -120,3 +121,4 @@ def test_mfma_duration_weighted(tmp_path):
     assert d["launches"] == 2
     assert abs(d["mfma_util_pct"] - (40.0 * 50000 + 10.0 * 70000) / 120000) < 1e-12
     assert d["mfma_flops_f64_per_launch"] == 1e9
+    assert d["mfma_flops_f16_per_launch"] == 1.5e9
