@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <map>
 
 #include "dmrj.h"
@@ -42,8 +43,11 @@ static int occupancy(Context& c, int cfg, int arith) {
 // (the same pointer, leading dimension, shape, storage order and alpha), its
 // planes at 256-byte aligned offsets of one transient buffer, the split work
 // items and every segment's plane pointers.
-static void prepare_f16_planes(Context& c, GemmLaunch& g, const std::vector<GemmTask>& tasks,
-                               const std::vector<GemmSeg>& segs) {
+struct F16PlaneOffsets {
+  size_t regionmaxwork = 0, regions = 0, splitwork = 0, f16segs = 0;
+};
+static F16PlaneOffsets prepare_f16_planes(Context& c, GemmLaunch& g, const std::vector<GemmTask>& tasks,
+                                          const std::vector<GemmSeg>& segs, UploadPack& up) {
   struct Key {
     const double* p;
     int64_t ld;
@@ -117,12 +121,14 @@ static void prepare_f16_planes(Context& c, GemmLaunch& g, const std::vector<Gemm
   g.nregions = (int)regs.size();
   g.regionmax = (double*)c.alloc.alloc(std::max<size_t>(1, regs.size()) * sizeof(double));
   c.transient.push_back(g.regionmax);
-  g.regionmaxwork = mwork.empty() ? nullptr : (const int4*)c.upload(mwork.data(), mwork.size() * sizeof(int4));
+  F16PlaneOffsets o;
+  o.regionmaxwork = up.add(mwork.data(), mwork.size() * sizeof(int4));
   g.nregionmaxwork = (int)mwork.size();
-  g.regions = regs.empty() ? nullptr : (const F16Region*)c.upload(regs.data(), regs.size() * sizeof(F16Region));
-  g.splitwork = work.empty() ? nullptr : (const int4*)c.upload(work.data(), work.size() * sizeof(int4));
+  o.regions = up.add(regs.data(), regs.size() * sizeof(F16Region));
+  o.splitwork = up.add(work.data(), work.size() * sizeof(int4));
   g.nsplitwork = (int)work.size();
-  g.f16segs = (const F16Seg*)c.upload(fs.data(), fs.size() * sizeof(F16Seg));
+  o.f16segs = up.add(fs.data(), fs.size() * sizeof(F16Seg));
+  return o;
 }
 
 // XCD-aware order.  Work-group b of a launch runs on XCD b mod 8 (the
@@ -195,9 +201,10 @@ GemmLaunch prepare_gemm(Context& c, const std::vector<GemmTask>& tasks, const st
   }
   if (tiles.empty()) return g;
   if (c.xcd_order) tiles = xcd_interleave(tiles, tasks, segs);
-  g.tiles = (const GemmTile*)c.upload(tiles.data(), tiles.size() * sizeof(GemmTile));
-  g.tasks = (const GemmTask*)c.upload(tasks.data(), tasks.size() * sizeof(GemmTask));
-  g.segs = (const GemmSeg*)c.upload(segs.data(), std::max<size_t>(1, segs.size()) * sizeof(GemmSeg));
+  UploadPack up;
+  const size_t o_tiles = up.add(tiles.data(), tiles.size() * sizeof(GemmTile));
+  const size_t o_tasks = up.add(tasks.data(), tasks.size() * sizeof(GemmTask));
+  const size_t o_segs = up.add(segs.data(), segs.size() * sizeof(GemmSeg));
   g.ntiles = (int)tiles.size();
   g.nseg = (int)segs.size();
   // persistent grid: never more work-groups than can be resident at once
@@ -224,16 +231,36 @@ GemmLaunch prepare_gemm(Context& c, const std::vector<GemmTask>& tasks, const st
         for (int64_t c0 = 0; c0 < cb; c0 += stepb) mw.push_back(make_int4(si, 1, (int)c0, (int)std::min(cb, c0 + stepb)));
       }
     }
-    g.segtask = (const int*)c.upload(st.data(), st.size() * sizeof(int));
+    const size_t o_segtask = up.add(st.data(), st.size() * sizeof(int));
     g.segmax = (double*)c.alloc.alloc(std::max<size_t>(2, 2 * segs.size()) * sizeof(double));
     c.transient.push_back(g.segmax);  // lives until the next synchronisation point
+    F16PlaneOffsets po;
+    size_t o_maxwork = 0;
     if (c.f16_presplit) {
-      prepare_f16_planes(c, g, tasks, segs);  // maxima per distinct operand block instead
+      po = prepare_f16_planes(c, g, tasks, segs, up);  // maxima per distinct operand block instead
     } else {
-      if (!mw.empty()) g.maxwork = (const int4*)c.upload(mw.data(), mw.size() * sizeof(int4));
+      o_maxwork = up.add(mw.data(), mw.size() * sizeof(int4));
       g.nmaxwork = (int)mw.size();
     }
+    char* base = up.commit(c);
+    g.segtask = (const int*)(base + o_segtask);
+    if (g.presplit) {
+      g.regionmaxwork = g.nregionmaxwork ? (const int4*)(base + po.regionmaxwork) : nullptr;
+      g.regions = g.nregions ? (const F16Region*)(base + po.regions) : nullptr;
+      g.splitwork = g.nsplitwork ? (const int4*)(base + po.splitwork) : nullptr;
+      g.f16segs = (const F16Seg*)(base + po.f16segs);
+    } else if (g.nmaxwork) {
+      g.maxwork = (const int4*)(base + o_maxwork);
+    }
+    g.tiles = (const GemmTile*)(base + o_tiles);
+    g.tasks = (const GemmTask*)(base + o_tasks);
+    g.segs = (const GemmSeg*)(base + o_segs);
+    return g;
   }
+  char* base = up.commit(c);
+  g.tiles = (const GemmTile*)(base + o_tiles);
+  g.tasks = (const GemmTask*)(base + o_tasks);
+  g.segs = (const GemmSeg*)(base + o_segs);
   return g;
 }
 

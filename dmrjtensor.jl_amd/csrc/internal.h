@@ -17,7 +17,9 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <cstring>
 #include <map>
 #include <unordered_map>
 #include <memory>
@@ -551,6 +553,22 @@ enum Slot {
   S_A2 = 14, S_E = 15, S_CA = 16, S_CB = 17, S_INVOC = 18, S_CHK = 19,
   S_RR = 20, S_RH = 21, S_NEGE = 22, S_RES = 23, S_ELP = 24,
   S_COUNT = 32
+};
+
+// The host arrays of one launch (tile list, tasks, segments, work lists,
+// FP16 plane tables) packed into one blob, each at a 256-byte aligned offset,
+// so that they reach the device in a single staged copy instead of one per
+// array.  add() returns an array's byte offset in the blob; commit() uploads
+// the blob once and returns its device address.
+struct UploadPack {
+  std::vector<char> blob;
+  size_t add(const void* p, size_t bytes) {
+    const size_t o = blob.size();
+    blob.resize(o + std::max<size_t>(256, (bytes + 255) / 256 * 256), 0);
+    if (p && bytes) std::memcpy(blob.data() + o, p, bytes);
+    return o;
+  }
+  char* commit(Context& c) { return (char*)c.upload(blob.data(), blob.size()); }
 };
 
 }  // namespace dmrj

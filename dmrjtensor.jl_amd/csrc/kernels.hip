@@ -840,9 +840,14 @@ MixLaunch prepare_mix(Context& c, const std::vector<MixDst>& dsts, const std::ve
   for (size_t i = 0; i < dsts.size(); ++i)
     for (int64_t e = 0; e < dsts[i].n; e += kMixChunk) work.push_back({(int)i, 0, e});
   if (work.empty()) return m;
-  m.work = (const MixWork*)c.upload(work.data(), work.size() * sizeof(MixWork));
-  m.dsts = (const MixDst*)c.upload(dsts.data(), dsts.size() * sizeof(MixDst));
-  m.terms = (const MixTerm*)c.upload(terms.data(), std::max<size_t>(1, terms.size()) * sizeof(MixTerm));
+  UploadPack up;
+  const size_t ow = up.add(work.data(), work.size() * sizeof(MixWork));
+  const size_t od = up.add(dsts.data(), dsts.size() * sizeof(MixDst));
+  const size_t ot = up.add(terms.data(), terms.size() * sizeof(MixTerm));
+  char* base = up.commit(c);
+  m.work = (const MixWork*)(base + ow);
+  m.dsts = (const MixDst*)(base + od);
+  m.terms = (const MixTerm*)(base + ot);
   m.nwork = (int)work.size();
   m.grid = (int)std::min<size_t>(work.size(), (size_t)c.num_cu * 8);
   for (const auto& d : dsts) {
@@ -887,8 +892,12 @@ void launch_copy(Context& c, const std::vector<CopyOp>& ops) {
     for (int64_t e = 0; e < n; e += kMixChunk) work.push_back({(int)i, 0, e});
   }
   if (work.empty()) return;
-  auto* dw = (CopyWork*)c.upload(work.data(), work.size() * sizeof(CopyWork));
-  auto* dops = (CopyOp*)c.upload(ops.data(), ops.size() * sizeof(CopyOp));
+  UploadPack up;
+  const size_t ow = up.add(work.data(), work.size() * sizeof(CopyWork));
+  const size_t oo = up.add(ops.data(), ops.size() * sizeof(CopyOp));
+  char* base = up.commit(c);
+  auto* dw = (CopyWork*)(base + ow);
+  auto* dops = (CopyOp*)(base + oo);
   const int grid = (int)std::min<size_t>(work.size(), (size_t)c.num_cu * 8);
   const int ps = c.prof_begin();
   hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kThreads), 0, c.stream, dw, (int)work.size(), dops);
