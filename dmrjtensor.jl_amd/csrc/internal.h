@@ -555,6 +555,14 @@ enum Slot {
   S_COUNT = 32
 };
 
+// Load through a global-address-space pointer: a global_load counts only
+// against vmcnt, so the prefetch of a GEMM's next K step stays in flight
+// across the LDS reads and MFMAs of the current one (a generic flat_load would
+// also hold every lgkmcnt wait of those LDS reads).
+__device__ __forceinline__ double ld_global(const double* p) {
+  return *(const __attribute__((address_space(1))) double*)p;
+}
+
 // The host arrays of one launch (tile list, tasks, segments, work lists,
 // FP16 plane tables) packed into one blob, each at a 256-byte aligned offset,
 // so that they reach the device in a single staged copy instead of one per

@@ -38,14 +38,6 @@ constexpr int kBK = 16;
 // each touch 16 distinct bank pairs.
 constexpr int kPitch = kBK + 1;
 
-// Load through a global-address-space pointer: a global_load counts only
-// against vmcnt, so the prefetch of the next K step stays in flight across the
-// LDS reads and MFMAs of the current one (a generic flat_load would also hold
-// every lgkmcnt wait of those LDS reads).
-__device__ __forceinline__ double ld_global(const double* p) {
-  return *(const __attribute__((address_space(1))) double*)p;
-}
-
 // The 32x32 shape is held to 64 registers per lane (eight waves per SIMD,
 // eight work-groups per CU as its LDS allows) without scratch; the larger
 // shapes keep the compiler's allocation.

@@ -78,8 +78,8 @@ __global__ __launch_bounds__(kLpThreads) void k_gemm_bf16x3(const GemmTile* __re
           const int gm = m0 + m, gk = k0 + k;
           double v = 0.0;
           if (gm < tk.M && gk < K)
-            v = tA ? sg.A[(int64_t)gk + (int64_t)gm * sg.lda] : sg.A[(int64_t)gm + (int64_t)gk * sg.lda];
-          ra[r] = v * sg.alpha;
+            v = ld_global(tA ? sg.A + ((int64_t)gk + (int64_t)gm * sg.lda) : sg.A + ((int64_t)gm + (int64_t)gk * sg.lda));
+          ra[r] = v;
         }
 #pragma unroll
         for (int r = 0; r < BE; ++r) {
@@ -89,7 +89,7 @@ __global__ __launch_bounds__(kLpThreads) void k_gemm_bf16x3(const GemmTile* __re
           const int gn = n0 + n, gk = k0 + k;
           double v = 0.0;
           if (gn < tk.N && gk < K)
-            v = tB ? sg.B[(int64_t)gn + (int64_t)gk * sg.ldb] : sg.B[(int64_t)gk + (int64_t)gn * sg.ldb];
+            v = ld_global(tB ? sg.B + ((int64_t)gn + (int64_t)gk * sg.ldb) : sg.B + ((int64_t)gk + (int64_t)gn * sg.ldb));
           rb[r] = v;
         }
       };
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(kLpThreads) void k_gemm_bf16x3(const GemmTile* __re
           int m, k;
           if (!tA) { m = idx % BM; k = idx / BM; } else { k = idx % kLpBK; m = idx / kLpBK; }
           __bf16 h, l;
-          split_bf16(ra[r], h, l);
+          split_bf16(ra[r] * sg.alpha, h, l);  // scaled here, not at the load: the loads stay in flight
           sAh[b][m * kLpPitch + k] = h;
           sAl[b][m * kLpPitch + k] = l;
         }
@@ -262,8 +262,8 @@ __global__ __launch_bounds__(kLpThreads) void k_gemm_f16x3(const GemmTile* __res
           const int gm = m0 + m, gk = k0 + k;
           double v = 0.0;
           if (gm < tk.M && gk < K)
-            v = tA ? sg.A[(int64_t)gk + (int64_t)gm * sg.lda] : sg.A[(int64_t)gm + (int64_t)gk * sg.lda];
-          ra[r] = v * fa;
+            v = ld_global(tA ? sg.A + ((int64_t)gk + (int64_t)gm * sg.lda) : sg.A + ((int64_t)gm + (int64_t)gk * sg.lda));
+          ra[r] = v;
         }
 #pragma unroll
         for (int r = 0; r < BE; ++r) {
@@ -273,8 +273,8 @@ __global__ __launch_bounds__(kLpThreads) void k_gemm_f16x3(const GemmTile* __res
           const int gn = n0 + n, gk = k0 + k;
           double v = 0.0;
           if (gn < tk.N && gk < K)
-            v = tB ? sg.B[(int64_t)gn + (int64_t)gk * sg.ldb] : sg.B[(int64_t)gk + (int64_t)gn * sg.ldb];
-          rb[r] = v * fb;
+            v = ld_global(tB ? sg.B + ((int64_t)gn + (int64_t)gk * sg.ldb) : sg.B + ((int64_t)gk + (int64_t)gn * sg.ldb));
+          rb[r] = v;
         }
       };
       auto store = [&](int b) {
@@ -284,7 +284,7 @@ __global__ __launch_bounds__(kLpThreads) void k_gemm_f16x3(const GemmTile* __res
           int m, k;
           if (!tA) { m = idx % BM; k = idx / BM; } else { k = idx % kLpBK; m = idx / kLpBK; }
           _Float16 h, l;
-          split_f16(ra[r], h, l);
+          split_f16(ra[r] * fa, h, l);  // scaled here, not at the load: the loads stay in flight
           sAh[b][m * kLpPitch + k] = h;
           sAl[b][m * kLpPitch + k] = l;
         }
@@ -294,7 +294,7 @@ __global__ __launch_bounds__(kLpThreads) void k_gemm_f16x3(const GemmTile* __res
           int n, k;
           if (!tB) { k = idx % kLpBK; n = idx / kLpBK; } else { n = idx % BN; k = idx / BN; }
           _Float16 h, l;
-          split_f16(rb[r], h, l);
+          split_f16(rb[r] * fb, h, l);
           sBh[b][n * kLpPitch + k] = h;
           sBl[b][n * kLpPitch + k] = l;
         }
