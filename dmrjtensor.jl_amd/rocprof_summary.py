@@ -13,12 +13,15 @@ HIP-event timing.
     python rocprof_summary.py traffic --fetch F.csv --write W.csv \
         --cal-fetch CF.csv --cal-write CW.csv --cal-doubles N [--kernel k_gemm_f64] [--json OUT]
     python rocprof_summary.py mfma COUNTERS.csv [--kernel k_gemm_f64] [--json OUT]
+    python rocprof_summary.py counters COUNTERS.csv [--kernel k_gemm_f64] [--json OUT]
 
 ``trace`` prints per-kernel launch counts and durations (whole run and the
 marked region).  ``mfma`` reads a run with ``--pmc MfmaUtil MfmaFlopsF64 [MfmaFlopsF16]``
 (rocprofv3's derived metrics: MFMA-busy cycles over GUI-active cycles of all
 SIMDs, and executed FP64 / FP16 MFMA flops) and prints the duration-weighted
-MFMA utilisation and the executed MFMA flops per launch of one kernel.  ``traffic`` prints the HBM bytes per launch of one kernel in
+MFMA utilisation and the executed MFMA flops per launch of one kernel.  ``counters``
+prints the duration-weighted mean of every counter or derived metric of one
+kernel over the marked region.  ``traffic`` prints the HBM bytes per launch of one kernel in
 the marked region from FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each
 scaled by what the same counter reads for the copy kernel of the calibration
 run, which moves exactly 8*N bytes each way.
@@ -202,6 +205,41 @@ def mfma_summary(path, kernel, json_out=None):
                        "kernel_ns_total": tdur}, fo, indent=1)
 
 
+def counters_summary(path, kernel, json_out=None):
+    """Duration-weighted mean of every counter (or derived metric) of one kernel
+    over its dispatches in the marked region."""
+    hdr, rows = _rows(path)
+    cn, cd = _col(hdr, "Kernel_Name"), _col(hdr, "Dispatch_Id", "Correlation_Id")
+    cc, cv = _col(hdr, "Counter_Name"), _col(hdr, "Counter_Value")
+    cs, ce = _col(hdr, "Start_Timestamp"), _col(hdr, "End_Timestamp")
+    vals, dur, names, markers = {}, {}, {}, {}
+    for r in rows:
+        i = int(r[cd])
+        names[i] = r[cn]
+        dur[i] = int(r[ce]) - int(r[cs])
+        if MARKER in r[cn]:
+            markers[i] = _marker_id(r, hdr)
+        vals.setdefault(r[cc], {})
+        vals[r[cc]][i] = vals[r[cc]].get(i, 0.0) + float(r[cv])
+    inside = _selector(sorted(markers.items()))
+    if inside is None:
+        raise SystemExit("%s: no %s dispatches with ids %d and %d" % (path, MARKER, REGION[0], REGION[1]))
+    out = {"kernel": kernel}
+    for name in sorted(vals):
+        sel = [i for i in vals[name] if kernel in names[i] and inside(i)]
+        if not sel:
+            continue
+        tdur = sum(dur[i] for i in sel)
+        mean = sum(vals[name][i] * dur[i] for i in sel) / max(1, tdur)
+        out[name] = mean
+        out["launches"] = len(sel)
+        print("%s in the marked region: %s = %.4g (duration-weighted mean over %d launches)"
+              % (kernel, name, mean, len(sel)))
+    if json_out:
+        with open(json_out, "w") as fo:
+            json.dump(out, fo, indent=1)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -221,7 +259,11 @@ def main(argv=None):
     mm.add_argument("csv")
     mm.add_argument("--kernel", default="k_gemm_f64")
     mm.add_argument("--json")
-    for sp in (t, f, mm):
+    cc = sub.add_parser("counters")
+    cc.add_argument("csv")
+    cc.add_argument("--kernel", default="k_gemm_f64")
+    cc.add_argument("--json")
+    for sp in (t, f, mm, cc):
         sp.add_argument("--region", type=int, nargs=2, default=[1, 2], metavar=("OPEN", "CLOSE"),
                         help="k_region_marker ids around the region (1 2: the timed sweeps; the headline "
                              "benchmark's profiled FP64 sweep is 5 6, its profiled mixed sweep 7 8)")
@@ -232,6 +274,8 @@ def main(argv=None):
         trace_summary(a.csv, a.kernel)
     elif a.cmd == "mfma":
         mfma_summary(a.csv, a.kernel, a.json)
+    elif a.cmd == "counters":
+        counters_summary(a.csv, a.kernel, a.json)
     else:
         traffic_summary(a)
 

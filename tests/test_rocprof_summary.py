@@ -122,3 +122,28 @@ def test_mfma_duration_weighted(tmp_path):
     assert abs(d["mfma_util_pct"] - (40.0 * 50000 + 10.0 * 70000) / 120000) < 1e-12
     assert d["mfma_flops_f64_per_launch"] == 1e9
     assert d["mfma_flops_f16_per_launch"] == 1.5e9
+
+
+def test_counters_duration_weighted(tmp_path):
+    m = _mod()
+    p = str(tmp_path / "c.csv")
+    with open(p, "w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(["Dispatch_Id", "Grid_Size", "Kernel_Name", "Workgroup_Size", "Counter_Name", "Counter_Value",
+                    "Start_Timestamp", "End_Timestamp"])
+        t = 0
+        for i, (n, wg, d) in enumerate(TRACE, start=1):
+            if n == MARK:
+                w.writerow([i, 64 * wg, n, 64, "VALUBusy", 0, t, t + d])
+            elif "k_gemm_f64" in n:
+                w.writerow([i, 64 * wg, n, 64, "VALUBusy", 20.0 if i == 3 else 60.0, t, t + d])
+                w.writerow([i, 64 * wg, n, 64, "MemUnitStalled", 5.0, t, t + d])
+            t += d + 10
+    buf = io.StringIO()
+    with redirect_stdout(buf):
+        m.counters_summary(p, "k_gemm_f64", str(tmp_path / "c.json"))
+    d = json.load(open(str(tmp_path / "c.json")))
+    # region launches 3 (50000 ns) and 8 (70000 ns)
+    assert d["launches"] == 2
+    assert abs(d["VALUBusy"] - (20.0 * 50000 + 60.0 * 70000) / 120000) < 1e-12
+    assert d["MemUnitStalled"] == 5.0
